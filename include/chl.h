@@ -1,0 +1,190 @@
+/* C ABI of the MI355X canonical hub labeling (CHL) engine, libchl.so.
+ *
+ * The reference (qingdo/plant) has no FFI: its boundary is a C++ source-level API
+ * plus the `hhl` command line. Each entry point below names the reference
+ * interface it replaces. Conventions: opaque handles; every function returns an
+ * int status (CHL_OK = 0, negative CHL_E_* on error) unless it is a pure getter;
+ * no C++ exceptions cross this boundary; chl_last_error() gives a message for the
+ * most recent failure on the calling thread.
+ *
+ * Types follow shared_memory/graph.hpp:42-47: vertex ids and hub ranks are
+ * uint32, distances int32, and CHL_INFTY (INT_MAX) is the "no common hub" result.
+ * Sides follow the reference: side 0 of v holds (hub rank h, d(order[h] -> v)),
+ * side 1 holds (h, d(v -> order[h])) (shared_memory/akiba.hpp:112,116).
+ */
+#ifndef CHL_H_
+#define CHL_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHL_INFTY 2147483647
+
+enum {
+  CHL_OK = 0,
+  CHL_E_INVAL = -1,   /* bad argument */
+  CHL_E_IO = -2,      /* file cannot be opened or written */
+  CHL_E_FORMAT = -3,  /* no reader accepts the file / malformed text */
+  CHL_E_WEIGHT = -4,  /* arc length <= 0 (the reference asserts, dijkstra.hpp:91) */
+  CHL_E_ORDER = -5,   /* order is not a permutation of the graph's vertices */
+  CHL_E_NOMEM = -6,   /* device or host allocation failed */
+  CHL_E_DEVICE = -7,  /* HIP runtime error, or no usable gfx950 device */
+  CHL_E_RANGE = -8,   /* id out of range / size beyond a supported maximum */
+  CHL_E_BUSY = -9,    /* server admission refused: no slot or memory free */
+  CHL_E_STATE = -10   /* call not valid in the handle's current state */
+};
+
+typedef struct chl_graph chl_graph;
+typedef struct chl_labels chl_labels;
+typedef struct chl_server chl_server;
+
+const char* chl_strerror(int code);
+const char* chl_last_error(void);
+/* Library version string and the device it will use ("" before first use). */
+const char* chl_version(void);
+/* Number of visible gfx950 devices (0 on a host without one). */
+int chl_device_count(void);
+
+/* ---- graph ------------------------------------------------------------------
+ * Replaces hl::Graph::read(char*, bool undirected) (shared_memory/graph.hpp:327-335):
+ * tries DIMACS, METIS, SNAP, edgelist in that order; returns the detected format
+ * (1..4) through *format if non-null. */
+int chl_graph_load(const char* path, int undirected, chl_graph** g, int* format);
+/* Replaces Graph::set_n + add_arc + finalize (graph.hpp:349-353) for in-memory
+ * arcs (0-based). undirected=1 adds every arc in both directions. */
+int chl_graph_from_arcs(int64_t n, int64_t m, const uint32_t* src, const uint32_t* dst, const int32_t* len,
+                        int undirected, chl_graph** g);
+/* Replaces Graph::write (graph.hpp:338-343): DIMACS, one "a" line per out-arc. */
+int chl_graph_write(const chl_graph* g, const char* path);
+int64_t chl_graph_n(const chl_graph* g);
+/* Arcs in a range: 1 = out-arcs, 0 = in-arcs (graph.hpp:319-320). */
+int64_t chl_graph_arcs(const chl_graph* g, int range);
+/* 1 if every arc has an equal-length reverse arc (labels of both sides agree). */
+int chl_graph_symmetric(const chl_graph* g);
+/* Copies a range as CSR: off[n+1], head[arcs], len[arcs]; rows sorted by head. */
+int chl_graph_csr(const chl_graph* g, int range, uint64_t* off, uint32_t* head, int32_t* len);
+void chl_graph_free(chl_graph* g);
+
+/* ---- workload generators (no reference counterpart: the reference reads
+ * graphs made by external tools, README.md:24-25) ------------------------------ */
+int chl_gen_grid(int64_t rows, int64_t cols, int32_t wmin, int32_t wmax, uint64_t seed, chl_graph** g);
+int chl_gen_knn(int64_t n, int k, uint64_t seed, chl_graph** g);
+int chl_gen_rmat(int scale, int64_t samples, double a, double b, double c, int32_t wmax, uint64_t seed,
+                 chl_graph** g);
+
+/* ---- vertex order (order[rank] = vertex, rank 0 most important) --------------
+ * Replaces hl::Order::read (shared_memory/ordering.hpp:52-67). Writes up to cap
+ * ids and the file's count through *n. */
+int chl_order_load(const char* path, uint32_t* order, int64_t cap, int64_t* n);
+/* Replaces hl::Order::write (ordering.hpp:37-49). */
+int chl_order_write(const char* path, const uint32_t* order, int64_t n);
+/* Sampled-betweenness order (external tool in the reference, README.md:24). */
+int chl_order_betweenness(const chl_graph* g, int64_t roots, uint64_t seed, uint32_t* order);
+/* Degree order, distributed_memory/degree.cpp:62-67 when reference_ties=1. */
+int chl_order_degree(const chl_graph* g, uint64_t seed, int reference_ties, uint32_t* order);
+
+/* ---- label construction --------------------------------------------------- */
+enum { CHL_MODE_GLL = 0, CHL_MODE_PLANT = 1, CHL_MODE_HYBRID = 2 };
+enum { CHL_BATCH_STATIC = 0, CHL_BATCH_LEARNED = 1 };
+
+typedef struct chl_build_opts {
+  int mode;             /* CHL_MODE_*; HYBRID = PLaNT until some root's tree/label ratio passes switch_ratio */
+  int batching;         /* CHL_BATCH_STATIC (rank order) or CHL_BATCH_LEARNED (MLP-predicted packing) */
+  int autotune;         /* 1: pick batch width / delta / slots by measurement on the first batches */
+  int64_t batch_roots;  /* roots per batch (0 = default) */
+  int32_t delta;        /* bucket width of the delta-stepping relaxation (0 = twice the mean arc length) */
+  int32_t slots;        /* concurrent root slots = workgroups of the tree kernel (0 = default) */
+  double switch_ratio;  /* ps of distributed_memory/hhl.cpp:23 (default 1024) */
+  int plant_prune;      /* 1: PLaNT trees also prune by the completed rank-prefix labels */
+  int device;           /* HIP device ordinal */
+  int verbose;          /* 1: per-batch lines on stderr */
+  int reserved[8];
+} chl_build_opts;
+
+typedef struct chl_build_stats {
+  double lt, st, ct, total_s;   /* tree, sort, clean+append, wall seconds (akiba.hpp:214) */
+  double avg_label;             /* (|L0|+|L1|)/n/2, labeling.hpp:153-158 */
+  int64_t labels;               /* total entries, both sides */
+  int64_t batches;
+  int64_t trees_plant, trees_pruned;
+  int64_t switch_rank;          /* first rank built with pruned trees (-1: none) */
+  int64_t settled;              /* vertices processed over all trees */
+  int64_t arcs_scanned;
+  int64_t candidates;           /* labels emitted before cleaning */
+  int64_t prune_entries;        /* label entries read by prune checks */
+  int64_t clean_entries;        /* label entries read by the cleaning merge-join */
+  int64_t delta_used, slots_used, batch_used;
+  double relax_ms, clean_ms;    /* summed device time of the tree kernels / cleaning + append */
+  double relax_bytes;           /* algorithmic bytes of the tree kernels */
+  double mlp_ms;                /* device time of MLP training/inference */
+  double reserved[8];
+} chl_build_stats;
+
+void chl_build_opts_default(chl_build_opts* o);
+/* Replaces hl::run_paraPLL (shared_memory/akiba.hpp:132) and the hybrid hl::run
+ * (distributed_memory/akiba.hpp:190) at one device: builds the canonical labels of
+ * (g, order) on the GPU. Labels stay device-resident in *labels. */
+int chl_build(const chl_graph* g, const uint32_t* order, int64_t n, const chl_build_opts* opts,
+              chl_labels** labels, chl_build_stats* stats);
+
+/* ---- labels ----------------------------------------------------------------- */
+/* Replaces Labeling::write (shared_memory/labeling.hpp:167-183), same text. */
+int chl_labels_write(const chl_labels* l, const char* path);
+/* Replaces Labeling::read (labeling.hpp:186-208); uploads to the device. */
+int chl_labels_read(const char* path, int device, chl_labels** l);
+/* Labels from host CSR arrays (rows must be sorted by hub). */
+int chl_labels_from_csr(int64_t n, const uint64_t* off0, const uint32_t* hub0, const int32_t* d0,
+                        const uint64_t* off1, const uint32_t* hub1, const int32_t* d1, int device,
+                        chl_labels** l);
+int64_t chl_labels_n(const chl_labels* l);
+int64_t chl_labels_total(const chl_labels* l, int side);
+/* Replaces Labeling::get_avg (labeling.hpp:153-158). */
+double chl_labels_avg(const chl_labels* l);
+/* Host copy of one side: off[n+1], hub[total], dist[total]. */
+int chl_labels_copy(const chl_labels* l, int side, uint64_t* off, uint32_t* hub, int32_t* dist);
+void chl_labels_free(chl_labels* l);
+
+/* ---- queries ----------------------------------------------------------------
+ * Replaces Labeling::query(u, v) (labeling.hpp:56-65) batched as batchLocalQuery
+ * (distributed_memory/query_funcs.hpp:23-30): out[k] = d(pairs[2k] -> pairs[2k+1]),
+ * CHL_INFTY when no hub is shared. Host pointers; synchronous. */
+int chl_query(const chl_labels* l, const uint32_t* pairs, int64_t q, int32_t* out);
+/* Same on device pointers, enqueued on `stream` (a hipStream_t; NULL = default). */
+int chl_query_device(const chl_labels* l, const uint32_t* d_pairs, int64_t q, int32_t* d_out, void* stream);
+
+/* ---- serving (continuous batching over resident labels; extends the QLSN
+ * latency/throughput modes of distributed_memory/query.hpp:62-91) ------------- */
+typedef struct chl_server_opts {
+  int streams;             /* concurrent device streams (default 4) */
+  int64_t max_inflight_pairs; /* admission budget in pairs resident on the device */
+  int64_t max_batch_pairs;    /* largest single request */
+  int reserved[8];
+} chl_server_opts;
+
+typedef struct chl_server_stats {
+  int64_t completed, pairs, rejected;
+  double p50_us, p99_us, mean_us, max_us;
+  double qps;              /* pairs per second since creation / last reset */
+  double reserved[8];
+} chl_server_stats;
+
+void chl_server_opts_default(chl_server_opts* o);
+int chl_server_create(const chl_labels* l, const chl_server_opts* o, chl_server** s);
+/* Admits a request when memory allows: copies pairs, returns a ticket at once;
+ * out must stay valid until the ticket completes. CHL_E_BUSY when over budget. */
+int chl_server_submit(chl_server* s, const uint32_t* pairs, int64_t q, int32_t* out, uint64_t* ticket);
+/* 1 if done, 0 if pending; wait=1 blocks until done. */
+int chl_server_poll(chl_server* s, uint64_t ticket, int wait);
+int chl_server_drain(chl_server* s);
+int chl_server_stats_get(chl_server* s, chl_server_stats* st);
+void chl_server_stats_reset(chl_server* s);
+void chl_server_free(chl_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHL_H_ */

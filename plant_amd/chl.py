@@ -1,0 +1,309 @@
+"""Python binding of libchl.so (the C ABI in include/chl.h) over ctypes.
+
+Mirrors the reference's C++ surface for the hot path:
+
+* ``Graph.load`` / ``Graph.from_arcs``   -- hl::Graph::read / add_arc + finalize
+  (shared_memory/graph.hpp:327-353)
+* ``order_load`` / ``order_write``        -- hl::Order::read / write (ordering.hpp:37-67)
+* ``build``                               -- hl::run_paraPLL (shared_memory/akiba.hpp:132) and the
+  hybrid hl::run (distributed_memory/akiba.hpp:190), on the GPU
+* ``Labels.query``                        -- Labeling::query batched like batchLocalQuery
+  (labeling.hpp:56-65, distributed_memory/query_funcs.hpp:23-30)
+* ``Labels.write`` / ``Labels.read``      -- Labeling::write / read (labeling.hpp:167-208)
+
+Errors raise ``ChlError`` carrying the library's status code. The library is
+loaded from this package directory; nothing here falls back to host code.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libchl.so")
+
+INFTY = 2147483647
+MODE_GLL, MODE_PLANT, MODE_HYBRID = 0, 1, 2
+BATCH_STATIC, BATCH_LEARNED = 0, 1
+
+
+class ChlError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"chl error {code}: {msg}")
+        self.code = code
+
+
+class BuildOpts(C.Structure):
+    _fields_ = [("mode", C.c_int), ("batching", C.c_int), ("autotune", C.c_int), ("batch_roots", C.c_int64),
+                ("delta", C.c_int32), ("slots", C.c_int32), ("switch_ratio", C.c_double),
+                ("plant_prune", C.c_int), ("device", C.c_int), ("verbose", C.c_int), ("reserved", C.c_int * 8)]
+
+
+class BuildStats(C.Structure):
+    _fields_ = [("lt", C.c_double), ("st", C.c_double), ("ct", C.c_double), ("total_s", C.c_double),
+                ("avg_label", C.c_double), ("labels", C.c_int64), ("batches", C.c_int64),
+                ("trees_plant", C.c_int64), ("trees_pruned", C.c_int64), ("switch_rank", C.c_int64),
+                ("settled", C.c_int64), ("arcs_scanned", C.c_int64), ("candidates", C.c_int64),
+                ("prune_entries", C.c_int64), ("clean_entries", C.c_int64), ("delta_used", C.c_int64),
+                ("slots_used", C.c_int64), ("batch_used", C.c_int64), ("relax_ms", C.c_double),
+                ("clean_ms", C.c_double), ("relax_bytes", C.c_double), ("mlp_ms", C.c_double),
+                ("reserved", C.c_double * 8)]
+
+    def as_dict(self):
+        return {k: (getattr(self, k) if k != "reserved" else None) for k, _ in self._fields_ if k != "reserved"}
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libchl.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ChlError(-7, f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        vp, i64, u64, i32 = C.c_void_p, C.c_int64, C.c_uint64, C.c_int32
+        pp = C.POINTER(C.c_void_p)
+        sig = {
+            "chl_strerror": ([C.c_int], C.c_char_p),
+            "chl_last_error": ([], C.c_char_p),
+            "chl_version": ([], C.c_char_p),
+            "chl_device_count": ([], C.c_int),
+            "chl_graph_load": ([C.c_char_p, C.c_int, pp, C.POINTER(C.c_int)], C.c_int),
+            "chl_graph_from_arcs": ([i64, i64, vp, vp, vp, C.c_int, pp], C.c_int),
+            "chl_graph_write": ([vp, C.c_char_p], C.c_int),
+            "chl_graph_n": ([vp], i64),
+            "chl_graph_arcs": ([vp, C.c_int], i64),
+            "chl_graph_symmetric": ([vp], C.c_int),
+            "chl_graph_csr": ([vp, C.c_int, vp, vp, vp], C.c_int),
+            "chl_graph_free": ([vp], None),
+            "chl_gen_grid": ([i64, i64, i32, i32, u64, pp], C.c_int),
+            "chl_gen_knn": ([i64, C.c_int, u64, pp], C.c_int),
+            "chl_gen_rmat": ([C.c_int, i64, C.c_double, C.c_double, C.c_double, i32, u64, pp], C.c_int),
+            "chl_order_load": ([C.c_char_p, vp, i64, C.POINTER(i64)], C.c_int),
+            "chl_order_write": ([C.c_char_p, vp, i64], C.c_int),
+            "chl_order_betweenness": ([vp, i64, u64, vp], C.c_int),
+            "chl_order_degree": ([vp, u64, C.c_int, vp], C.c_int),
+            "chl_build_opts_default": ([C.POINTER(BuildOpts)], None),
+            "chl_build": ([vp, vp, i64, C.POINTER(BuildOpts), pp, C.POINTER(BuildStats)], C.c_int),
+            "chl_labels_write": ([vp, C.c_char_p], C.c_int),
+            "chl_labels_read": ([C.c_char_p, C.c_int, pp], C.c_int),
+            "chl_labels_from_csr": ([i64, vp, vp, vp, vp, vp, vp, C.c_int, pp], C.c_int),
+            "chl_labels_n": ([vp], i64),
+            "chl_labels_total": ([vp, C.c_int], i64),
+            "chl_labels_avg": ([vp], C.c_double),
+            "chl_labels_copy": ([vp, C.c_int, vp, vp, vp], C.c_int),
+            "chl_labels_free": ([vp], None),
+            "chl_query": ([vp, vp, i64, vp], C.c_int),
+            "chl_query_device": ([vp, vp, i64, vp, vp], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        L = lib()
+        raise ChlError(rc, L.chl_last_error().decode(errors="replace") or L.chl_strerror(rc).decode())
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+def header_symbols():
+    """Function names declared in include/chl.h."""
+    import re
+    hdr = os.path.join(_HERE, "..", "include", "chl.h")
+    text = open(hdr).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\**\s+\**(chl_[a-z_0-9]+)\s*\(", text, re.M)))
+
+
+class Graph:
+    """A directed graph with the reference's in/out arc ranges (graph.hpp:61-67)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.chl_graph_free(self._h)
+            self._h = C.c_void_p()
+
+    @classmethod
+    def load(cls, path, undirected=False):
+        h = C.c_void_p()
+        fmt = C.c_int(0)
+        _check(lib().chl_graph_load(os.fsencode(path), int(undirected), C.byref(h), C.byref(fmt)))
+        g = cls(h)
+        g.format = fmt.value
+        return g
+
+    @classmethod
+    def from_arcs(cls, n, src, dst, length, undirected=False):
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        length = np.ascontiguousarray(length, np.int32)
+        h = C.c_void_p()
+        _check(lib().chl_graph_from_arcs(int(n), int(src.size), _ptr(src), _ptr(dst), _ptr(length),
+                                         int(undirected), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def grid(cls, rows, cols, wmin=1, wmax=100, seed=1):
+        h = C.c_void_p()
+        _check(lib().chl_gen_grid(rows, cols, wmin, wmax, seed, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def knn(cls, n, k=3, seed=1):
+        h = C.c_void_p()
+        _check(lib().chl_gen_knn(n, k, seed, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def rmat(cls, scale, samples, a=0.57, b=0.19, c=0.19, wmax=1, seed=1):
+        h = C.c_void_p()
+        _check(lib().chl_gen_rmat(scale, samples, a, b, c, wmax, seed, C.byref(h)))
+        return cls(h)
+
+    @property
+    def n(self):
+        return lib().chl_graph_n(self._h)
+
+    def arcs(self, rng=1):
+        return lib().chl_graph_arcs(self._h, rng)
+
+    @property
+    def symmetric(self):
+        return bool(lib().chl_graph_symmetric(self._h))
+
+    def csr(self, rng=1):
+        n = self.n
+        m = self.arcs(rng)
+        off = np.zeros(n + 1, np.uint64)
+        head = np.zeros(max(m, 1), np.uint32)
+        ln = np.zeros(max(m, 1), np.int32)
+        _check(lib().chl_graph_csr(self._h, rng, _ptr(off), _ptr(head), _ptr(ln)))
+        return off, head[:m], ln[:m]
+
+    def arc_list(self, rng=1):
+        """(src, dst, len) of one range, src ascending."""
+        off, head, ln = self.csr(rng)
+        src = np.repeat(np.arange(self.n, dtype=np.uint32), np.diff(off).astype(np.int64))
+        return src, head, ln
+
+    def write(self, path):
+        _check(lib().chl_graph_write(self._h, os.fsencode(path)))
+
+    def order_betweenness(self, roots, seed=7):
+        o = np.zeros(self.n, np.uint32)
+        _check(lib().chl_order_betweenness(self._h, roots, seed, _ptr(o)))
+        return o
+
+    def order_degree(self, seed=7, reference_ties=False):
+        o = np.zeros(self.n, np.uint32)
+        _check(lib().chl_order_degree(self._h, seed, int(reference_ties), _ptr(o)))
+        return o
+
+
+def order_load(path):
+    n = C.c_int64(0)
+    _check(lib().chl_order_load(os.fsencode(path), None, 0, C.byref(n)))
+    o = np.zeros(n.value, np.uint32)
+    _check(lib().chl_order_load(os.fsencode(path), _ptr(o), n.value, C.byref(n)))
+    return o
+
+
+def order_write(path, order):
+    order = np.ascontiguousarray(order, np.uint32)
+    _check(lib().chl_order_write(os.fsencode(path), _ptr(order), order.size))
+
+
+def default_opts(**kw):
+    o = BuildOpts()
+    lib().chl_build_opts_default(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise TypeError(f"unknown build option {k}")
+        setattr(o, k, v)
+    return o
+
+
+class Labels:
+    """Device-resident canonical labels (per vertex, per side, hub-ascending)."""
+
+    def __init__(self, handle, stats=None):
+        self._h = handle
+        self.stats = stats
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.chl_labels_free(self._h)
+            self._h = C.c_void_p()
+
+    @property
+    def n(self):
+        return lib().chl_labels_n(self._h)
+
+    def total(self, side):
+        return lib().chl_labels_total(self._h, side)
+
+    @property
+    def avg(self):
+        return lib().chl_labels_avg(self._h)
+
+    def copy(self, side):
+        """(off u64[n+1], hub u32[T], dist i32[T]) of one side, rows by vertex id."""
+        n = self.n
+        t = self.total(side)
+        off = np.zeros(n + 1, np.uint64)
+        hub = np.zeros(max(t, 1), np.uint32)
+        dist = np.zeros(max(t, 1), np.int32)
+        _check(lib().chl_labels_copy(self._h, side, _ptr(off), _ptr(hub), _ptr(dist)))
+        return off, hub[:t], dist[:t]
+
+    def write(self, path):
+        _check(lib().chl_labels_write(self._h, os.fsencode(path)))
+
+    @classmethod
+    def read(cls, path, device=0):
+        h = C.c_void_p()
+        _check(lib().chl_labels_read(os.fsencode(path), device, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_csr(cls, n, side0, side1, device=0):
+        arrs = []
+        for off, hub, dist in (side0, side1):
+            arrs += [np.ascontiguousarray(off, np.uint64), np.ascontiguousarray(hub, np.uint32),
+                     np.ascontiguousarray(dist, np.int32)]
+        h = C.c_void_p()
+        _check(lib().chl_labels_from_csr(n, *[_ptr(a) for a in arrs], device, C.byref(h)))
+        return cls(h)
+
+    def query(self, pairs):
+        """d(u -> v) for each row (u, v) of pairs; INFTY when no hub is shared."""
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        out = np.zeros(pairs.shape[0], np.int32)
+        _check(lib().chl_query(self._h, _ptr(pairs), pairs.shape[0], _ptr(out)))
+        return out
+
+
+def build(graph, order, **opts):
+    """Canonical hub labels of (graph, order) built on the GPU."""
+    order = np.ascontiguousarray(order, np.uint32)
+    o = default_opts(**opts)
+    st = BuildStats()
+    h = C.c_void_p()
+    _check(lib().chl_build(graph._h, _ptr(order), order.size, C.byref(o), C.byref(h), C.byref(st)))
+    return Labels(h, st)
+
+
+def device_count():
+    return lib().chl_device_count()

@@ -1,0 +1,247 @@
+// C ABI: label construction, label handles and distance queries.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../include/chl.h"
+#include "dev.hpp"
+#include "engine.hpp"
+#include "internal.hpp"
+
+using namespace chl;
+
+struct chl_labels {
+  LabelStore store;
+  DBuf<uint32_t> rank_of;        // vertex id -> row id
+  std::vector<uint32_t> row_of;  // host copy of rank_of
+  // query staging for host-pointer calls
+  DBuf<uint32_t> qpairs;
+  DBuf<int32_t> qout;
+  hipStream_t stream = nullptr;
+  ~chl_labels() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int make_handle(chl_labels* l, const std::vector<uint32_t>& row_of) {
+  l->row_of = row_of;
+  CHL_HIP(l->rank_of.reserve(row_of.size()));
+  if (!row_of.empty())
+    CHL_HIP(hipMemcpy(l->rank_of.p, row_of.data(), sizeof(uint32_t) * row_of.size(), hipMemcpyHostToDevice));
+  CHL_HIP(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking));
+  return CHL_OK;
+}
+
+// Host rows in vertex order from the device store.
+int to_host(const chl_labels* l, int side, std::vector<uint64_t>& off, std::vector<uint64_t>& ent) {
+  std::vector<uint64_t> roff, rent;
+  int rc = l->store.download(side, roff, rent);
+  if (rc) return rc;
+  const int64_t n = l->store.n;
+  off.assign(n + 1, 0);
+  for (int64_t v = 0; v < n; ++v) {
+    uint32_t r = l->row_of[v];
+    off[v + 1] = off[v] + (roff[r + 1] - roff[r]);
+  }
+  ent.resize(off[n]);
+  parallel_for(n, host_threads(), 4096, [&](int64_t v) {
+    uint32_t r = l->row_of[v];
+    std::copy(rent.begin() + roff[r], rent.begin() + roff[r + 1], ent.begin() + off[v]);
+  });
+  return CHL_OK;
+}
+
+int upload_side(LabelStore& S, int side, const std::vector<uint64_t>& off, const std::vector<uint64_t>& ent) {
+  CHL_HIP(S.off[side].reserve(S.n + 1));
+  CHL_HIP(hipMemcpy(S.off[side].p, off.data(), sizeof(uint64_t) * (S.n + 1), hipMemcpyHostToDevice));
+  CHL_HIP(S.ent[side].reserve(std::max<size_t>(ent.size(), 1)));
+  if (!ent.empty()) CHL_HIP(hipMemcpy(S.ent[side].p, ent.data(), sizeof(uint64_t) * ent.size(), hipMemcpyHostToDevice));
+  S.total[side] = ent.size();
+  return CHL_OK;
+}
+
+bool rows_sorted(const std::vector<uint64_t>& off, const std::vector<uint64_t>& ent) {
+  for (size_t v = 0; v + 1 < off.size(); ++v)
+    for (uint64_t k = off[v] + 1; k < off[v + 1]; ++k)
+      if ((ent[k] >> 32) <= (ent[k - 1] >> 32)) return false;
+  return true;
+}
+
+int from_host_labels(const Labels& H, int device, chl_labels** out) {
+  for (int s = 0; s < 2; ++s)
+    if (!rows_sorted(H.off[s], H.ent[s])) return set_error(CHL_E_FORMAT, "label rows must be hub-ascending");
+  int rc = ensure_device(device);
+  if (rc) return rc;
+  chl_labels* l = new chl_labels();
+  l->store.n = H.n;
+  l->store.device = device;
+  l->store.shared_sides = false;
+  for (int s = 0; s < 2 && rc == CHL_OK; ++s) rc = upload_side(l->store, s, H.off[s], H.ent[s]);
+  std::vector<uint32_t> ident(H.n);
+  for (int64_t v = 0; v < H.n; ++v) ident[v] = (uint32_t)v;
+  if (rc == CHL_OK) rc = make_handle(l, ident);
+  if (rc) {
+    delete l;
+    return rc;
+  }
+  *out = l;
+  return CHL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* chl_version(void) { return "chl-mi355x 0.1 (gfx950)"; }
+
+int chl_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+void chl_build_opts_default(chl_build_opts* o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->mode = CHL_MODE_PLANT;
+  o->batching = CHL_BATCH_STATIC;
+  o->autotune = 0;
+  o->batch_roots = 0;
+  o->delta = 0;
+  o->slots = 0;
+  o->switch_ratio = 1024.0;
+  o->plant_prune = 0;
+  o->device = 0;
+}
+
+int chl_build(const chl_graph* g, const uint32_t* order, int64_t n, const chl_build_opts* opts, chl_labels** labels,
+              chl_build_stats* stats) {
+  if (!g || !order || !labels) return set_error(CHL_E_INVAL, "null argument");
+  *labels = nullptr;
+  if (n != g->g.n)
+    return set_error(CHL_E_ORDER, "order has %lld entries, graph has %lld vertices", (long long)n, (long long)g->g.n);
+  std::vector<uint32_t> ord(order, order + n), rank(n, 0xFFFFFFFFu);
+  for (int64_t i = 0; i < n; ++i) {
+    if (ord[i] >= (uint64_t)n || rank[ord[i]] != 0xFFFFFFFFu)
+      return set_error(CHL_E_ORDER, "order is not a permutation (entry %lld = %u)", (long long)i, ord[i]);
+    rank[ord[i]] = (uint32_t)i;
+  }
+  for (int r = 0; r < 2; ++r)
+    for (int32_t w : g->g.len[r])
+      if (w <= 0) return set_error(CHL_E_WEIGHT, "arc length %d <= 0", w);
+  chl_build_opts o;
+  if (opts) o = *opts;
+  else chl_build_opts_default(&o);
+  if (o.mode < CHL_MODE_GLL || o.mode > CHL_MODE_HYBRID) return set_error(CHL_E_INVAL, "bad mode %d", o.mode);
+  if (o.switch_ratio <= 0) o.switch_ratio = 1024.0;
+  int rc = ensure_device(o.device);
+  if (rc) return rc;
+  if (stats) memset(stats, 0, sizeof(*stats));
+  chl_labels* l = new chl_labels();
+  rc = run_build(g->g, ord, o, stats, l->store);
+  if (rc == CHL_OK) rc = make_handle(l, rank);
+  if (rc) {
+    delete l;
+    return rc;
+  }
+  *labels = l;
+  return CHL_OK;
+}
+
+int chl_labels_write(const chl_labels* l, const char* path) {
+  if (!l || !path) return set_error(CHL_E_INVAL, "null argument");
+  Labels H;
+  H.n = l->store.n;
+  for (int s = 0; s < 2; ++s) {
+    int rc = to_host(l, s, H.off[s], H.ent[s]);
+    if (rc) return rc;
+  }
+  int rc = labels_write(H, path);
+  return rc == CHL_OK ? rc : set_error(rc, "cannot write labels to %s", path);
+}
+
+int chl_labels_read(const char* path, int device, chl_labels** out) {
+  if (!path || !out) return set_error(CHL_E_INVAL, "null argument");
+  *out = nullptr;
+  Labels H;
+  int rc = labels_read(H, path);
+  if (rc) return set_error(rc, "cannot read labels from %s", path);
+  return from_host_labels(H, device, out);
+}
+
+int chl_labels_from_csr(int64_t n, const uint64_t* off0, const uint32_t* hub0, const int32_t* d0, const uint64_t* off1,
+                        const uint32_t* hub1, const int32_t* d1, int device, chl_labels** out) {
+  if (!out || n <= 0 || !off0 || !off1) return set_error(CHL_E_INVAL, "bad arguments");
+  *out = nullptr;
+  Labels H;
+  H.n = n;
+  const uint64_t* offs[2] = {off0, off1};
+  const uint32_t* hubs[2] = {hub0, hub1};
+  const int32_t* ds[2] = {d0, d1};
+  for (int s = 0; s < 2; ++s) {
+    H.off[s].assign(offs[s], offs[s] + n + 1);
+    const uint64_t T = H.off[s][n];
+    if (T && (!hubs[s] || !ds[s])) return set_error(CHL_E_INVAL, "null label arrays");
+    H.ent[s].resize(T);
+    for (uint64_t k = 0; k < T; ++k) {
+      if (ds[s][k] < 0) return set_error(CHL_E_INVAL, "negative label distance");
+      H.ent[s][k] = ((uint64_t)hubs[s][k] << 32) | (uint32_t)ds[s][k];
+    }
+  }
+  return from_host_labels(H, device, out);
+}
+
+int64_t chl_labels_n(const chl_labels* l) { return l ? l->store.n : -1; }
+int64_t chl_labels_total(const chl_labels* l, int side) {
+  if (!l || side < 0 || side > 1) return -1;
+  return (int64_t)l->store.side_total(side);
+}
+double chl_labels_avg(const chl_labels* l) {
+  if (!l || l->store.n == 0) return 0.0;
+  return (double)(l->store.side_total(0) + l->store.side_total(1)) / (double)l->store.n / 2.0;
+}
+
+int chl_labels_copy(const chl_labels* l, int side, uint64_t* off, uint32_t* hub, int32_t* dist) {
+  if (!l || side < 0 || side > 1 || !off) return set_error(CHL_E_INVAL, "bad arguments");
+  std::vector<uint64_t> o, e;
+  int rc = to_host(l, side, o, e);
+  if (rc) return rc;
+  memcpy(off, o.data(), sizeof(uint64_t) * o.size());
+  for (size_t k = 0; k < e.size(); ++k) {
+    if (hub) hub[k] = (uint32_t)(e[k] >> 32);
+    if (dist) dist[k] = (int32_t)(uint32_t)e[k];
+  }
+  return CHL_OK;
+}
+
+void chl_labels_free(chl_labels* l) { delete l; }
+
+int chl_query_device(const chl_labels* l, const uint32_t* d_pairs, int64_t q, int32_t* d_out, void* stream) {
+  if (!l || q < 0 || (q > 0 && (!d_pairs || !d_out))) return set_error(CHL_E_INVAL, "bad arguments");
+  if (q == 0) return CHL_OK;
+  CHL_HIP(hipSetDevice(l->store.device));
+  CHL_HIP(launch_query(l->store.d_off(1), l->store.d_ent(1), l->store.d_off(0), l->store.d_ent(0), d_pairs,
+                       l->rank_of.p, (uint32_t)l->store.n, (uint64_t)q, d_out, (hipStream_t)stream));
+  return CHL_OK;
+}
+
+int chl_query(const chl_labels* lc, const uint32_t* pairs, int64_t q, int32_t* out) {
+  chl_labels* l = const_cast<chl_labels*>(lc);
+  if (!l || q < 0 || (q > 0 && (!pairs || !out))) return set_error(CHL_E_INVAL, "bad arguments");
+  if (q == 0) return CHL_OK;
+  for (int64_t k = 0; k < 2 * q; ++k)
+    if ((int64_t)pairs[k] >= l->store.n) return set_error(CHL_E_RANGE, "vertex %u out of range", pairs[k]);
+  CHL_HIP(hipSetDevice(l->store.device));
+  CHL_HIP(l->qpairs.reserve(2 * q));
+  CHL_HIP(l->qout.reserve(q));
+  CHL_HIP(hipMemcpyAsync(l->qpairs.p, pairs, sizeof(uint32_t) * 2 * q, hipMemcpyHostToDevice, l->stream));
+  CHL_HIP(launch_query(l->store.d_off(1), l->store.d_ent(1), l->store.d_off(0), l->store.d_ent(0), l->qpairs.p,
+                       l->rank_of.p, (uint32_t)l->store.n, (uint64_t)q, l->qout.p, l->stream));
+  CHL_HIP(hipMemcpyAsync(out, l->qout.p, sizeof(int32_t) * q, hipMemcpyDeviceToHost, l->stream));
+  CHL_HIP(hipStreamSynchronize(l->stream));
+  return CHL_OK;
+}
+
+}  // extern "C"

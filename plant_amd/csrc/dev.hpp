@@ -1,0 +1,142 @@
+// Device-side declarations shared by the HIP kernels and the host engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/chl.h"
+#include "host.hpp"
+
+namespace chl {
+
+// Returns CHL_E_DEVICE (and records the message) from the enclosing int-returning
+// function when a HIP call fails.
+#define CHL_HIP(x)                                                                                  \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess)                                                                           \
+      return ::chl::set_error(CHL_E_DEVICE, "%s:%d %s: %s", __FILE__, __LINE__, #x,                 \
+                              hipGetErrorString(e_));                                               \
+  } while (0)
+
+// Device buffer that frees itself.
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // Allocates exactly `count` elements (contents undefined); keeps a larger buffer.
+  hipError_t reserve(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    size_t bytes = (count ? count : 1) * sizeof(T);
+    hipError_t e = hipMalloc((void**)&p, bytes);
+    if (e == hipSuccess) n = count;
+    else p = nullptr;
+    return e;
+  }
+  void swap(DBuf& o) {
+    T* tp = p; p = o.p; o.p = tp;
+    size_t tn = n; n = o.n; o.n = tn;
+  }
+};
+
+// Tree-kernel mode bits.
+enum : uint32_t {
+  TREE_FLAGS = 1u,  // propagate the most important rank on the path (PLaNT, distributed_memory/akiba.hpp:71,89)
+  TREE_PRUNE = 2u,  // prune by the completed rank-prefix labels (shared_memory/akiba.hpp:94-110)
+};
+
+// Packed per-vertex search state: (dist << 32) | low, where low is the smallest
+// rank on the best path found so far (TREE_FLAGS) or the root rank, and 0 once
+// the vertex is pruned at that distance. Unreached = all ones.
+static constexpr uint64_t kStateInf = ~0ull;
+
+// Counters of algorithmic work, summed over a launch.
+enum { CTR_PROCESSED = 0, CTR_ARCS = 1, CTR_PRUNE_ENT = 2, CTR_LABELS = 3, CTR_OVERFLOW = 4, CTR_NEXT = 5,
+       CTR_COUNT = 8 };
+
+struct TreeParams {
+  // graph in rank ids: arcs of the range walked by the tree (out-arcs for a
+  // forward tree), arc = (head << 32) | length
+  const uint32_t* off;
+  const uint64_t* arc;
+  uint32_t n;
+  // label store (rows in rank ids, hub-sorted, all hubs < prefix)
+  const uint64_t* scan_off;  // side read at a visited vertex
+  const uint64_t* scan_ent;  // (hub << 32) | dist
+  const uint64_t* root_off;  // side read at the root
+  const uint64_t* root_ent;
+  uint32_t prefix;
+  // batch
+  const uint32_t* roots;
+  uint32_t R;
+  uint32_t delta;
+  // per-slot scratch, slot = workgroup index; each array holds n entries per slot
+  uint64_t* st;
+  uint32_t* stamp;
+  uint32_t* qa;
+  uint32_t* qb;
+  uint32_t* vis;
+  uint32_t* gkey;   // global root-label hash per slot (gcap entries) when the row is too long for LDS
+  uint32_t* gval;
+  uint32_t gcap;
+  // output: emitted labels, key = (vertex << 32) | hub, ent = (hub << 32) | dist
+  uint64_t* out_key;
+  uint64_t* out_ent;
+  uint64_t out_cap;
+  // per root statistics and launch counters
+  uint32_t* tree_size;
+  uint32_t* label_count;
+  unsigned long long* ctr;  // CTR_* (CTR_NEXT is the work-queue cursor)
+};
+
+hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t count, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s);
+
+// Label-store maintenance.
+// seg[v] = first index k of the sorted keys with key >> 32 >= v, for v in [0, n].
+hipError_t launch_seg_offsets(const uint64_t* keys, uint64_t m, uint32_t n, uint64_t* seg, hipStream_t s);
+// new_off[v] = old_off[v] + excl[v] for v in [0, n].
+hipError_t launch_merge_offsets(const uint64_t* old_off, const uint64_t* excl, uint32_t n, uint64_t* new_off,
+                                hipStream_t s);
+// out[0] = max over v of off[v+1] - off[v] (out must be zeroed).
+hipError_t launch_max_row(const uint64_t* off, uint32_t n, unsigned long long* out, hipStream_t s);
+// Switch test of the hybrid build (distributed_memory/akiba.hpp:297-303): sets
+// *flag when some root's tree_size / label_count exceeds ratio.
+hipError_t launch_switch_test(const uint32_t* tree_size, const uint32_t* label_count, uint32_t R, double ratio,
+                              unsigned long long* flag, hipStream_t s);
+// Rows of the new store: old row v, then the batch segment of v (sorted).
+hipError_t launch_merge_rows(const uint64_t* old_off, const uint64_t* old_ent, const uint64_t* excl,
+                             const uint64_t* new_off, const uint64_t* bat_ent, uint32_t n, uint64_t* new_ent,
+                             hipStream_t s);
+// In-batch cleaning (shared_memory/labeling.hpp:78-88 over the batch labels, which
+// are sorted by (vertex, hub)): keep[k] = 0 when candidate k is covered.
+hipError_t launch_clean(const uint64_t* key, const uint64_t* ent, uint64_t m, const uint64_t* seg_off,
+                        const uint64_t* oent, const uint64_t* oseg_off, uint8_t* keep, unsigned long long* ctr,
+                        hipStream_t s);
+// Label merge-join query (shared_memory/labeling.hpp:56-65). pairs hold vertex
+// ids, mapped to rows by rank_of; an id >= n gives -1.
+hipError_t launch_query(const uint64_t* off1, const uint64_t* ent1, const uint64_t* off0, const uint64_t* ent0,
+                        const uint32_t* pairs, const uint32_t* rank_of, uint32_t n, uint64_t q, int32_t* out,
+                        hipStream_t s);
+
+// Library primitives (hipCUB): radix sort of 64-bit key/value pairs, exclusive
+// scan of u32 counts into u64 offsets, stream compaction by flags.
+hipError_t sort_pairs_u64(void* tmp, size_t& tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint64_t* vin,
+                          uint64_t* vout, uint64_t m, int end_bit, hipStream_t s);
+hipError_t exclusive_scan_u32_u64(void* tmp, size_t& tmp_bytes, const uint32_t* in, uint64_t* out, uint64_t m,
+                                  hipStream_t s);
+hipError_t select_flagged_u64(void* tmp, size_t& tmp_bytes, const uint64_t* in, const uint8_t* flags, uint64_t* out,
+                              uint64_t* nsel, uint64_t m, hipStream_t s);
+
+}  // namespace chl

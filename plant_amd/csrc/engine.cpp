@@ -1,0 +1,481 @@
+// Label construction on one GPU: the drop-in replacement of run_paraPLL
+// (shared_memory/akiba.hpp:132-217) and of the hybrid run
+// (distributed_memory/akiba.hpp:190-510) at one rank, plus the label handle and
+// the batched distance query (shared_memory/labeling.hpp:56-65,
+// distributed_memory/query_funcs.hpp:23-30).
+//
+// Device layout (all in HBM for the whole build):
+//   * graph in rank ids (vertex i = order[i]); per arc range an offset array
+//     u32[n+1] and packed arcs u64[A] = (head << 32) | length. A symmetric graph
+//     keeps one range, since both trees of a root are then the same.
+//   * label store per side: off u64[n+1], ent u64[T] = (hub << 32) | dist, rows
+//     hub-ascending; a symmetric graph stores side 0 only (side 1 is identical).
+//   * per-slot search scratch (tree.hip), S slots x 24 n bytes.
+//   * batch output: key (vertex << 32 | hub), ent; sorted, cleaned (GLL) and
+//     appended to the store after every batch.
+//
+// A batch is a contiguous rank range [r0, r1). Its trees read only labels of
+// hubs < r0, which are final, so the labels do not depend on how roots are
+// spread over workgroups, on the batch width or on the bucket width.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dev.hpp"
+#include "engine.hpp"
+#include "internal.hpp"
+
+using namespace chl;
+
+namespace chl {
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int ensure_device(int device) {
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return set_error(CHL_E_DEVICE, "no HIP device visible");
+  if (device < 0 || device >= count) return set_error(CHL_E_DEVICE, "device %d out of range (%d visible)", device, count);
+  CHL_HIP(hipSetDevice(device));
+  return CHL_OK;
+}
+
+// Builds the rank-id CSR of one arc range.
+static void relabel_range(const Graph& g, int range, const std::vector<uint32_t>& order,
+                          const std::vector<uint32_t>& rank, std::vector<uint32_t>& off, std::vector<uint64_t>& arc) {
+  const int64_t n = g.n;
+  off.assign(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t v = order[i];
+    off[i + 1] = off[i] + (uint32_t)(g.off[range][v + 1] - g.off[range][v]);
+  }
+  arc.resize(off[n]);
+  parallel_for(n, host_threads(), 4096, [&](int64_t i) {
+    uint32_t v = order[i];
+    uint64_t o = off[i];
+    for (uint64_t a = g.off[range][v]; a < g.off[range][v + 1]; ++a)
+      arc[o++] = ((uint64_t)rank[g.head[range][a]] << 32) | (uint32_t)g.len[range][a];
+  });
+}
+
+int LabelStore::init(int64_t nn, bool shared, int dev) {
+  n = nn;
+  shared_sides = shared;
+  device = dev;
+  for (int s = 0; s < (shared ? 1 : 2); ++s) {
+    CHL_HIP(off[s].reserve(n + 1));
+    CHL_HIP(hipMemset(off[s].p, 0, sizeof(uint64_t) * (n + 1)));
+    CHL_HIP(ent[s].reserve(1));
+    total[s] = 0;
+  }
+  return CHL_OK;
+}
+
+int LabelStore::download(int side, std::vector<uint64_t>& hoff, std::vector<uint64_t>& hent) const {
+  const int s = shared_sides ? 0 : side;
+  hoff.resize(n + 1);
+  CHL_HIP(hipMemcpy(hoff.data(), off[s].p, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost));
+  hent.resize(hoff[n]);
+  if (hoff[n]) CHL_HIP(hipMemcpy(hent.data(), ent[s].p, sizeof(uint64_t) * hoff[n], hipMemcpyDeviceToHost));
+  return CHL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Builder
+// ---------------------------------------------------------------------------
+struct SideBatch {
+  DBuf<uint64_t> key, ent, skey, sent, seg, ckey, cent;
+  DBuf<uint8_t> keep;
+  uint64_t m = 0;       // labels emitted (before cleaning)
+  uint64_t kept = 0;    // labels appended
+};
+
+struct Builder {
+  const Graph& g;
+  const std::vector<uint32_t>& order;
+  chl_build_opts o;
+  chl_build_stats* stats;
+  BuildHooks* hooks;
+  int64_t n;
+  bool sym;
+  int nsides;
+  hipStream_t stream = nullptr;
+
+  DBuf<uint32_t> goff[2];
+  DBuf<uint64_t> garc[2];
+  DBuf<uint64_t> st;
+  DBuf<uint32_t> stamp, qa, qb, vis, gkey, gval;
+  uint32_t slots = 0, gcap = 0;
+  DBuf<uint32_t> roots, tree_size, label_count;
+  DBuf<unsigned long long> ctr;
+  DBuf<unsigned long long> misc;  // [0] switch flag, [1] max row
+  DBuf<uint64_t> noff, nent;
+  DBuf<uint8_t> tmp;
+  SideBatch sb[2];
+  uint64_t out_cap = 0;
+  hipEvent_t ev[8];
+
+  Builder(const Graph& gg, const std::vector<uint32_t>& ord, const chl_build_opts& opts, chl_build_stats* s,
+          BuildHooks* h)
+      : g(gg), order(ord), o(opts), stats(s), hooks(h), n(gg.n), sym(gg.symmetric) {
+    nsides = sym ? 1 : 2;
+    for (auto& e : ev) e = nullptr;
+  }
+  ~Builder() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int upload_graph(const std::vector<uint32_t>& rank) {
+    for (int r = 0; r < 2; ++r) {
+      if (sym && r == 0) continue;  // in-arcs equal out-arcs
+      if (g.off[r][n] >= 0xFFFFFFFFull) return set_error(CHL_E_RANGE, "more than 2^32 arcs in a range");
+      std::vector<uint32_t> off;
+      std::vector<uint64_t> arc;
+      relabel_range(g, r, order, rank, off, arc);
+      CHL_HIP(goff[r].reserve(n + 1));
+      CHL_HIP(garc[r].reserve(arc.size()));
+      CHL_HIP(hipMemcpy(goff[r].p, off.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
+      if (!arc.empty())
+        CHL_HIP(hipMemcpy(garc[r].p, arc.data(), sizeof(uint64_t) * arc.size(), hipMemcpyHostToDevice));
+    }
+    return CHL_OK;
+  }
+
+  int alloc_slots(uint32_t want) {
+    size_t freeb = 0, totalb = 0;
+    CHL_HIP(hipMemGetInfo(&freeb, &totalb));
+    const size_t per = (size_t)n * 24;
+    size_t budget = freeb / 3;
+    uint32_t fit = (uint32_t)std::max<size_t>(1, budget / std::max<size_t>(per, 1));
+    slots = std::max<uint32_t>(1, std::min(want, fit));
+    const size_t cells = (size_t)slots * n;
+    CHL_HIP(st.reserve(cells));
+    CHL_HIP(stamp.reserve(cells));
+    CHL_HIP(qa.reserve(cells));
+    CHL_HIP(qb.reserve(cells));
+    CHL_HIP(vis.reserve(cells));
+    CHL_HIP(launch_fill_u64(st.p, kStateInf, cells, stream));
+    CHL_HIP(launch_fill_u32(stamp.p, 0, cells, stream));
+    return CHL_OK;
+  }
+
+  int ensure_gtab(uint64_t max_row) {
+    uint32_t need = 1;
+    while (need < 2 * max_row + 2) need <<= 1;
+    if (need <= gcap) return CHL_OK;
+    gcap = need;
+    CHL_HIP(gkey.reserve((size_t)slots * gcap));
+    CHL_HIP(gval.reserve((size_t)slots * gcap));
+    return CHL_OK;
+  }
+
+  int ensure_out(SideBatch& b, uint64_t cap) {
+    CHL_HIP(b.key.reserve(cap));
+    CHL_HIP(b.ent.reserve(cap));
+    return CHL_OK;
+  }
+
+  // Runs the trees of roots[0..R) for one side; returns labels emitted in b.m.
+  int run_trees(int side, uint32_t R, uint32_t prefix, uint32_t flags, LabelStore& L, double& ms) {
+    SideBatch& b = sb[side];
+    const int range = side == 0 ? 1 : 0;  // forward tree walks out-arcs and fills side 0
+    for (;;) {
+      int rc = ensure_out(b, out_cap);
+      if (rc) return rc;
+      TreeParams p;
+      memset(&p, 0, sizeof(p));
+      p.off = goff[sym ? 1 : range].p;
+      p.arc = garc[sym ? 1 : range].p;
+      p.n = (uint32_t)n;
+      p.scan_off = L.d_off(side);
+      p.scan_ent = L.d_ent(side);
+      p.root_off = L.d_off(1 - side);
+      p.root_ent = L.d_ent(1 - side);
+      p.prefix = prefix;
+      p.roots = roots.p;
+      p.R = R;
+      p.delta = (uint32_t)o.delta;
+      p.st = st.p;
+      p.stamp = stamp.p;
+      p.qa = qa.p;
+      p.qb = qb.p;
+      p.vis = vis.p;
+      p.gkey = gkey.p;
+      p.gval = gval.p;
+      p.gcap = gcap;
+      p.out_key = b.key.p;
+      p.out_ent = b.ent.p;
+      p.out_cap = out_cap;
+      p.tree_size = tree_size.p;
+      p.label_count = label_count.p;
+      p.ctr = ctr.p;
+      CHL_HIP(launch_fill_u64((uint64_t*)ctr.p, 0, CTR_COUNT, stream));
+      CHL_HIP(hipEventRecord(ev[0], stream));
+      CHL_HIP(launch_tree(p, flags, std::min<uint32_t>(slots, R), stream));
+      CHL_HIP(hipEventRecord(ev[1], stream));
+      unsigned long long hc[CTR_COUNT];
+      CHL_HIP(hipMemcpyAsync(hc, ctr.p, sizeof(hc), hipMemcpyDeviceToHost, stream));
+      CHL_HIP(hipStreamSynchronize(stream));
+      float t = 0;
+      CHL_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
+      ms += t;
+      if (hc[CTR_OVERFLOW]) {
+        out_cap = std::max<uint64_t>(out_cap * 2, hc[CTR_LABELS] + hc[CTR_LABELS] / 4);
+        continue;
+      }
+      b.m = hc[CTR_LABELS];
+      if (stats) {
+        stats->settled += (int64_t)hc[CTR_PROCESSED];
+        stats->arcs_scanned += (int64_t)hc[CTR_ARCS];
+        stats->prune_entries += (int64_t)hc[CTR_PRUNE_ENT];
+        stats->candidates += (int64_t)hc[CTR_LABELS];
+        stats->relax_ms += t;
+        stats->relax_bytes += 16.0 * hc[CTR_PROCESSED] + 16.0 * hc[CTR_ARCS] + 8.0 * hc[CTR_PRUNE_ENT] +
+                              16.0 * hc[CTR_LABELS];
+      }
+      if (hooks && hooks->on_tree_launch) hooks->on_tree_launch(hooks->ctx, side, R, t, hc);
+      return CHL_OK;
+    }
+  }
+
+  int sort_side(int side, double& ms) {
+    SideBatch& b = sb[side];
+    const uint64_t m = b.m;
+    CHL_HIP(b.skey.reserve(std::max<uint64_t>(m, 1)));
+    CHL_HIP(b.sent.reserve(std::max<uint64_t>(m, 1)));
+    CHL_HIP(b.seg.reserve(n + 1));
+    CHL_HIP(hipEventRecord(ev[2], stream));
+    if (m) {
+      int bits = 0;
+      while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)n) ++bits;
+      size_t tb = 0;
+      CHL_HIP(sort_pairs_u64(nullptr, tb, b.key.p, b.skey.p, b.ent.p, b.sent.p, m, 32 + bits, stream));
+      CHL_HIP(tmp.reserve(tb));
+      CHL_HIP(sort_pairs_u64(tmp.p, tb, b.key.p, b.skey.p, b.ent.p, b.sent.p, m, 32 + bits, stream));
+    }
+    CHL_HIP(launch_seg_offsets(b.skey.p, m, (uint32_t)n, b.seg.p, stream));
+    CHL_HIP(hipEventRecord(ev[3], stream));
+    CHL_HIP(hipEventSynchronize(ev[3]));
+    float t = 0;
+    CHL_HIP(hipEventElapsedTime(&t, ev[2], ev[3]));
+    ms += t;
+    return CHL_OK;
+  }
+
+  // GLL cleaning of one side against the batch labels of both sides.
+  int clean_side(int side, double& ms) {
+    SideBatch& b = sb[side];
+    SideBatch& ob = sb[sym ? side : 1 - side];
+    const uint64_t m = b.m;
+    CHL_HIP(hipEventRecord(ev[4], stream));
+    if (m) {
+      CHL_HIP(b.keep.reserve(m));
+      CHL_HIP(b.ckey.reserve(m));
+      CHL_HIP(b.cent.reserve(m));
+      CHL_HIP(launch_clean(b.skey.p, b.sent.p, m, b.seg.p, ob.sent.p, ob.seg.p, b.keep.p, ctr.p, stream));
+    }
+    CHL_HIP(hipEventRecord(ev[5], stream));
+    CHL_HIP(hipEventSynchronize(ev[5]));
+    float t = 0;
+    CHL_HIP(hipEventElapsedTime(&t, ev[4], ev[5]));
+    ms += t;
+    return CHL_OK;
+  }
+
+  int compact_side(int side, double& ms) {
+    SideBatch& b = sb[side];
+    const uint64_t m = b.m;
+    if (!m) return CHL_OK;
+    CHL_HIP(hipEventRecord(ev[4], stream));
+    DBuf<uint64_t> nsel;
+    CHL_HIP(nsel.reserve(1));
+    size_t tb = 0;
+    CHL_HIP(select_flagged_u64(nullptr, tb, b.skey.p, b.keep.p, b.ckey.p, nsel.p, m, stream));
+    CHL_HIP(tmp.reserve(tb));
+    CHL_HIP(select_flagged_u64(tmp.p, tb, b.skey.p, b.keep.p, b.ckey.p, nsel.p, m, stream));
+    CHL_HIP(select_flagged_u64(tmp.p, tb, b.sent.p, b.keep.p, b.cent.p, nsel.p, m, stream));
+    uint64_t kept = 0;
+    CHL_HIP(hipMemcpyAsync(&kept, nsel.p, sizeof(kept), hipMemcpyDeviceToHost, stream));
+    CHL_HIP(hipStreamSynchronize(stream));
+    b.skey.swap(b.ckey);
+    b.sent.swap(b.cent);
+    b.m = kept;
+    CHL_HIP(launch_seg_offsets(b.skey.p, kept, (uint32_t)n, b.seg.p, stream));
+    CHL_HIP(hipEventRecord(ev[5], stream));
+    CHL_HIP(hipEventSynchronize(ev[5]));
+    float t = 0;
+    CHL_HIP(hipEventElapsedTime(&t, ev[4], ev[5]));
+    ms += t;
+    return CHL_OK;
+  }
+
+  int merge_side(int side, LabelStore& L, double& ms) {
+    SideBatch& b = sb[side];
+    const uint64_t m = b.m;
+    b.kept = m;
+    if (!m) return CHL_OK;
+    CHL_HIP(hipEventRecord(ev[6], stream));
+    const uint64_t newt = L.total[side] + m;
+    CHL_HIP(noff.reserve(n + 1));
+    if (nent.n < newt) {
+      nent.release();
+      CHL_HIP(nent.reserve(newt + newt / 2));
+    }
+    CHL_HIP(launch_merge_offsets(L.off[side].p, b.seg.p, (uint32_t)n, noff.p, stream));
+    CHL_HIP(launch_merge_rows(L.off[side].p, L.ent[side].p, b.seg.p, noff.p, b.sent.p, (uint32_t)n, nent.p, stream));
+    L.off[side].swap(noff);
+    L.ent[side].swap(nent);
+    L.total[side] = newt;
+    CHL_HIP(hipEventRecord(ev[7], stream));
+    CHL_HIP(hipEventSynchronize(ev[7]));
+    float t = 0;
+    CHL_HIP(hipEventElapsedTime(&t, ev[6], ev[7]));
+    ms += t;
+    return CHL_OK;
+  }
+
+  int run(LabelStore& L) {
+    const double t0 = now_s();
+    CHL_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto& e : ev) CHL_HIP(hipEventCreate(&e));
+    std::vector<uint32_t> rank(n);
+    for (int64_t i = 0; i < n; ++i) rank[order[i]] = (uint32_t)i;
+    int rc = upload_graph(rank);
+    if (rc) return rc;
+    if (o.delta <= 0) {
+      // default bucket width: twice the mean arc length
+      double sum = 0;
+      for (int32_t w : g.len[1]) sum += w;
+      double mean = g.len[1].empty() ? 1.0 : sum / (double)g.len[1].size();
+      o.delta = (int32_t)std::max(1.0, std::min(1e9, std::floor(2.0 * mean)));
+    }
+    const uint32_t W = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(n, o.batch_roots > 0 ? o.batch_roots : 8192));
+    rc = alloc_slots(o.slots > 0 ? (uint32_t)o.slots : 1024u);
+    if (rc) return rc;
+    CHL_HIP(roots.reserve(W));
+    CHL_HIP(tree_size.reserve(W));
+    CHL_HIP(label_count.reserve(W));
+    CHL_HIP(ctr.reserve(CTR_COUNT));
+    CHL_HIP(misc.reserve(4));
+    CHL_HIP(launch_fill_u64((uint64_t*)misc.p, 0, 4, stream));
+    out_cap = std::max<uint64_t>((uint64_t)1 << 20, (uint64_t)W * 64);
+    rc = L.init(n, sym, o.device);
+    if (rc) return rc;
+    if (stats) {
+      stats->switch_rank = -1;
+      stats->delta_used = o.delta;
+      stats->slots_used = slots;
+      stats->batch_used = W;
+    }
+
+    bool plant_phase = o.mode != CHL_MODE_GLL;
+    std::vector<uint32_t> hroots(W), hts(W), hlc(W);
+    double lt = 0, stt = 0, ct = 0;
+    for (int64_t r0 = 0; r0 < n;) {
+      const uint32_t R = (uint32_t)std::min<int64_t>(W, n - r0);
+      const uint32_t prefix = (uint32_t)r0;
+      for (uint32_t k = 0; k < R; ++k) hroots[k] = (uint32_t)(r0 + k);
+      if (hooks && hooks->order_roots) hooks->order_roots(hooks->ctx, hroots.data(), R);
+      CHL_HIP(hipMemcpyAsync(roots.p, hroots.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, stream));
+      uint32_t flags = 0;
+      if (plant_phase) flags = TREE_FLAGS | ((o.plant_prune && prefix > 0) ? TREE_PRUNE : 0u);
+      else flags = prefix > 0 ? TREE_PRUNE : 0u;
+      if (flags & TREE_PRUNE) {
+        uint64_t mr = 0;
+        for (int s = 0; s < nsides; ++s) {
+          CHL_HIP(launch_fill_u64((uint64_t*)misc.p + 1, 0, 1, stream));
+          CHL_HIP(launch_max_row(L.off[s].p, (uint32_t)n, misc.p + 1, stream));
+          unsigned long long h = 0;
+          CHL_HIP(hipMemcpyAsync(&h, misc.p + 1, sizeof(h), hipMemcpyDeviceToHost, stream));
+          CHL_HIP(hipStreamSynchronize(stream));
+          mr = std::max<uint64_t>(mr, h);
+        }
+        rc = ensure_gtab(mr);
+        if (rc) return rc;
+      }
+      for (int s = 0; s < nsides; ++s) {
+        rc = run_trees(s, R, prefix, flags, L, lt);
+        if (rc) return rc;
+        if (s == 0 && nsides == 2) {
+          // tree statistics are per side; keep the forward side for the hooks
+          CHL_HIP(hipMemcpy(hts.data(), tree_size.p, sizeof(uint32_t) * R, hipMemcpyDeviceToHost));
+          CHL_HIP(hipMemcpy(hlc.data(), label_count.p, sizeof(uint32_t) * R, hipMemcpyDeviceToHost));
+        }
+      }
+      if (nsides == 1) {
+        CHL_HIP(hipMemcpy(hts.data(), tree_size.p, sizeof(uint32_t) * R, hipMemcpyDeviceToHost));
+        CHL_HIP(hipMemcpy(hlc.data(), label_count.p, sizeof(uint32_t) * R, hipMemcpyDeviceToHost));
+      }
+      if (plant_phase && o.mode == CHL_MODE_HYBRID)
+        CHL_HIP(launch_switch_test(tree_size.p, label_count.p, R, o.switch_ratio, misc.p, stream));
+      for (int s = 0; s < nsides; ++s) {
+        rc = sort_side(s, stt);
+        if (rc) return rc;
+      }
+      if (!(flags & TREE_FLAGS)) {
+        for (int s = 0; s < nsides; ++s) {
+          rc = clean_side(s, ct);
+          if (rc) return rc;
+        }
+        for (int s = 0; s < nsides; ++s) {
+          rc = compact_side(s, ct);
+          if (rc) return rc;
+        }
+      }
+      for (int s = 0; s < nsides; ++s) {
+        rc = merge_side(s, L, ct);
+        if (rc) return rc;
+      }
+      if (stats) {
+        stats->batches++;
+        if (flags & TREE_FLAGS) stats->trees_plant += (int64_t)R * nsides;
+        else stats->trees_pruned += (int64_t)R * nsides;
+      }
+      if (hooks && hooks->on_batch) hooks->on_batch(hooks->ctx, hroots.data(), hts.data(), hlc.data(), R, flags);
+      if (plant_phase && o.mode == CHL_MODE_HYBRID) {
+        unsigned long long f = 0;
+        CHL_HIP(hipMemcpyAsync(&f, misc.p, sizeof(f), hipMemcpyDeviceToHost, stream));
+        CHL_HIP(hipStreamSynchronize(stream));
+        if (f) {
+          plant_phase = false;
+          if (stats) stats->switch_rank = r0 + R;
+        }
+      }
+      r0 += R;
+    }
+    CHL_HIP(hipStreamSynchronize(stream));
+    const double t1 = now_s();
+    if (stats) {
+      stats->lt = lt / 1e3;
+      stats->st = stt / 1e3;
+      stats->ct = ct / 1e3;
+      stats->total_s = t1 - t0;
+      stats->labels = (int64_t)(L.total[0] + (sym ? L.total[0] : L.total[1]));
+      stats->avg_label = (double)stats->labels / (double)n / 2.0;
+      stats->clean_ms += ct;
+    }
+    return CHL_OK;
+  }
+};
+
+int build_labels(const Graph& g, const std::vector<uint32_t>& order, const chl_build_opts& opts,
+                 chl_build_stats* stats, BuildHooks* hooks, LabelStore& L) {
+  Builder b(g, order, opts, stats, hooks);
+  return b.run(L);
+}
+
+int run_build(const Graph& g, const std::vector<uint32_t>& order, const chl_build_opts& opts, chl_build_stats* stats,
+              LabelStore& L) {
+  return build_labels(g, order, opts, stats, nullptr, L);
+}
+
+}  // namespace chl

@@ -1,0 +1,46 @@
+// Host-side interface of the device engine (label store, build, hooks).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "dev.hpp"
+
+namespace chl {
+
+// Labels resident on the device. Rows are indexed by row id (the rank for built
+// labels, the vertex id for labels read from a file) and hold (hub << 32) | dist
+// with hubs ascending. shared_sides: side 1 is side 0 (symmetric graph).
+struct LabelStore {
+  int64_t n = 0;
+  bool shared_sides = false;
+  int device = 0;
+  DBuf<uint64_t> off[2];
+  DBuf<uint64_t> ent[2];
+  uint64_t total[2] = {0, 0};
+  const uint64_t* d_off(int s) const { return off[shared_sides ? 0 : s].p; }
+  const uint64_t* d_ent(int s) const { return ent[shared_sides ? 0 : s].p; }
+  uint64_t side_total(int s) const { return total[shared_sides ? 0 : s]; }
+  int init(int64_t n, bool shared, int dev);
+  int download(int side, std::vector<uint64_t>& off, std::vector<uint64_t>& ent) const;
+};
+
+// Callbacks into the batching policy (static order, learned packing).
+struct BuildHooks {
+  void* ctx = nullptr;
+  // May permute roots[0..R) of the next batch (the set is fixed).
+  void (*order_roots)(void* ctx, uint32_t* roots, uint32_t R) = nullptr;
+  // Per-root measured tree sizes and label counts of a finished batch.
+  void (*on_batch)(void* ctx, const uint32_t* roots, const uint32_t* tree_size, const uint32_t* label_count,
+                   uint32_t R, uint32_t flags) = nullptr;
+  // One tree launch finished: device time and the launch counters (CTR_*).
+  void (*on_tree_launch)(void* ctx, int side, uint32_t R, float ms, const unsigned long long* ctr) = nullptr;
+};
+
+int ensure_device(int device);
+int build_labels(const Graph& g, const std::vector<uint32_t>& order, const chl_build_opts& opts,
+                 chl_build_stats* stats, BuildHooks* hooks, LabelStore& L);
+// Build under the batching policy of opts.batching (static rank order or learned packing).
+int run_build(const Graph& g, const std::vector<uint32_t>& order, const chl_build_opts& opts, chl_build_stats* stats,
+              LabelStore& L);
+
+}  // namespace chl

@@ -1,0 +1,328 @@
+// Graph ingest (DIMACS, METIS, SNAP, edgelist with autodetection), CSR build,
+// order files and label files for the CHL engine.
+//
+// Semantics follow the reference readers (shared_memory/graph.hpp:164-335):
+// the same formats are tried in the same order, ids are 1-based for DIMACS/METIS
+// and 0-based for SNAP/edgelist, out-of-range ids make a reader fail, and parallel
+// arcs keep their minimum length (graph.hpp:107-114). Where the reference would
+// abort later on a non-positive length (assert at shared_memory/dijkstra.hpp:91)
+// the engine rejects the graph at load time with CHL_E_WEIGHT.
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <set>
+
+#include "../../include/chl.h"
+#include "host.hpp"
+
+namespace chl {
+
+int host_threads() {
+  const char* e = getenv("CHL_HOST_THREADS");
+  if (e && atoi(e) > 0) return atoi(e);
+  unsigned h = std::thread::hardware_concurrency();
+  if (h == 0) h = 1;
+  return h > 16 ? 16 : (int)h;
+}
+
+// One range: rows of (head, min length) sorted by head.
+static void build_range(int64_t n, std::vector<Arc3>& a, bool by_tail, std::vector<uint64_t>& off,
+                        std::vector<uint32_t>& head, std::vector<int32_t>& len) {
+  // key (row, col, w) with row = tail for out-arcs, head for in-arcs
+  std::vector<uint64_t> cnt(n + 1, 0);
+  for (const Arc3& x : a) cnt[(by_tail ? x.u : x.v) + 1]++;
+  for (int64_t v = 0; v < n; ++v) cnt[v + 1] += cnt[v];
+  std::vector<uint64_t> pos(cnt.begin(), cnt.end() - 1);
+  std::vector<std::pair<uint32_t, int32_t>> tmp(a.size());
+  for (const Arc3& x : a) {
+    uint32_t r = by_tail ? x.u : x.v, c = by_tail ? x.v : x.u;
+    tmp[pos[r]++] = {c, x.w};
+  }
+  off.assign(n + 1, 0);
+  head.clear();
+  len.clear();
+  head.reserve(a.size());
+  len.reserve(a.size());
+  for (int64_t v = 0; v < n; ++v) {
+    auto b = tmp.begin() + cnt[v], e = tmp.begin() + cnt[v + 1];
+    std::sort(b, e);
+    for (auto it = b; it != e; ++it) {
+      if (it != b && it->first == (it - 1)->first) continue;  // keep the minimum length
+      head.push_back(it->first);
+      len.push_back(it->second);
+    }
+    off[v + 1] = head.size();
+  }
+}
+
+int graph_from_arcs(Graph& g, int64_t n, std::vector<Arc3>& arcs, bool undirected) {
+  if (n <= 0 || n >= (int64_t)1 << 30) return CHL_E_RANGE;  // Arc::head is 30 bits (graph.hpp:51)
+  g.n = n;
+  g.m_input = 0;
+  for (const Arc3& x : arcs) {
+    if ((int64_t)x.u >= n || (int64_t)x.v >= n) return CHL_E_RANGE;
+    if (x.w <= 0) return CHL_E_WEIGHT;
+  }
+  g.m_input = (int64_t)arcs.size() * (undirected ? 2 : 1);
+  if (undirected) {
+    size_t m = arcs.size();
+    arcs.reserve(2 * m);
+    for (size_t k = 0; k < m; ++k) arcs.push_back({arcs[k].v, arcs[k].u, arcs[k].w});
+  }
+  build_range(n, arcs, true, g.off[1], g.head[1], g.len[1]);
+  build_range(n, arcs, false, g.off[0], g.head[0], g.len[0]);
+  // symmetric iff the in-range equals the out-range row by row
+  g.symmetric = g.off[0] == g.off[1] && g.head[0] == g.head[1] && g.len[0] == g.len[1];
+  return CHL_OK;
+}
+
+// ---- readers ---------------------------------------------------------------
+
+static bool parse_dimacs(FILE* f, int64_t& n, std::vector<Arc3>& arcs, int& err) {
+  char buf[512];
+  bool inited = false;
+  n = 0;
+  while (fgets(buf, sizeof(buf), f)) {
+    size_t L = strlen(buf);
+    if (L == 0 || buf[L - 1] != '\n') {  // graph.hpp:175-177: only comments may be long or unterminated
+      if (buf[0] != 'c') return false;
+      int c;
+      while ((c = fgetc(f)) != EOF && c != '\n') {}
+      continue;
+    }
+    if (buf[0] == 'p') {
+      long long nn, mm;
+      if (inited || sscanf(buf, "p sp %lld %lld", &nn, &mm) != 2) return false;
+      inited = true;
+      n = nn;
+      if (mm > 0) arcs.reserve((size_t)mm);
+    } else if (buf[0] == 'a') {
+      long long u, v, w;
+      if (sscanf(buf, "a %lld %lld %lld", &u, &v, &w) != 3) return false;
+      if (u < 1 || v < 1 || u > n || v > n) return false;
+      if (w <= 0 || w > 2147483647LL) err = CHL_E_WEIGHT;
+      arcs.push_back({(uint32_t)(u - 1), (uint32_t)(v - 1), (int32_t)w});
+    } else if (buf[0] != 'c') {
+      return false;
+    }
+  }
+  return true;
+}
+
+static bool parse_metis(FILE* f, int64_t& n, std::vector<Arc3>& arcs, int& err) {
+  int c, i = 0, fmt = 0, ncon = 0, skip = 0;
+  long long elem = 0;
+  int64_t v = 0;
+  long long headv = -1;
+  bool inelem = false;
+  n = 0;
+  do {
+    c = fgetc(f);
+    if (c == '%' && i == 0 && !inelem) {  // comment line (graph.hpp:220)
+      while (c != '\n' && c != EOF) c = fgetc(f);
+      continue;
+    }
+    if (isdigit(c)) {
+      inelem = true;
+      elem = elem * 10 + (c - '0');
+    } else if (c == EOF || isspace(c)) {
+      if (inelem) {
+        if (v == 0) {
+          if (i == 0) n = elem;
+          else if (i == 2) {
+            if (elem % 10 > 1 || (elem / 10) % 10 > 1 || elem > 111) return false;
+            fmt = (int)elem;
+            skip = (fmt >= 100) + (fmt % 100 >= 10);
+          } else if (i == 3) {
+            if (fmt % 100 < 10) return false;
+            ncon = (int)elem;
+            skip = (fmt >= 100) + ncon;
+          } else if (i > 3) return false;
+        } else if (i >= skip) {
+          if (fmt % 10 == 1) {
+            if ((i - skip) % 2 == 0) headv = elem - 1;
+            else {
+              if (headv < 0 || headv >= n || v - 1 >= n) return false;
+              if (elem <= 0 || elem > 2147483647LL) err = CHL_E_WEIGHT;
+              arcs.push_back({(uint32_t)(v - 1), (uint32_t)headv, (int32_t)elem});
+            }
+          } else {
+            if (elem < 1 || elem > n || v - 1 >= n) return false;
+            arcs.push_back({(uint32_t)(v - 1), (uint32_t)(elem - 1), 1});
+          }
+        }
+        ++i;
+        elem = 0;
+        inelem = false;
+      }
+      if (c == '\n' || c == EOF) {
+        if (c == EOF && i == 0) break;
+        if (v > 0 && (i < skip || (fmt % 10 == 1 && (i - skip) % 2 == 1))) return false;
+        ++v;
+        i = 0;
+      }
+    } else {
+      return false;
+    }
+  } while (c != EOF);
+  return n > 0;
+}
+
+static bool parse_snap(FILE* f, int64_t& n, std::vector<Arc3>& arcs) {
+  char buf[512];
+  bool inited = false;
+  n = 0;
+  while (fgets(buf, sizeof(buf), f)) {
+    size_t L = strlen(buf);
+    if (buf[0] == '#') {
+      long long nn, mm;
+      if (sscanf(buf, "# Nodes: %lld Edges: %lld", &nn, &mm) == 2) {
+        if (inited) return false;
+        inited = true;
+        n = nn;
+      }
+      if (buf[L - 1] != '\n') {
+        int c;
+        while ((c = fgetc(f)) != EOF && c != '\n') {}
+      }
+    } else {
+      if (buf[L - 1] != '\n' && !feof(f)) return false;
+      long long u, v;
+      if (sscanf(buf, "%lld %lld", &u, &v) != 2) return false;
+      if (u < 0 || v < 0 || u >= n || v >= n) return false;
+      arcs.push_back({(uint32_t)u, (uint32_t)v, 1});
+    }
+  }
+  return true;
+}
+
+static bool parse_edgelist(FILE* f, int64_t& n, std::vector<Arc3>& arcs, int& err) {
+  char buf[512];
+  std::set<long long> ids;
+  while (fgets(buf, sizeof(buf), f)) {
+    size_t L = strlen(buf);
+    if (buf[L - 1] != '\n' && !feof(f)) return false;
+    long long u, v, w;
+    if (sscanf(buf, "%lld %lld %lld", &u, &v, &w) != 3) return false;
+    if (u < 0 || v < 0) return false;
+    if (w <= 0 || w > 2147483647LL) err = CHL_E_WEIGHT;
+    ids.insert(u);
+    ids.insert(v);
+    arcs.push_back({(uint32_t)u, (uint32_t)v, (int32_t)w});
+  }
+  n = (int64_t)ids.size();  // graph.hpp:297-304 counts distinct ids
+  for (const Arc3& a : arcs)
+    if ((int64_t)a.u >= n || (int64_t)a.v >= n) return false;  // ids must be contiguous
+  return true;
+}
+
+int graph_read(Graph& g, const char* path, bool undirected) {
+  FILE* f = fopen(path, "r");
+  if (!f) return CHL_E_IO;
+  for (int fmt = 1; fmt <= 4; ++fmt) {
+    rewind(f);
+    int64_t n = 0;
+    int err = CHL_OK;
+    std::vector<Arc3> arcs;
+    bool ok = false;
+    switch (fmt) {
+      case 1: ok = parse_dimacs(f, n, arcs, err); break;
+      case 2: ok = parse_metis(f, n, arcs, err); break;
+      case 3: ok = parse_snap(f, n, arcs); break;
+      case 4: ok = parse_edgelist(f, n, arcs, err); break;
+    }
+    if (!ok) continue;
+    fclose(f);
+    if (err != CHL_OK) return err;
+    int r = graph_from_arcs(g, n, arcs, undirected);
+    return r == CHL_OK ? fmt : r;
+  }
+  fclose(f);
+  return CHL_E_FORMAT;
+}
+
+int graph_write_dimacs(const Graph& g, const char* path) {
+  FILE* f = fopen(path, "w");
+  if (!f) return CHL_E_IO;
+  std::vector<char> big(1 << 22);
+  setvbuf(f, big.data(), _IOFBF, big.size());
+  fprintf(f, "p sp %lld %lld\n", (long long)g.n, (long long)g.head[1].size());
+  for (int64_t v = 0; v < g.n; ++v)
+    for (uint64_t a = g.off[1][v]; a < g.off[1][v + 1]; ++a)
+      fprintf(f, "a %lld %u %d\n", (long long)v + 1, g.head[1][a] + 1, g.len[1][a]);
+  int r = fclose(f);
+  return r == 0 ? CHL_OK : CHL_E_IO;
+}
+
+// ---- orders (ordering.hpp:37-67) -------------------------------------------
+
+int order_read(const char* path, std::vector<uint32_t>& order) {
+  FILE* f = fopen(path, "r");
+  if (!f) return CHL_E_IO;
+  long long s;
+  if (fscanf(f, "%lld", &s) != 1 || s < 0) { fclose(f); return CHL_E_FORMAT; }
+  order.resize((size_t)s);
+  for (long long i = 0; i < s; ++i) {
+    long long x;
+    if (fscanf(f, "%lld", &x) != 1 || x < 0) { fclose(f); return CHL_E_FORMAT; }
+    order[i] = (uint32_t)x;
+  }
+  fclose(f);
+  return CHL_OK;
+}
+
+int order_write(const char* path, const std::vector<uint32_t>& order) {
+  FILE* f = fopen(path, "w");
+  if (!f) return CHL_E_IO;
+  fprintf(f, "%zu\n", order.size());
+  for (uint32_t v : order) fprintf(f, "%u\n", v);
+  return fclose(f) == 0 ? CHL_OK : CHL_E_IO;
+}
+
+// ---- label files (labeling.hpp:167-208) ------------------------------------
+
+int labels_write(const Labels& L, const char* path) {
+  FILE* f = fopen(path, "w");
+  if (!f) return CHL_E_IO;
+  std::vector<char> big(1 << 22);
+  setvbuf(f, big.data(), _IOFBF, big.size());
+  fprintf(f, "%lld\n", (long long)L.n);
+  char line[64];
+  for (int64_t v = 0; v < L.n; ++v)
+    for (int s = 0; s < 2; ++s) {
+      fprintf(f, "%llu", (unsigned long long)(L.off[s][v + 1] - L.off[s][v]));
+      for (uint64_t k = L.off[s][v]; k < L.off[s][v + 1]; ++k) {
+        uint64_t e = L.ent[s][k];
+        int len = snprintf(line, sizeof(line), " %u %d", (uint32_t)(e >> 32), (int32_t)(uint32_t)e);
+        fwrite(line, 1, len, f);
+      }
+      fputc('\n', f);
+    }
+  return fclose(f) == 0 ? CHL_OK : CHL_E_IO;
+}
+
+int labels_read(Labels& L, const char* path) {
+  FILE* f = fopen(path, "r");
+  if (!f) return CHL_E_IO;
+  long long n;
+  if (fscanf(f, "%lld", &n) != 1 || n < 0) { fclose(f); return CHL_E_FORMAT; }
+  L.n = n;
+  for (int s = 0; s < 2; ++s) { L.off[s].assign(n + 1, 0); L.ent[s].clear(); }
+  for (long long v = 0; v < n; ++v)
+    for (int s = 0; s < 2; ++s) {
+      long long k;
+      if (fscanf(f, "%lld", &k) != 1 || k < 0) { fclose(f); return CHL_E_FORMAT; }
+      for (long long i = 0; i < k; ++i) {
+        long long h, d;
+        if (fscanf(f, "%lld %lld", &h, &d) != 2) { fclose(f); return CHL_E_FORMAT; }
+        L.ent[s].push_back(((uint64_t)(uint32_t)h << 32) | (uint32_t)(int32_t)d);
+      }
+      L.off[s][v + 1] = L.ent[s].size();
+    }
+  fclose(f);
+  return CHL_OK;
+}
+
+}  // namespace chl

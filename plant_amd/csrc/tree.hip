@@ -1,0 +1,353 @@
+// Multi-root shortest-path-tree kernel: the label-producing hot loop of PLaNT
+// (distributed_memory/akiba.hpp:52-104) and of GLL's pruned Dijkstra
+// (shared_memory/akiba.hpp:65-125), run for many roots per persistent launch.
+//
+// One workgroup owns one root at a time (it pulls the next root index from a
+// device cursor) and a private slot of scratch arrays sized n: the packed
+// state word of every vertex, a round stamp, two frontier lists and the list of
+// touched vertices. The search is a bucketed label-correcting Dijkstra (delta
+// stepping): each round processes the frontier entries whose distance is below
+// the bucket bound B and re-queues the rest; a vertex whose state improves is
+// queued once per round (stamp) and processed again, so every vertex is finally
+// processed at its final state.
+//
+// State word = (dist << 32) | low. With TREE_FLAGS, low is the smallest rank on
+// the best path found so far, including the vertex itself; a 64-bit atomicMin
+// then keeps the shortest distance and, among equal distances, the path through
+// the most important vertex. That is the tie rule of update_wo_prune
+// (distributed_memory/dijkstra.hpp:56-78): a label (root, d) is emitted iff the
+// final low equals the root's rank. Without TREE_FLAGS (GLL), low is the root
+// rank and vertices ranked above the root are not expanded (akiba.hpp:92).
+//
+// With TREE_PRUNE a processed vertex u at distance d is dropped when the
+// completed labels (all hubs < prefix) already give d(root, u) <= d: one
+// wavefront scans u's label row coalesced and probes each hub in the root's
+// label, held in an LDS hash table (akiba.hpp:94-110). A pruned vertex gets low
+// = 0 at that distance, so an equal-distance update cannot revive it.
+//
+// With TREE_FLAGS the search stops at a bucket boundary once no queued vertex
+// has low == root: every later vertex is then reached through a more important
+// vertex (the queue_left test, distributed_memory/akiba.hpp:95).
+#include "dev.hpp"
+
+namespace chl {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr uint32_t kLdsHashBits = 12;
+constexpr uint32_t kLdsHash = 1u << kLdsHashBits;  // entries of the LDS root-label table
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t hash_slot(uint32_t h, uint32_t shift) { return (h * 0x9E3779B1u) >> shift; }
+
+__device__ __forceinline__ uint32_t table_find(const uint32_t* key, const uint32_t* val, uint32_t shift,
+                                               uint32_t mask, uint32_t h) {
+  uint32_t s = hash_slot(h, shift);
+  for (;;) {
+    uint32_t k = key[s];
+    if (k == h) return val[s];
+    if (k == kEmpty) return kEmpty;
+    s = (s + 1) & mask;
+  }
+}
+
+__device__ __forceinline__ void table_insert(uint32_t* key, uint32_t* val, uint32_t shift, uint32_t mask,
+                                             uint32_t h, uint32_t d) {
+  uint32_t s = hash_slot(h, shift);
+  for (;;) {
+    if (atomicCAS(&key[s], kEmpty, h) == kEmpty) {
+      val[s] = d;
+      return;
+    }
+    s = (s + 1) & mask;
+  }
+}
+
+template <bool FLAGS, bool PRUNE>
+__global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
+  __shared__ uint32_t s_key[PRUNE ? kLdsHash : 1];
+  __shared__ uint32_t s_val[PRUNE ? kLdsHash : 1];
+  __shared__ uint32_t s_ri, s_fsize, s_nsize, s_nvis, s_below, s_clean, s_minnext, s_emit, s_cursor;
+  __shared__ unsigned long long s_base;
+  __shared__ unsigned long long s_ctr[4];
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63;
+  const uint32_t wave = tid >> 6;
+  const size_t n = p.n;
+  const size_t slot = blockIdx.x;
+  uint64_t* st = p.st + slot * n;
+  uint32_t* stamp = p.stamp + slot * n;
+  uint32_t* vis = p.vis + slot * n;
+
+  unsigned long long c_proc = 0, c_arcs = 0, c_prune = 0;
+  if (tid < 4) s_ctr[tid] = 0;
+
+  for (;;) {
+    if (tid == 0) {
+      unsigned long long r = atomicAdd(&p.ctr[CTR_NEXT], 1ull);
+      s_ri = r < p.R ? (uint32_t)r : kEmpty;
+    }
+    __syncthreads();
+    const uint32_t ri = s_ri;
+    if (ri == kEmpty) break;
+    const uint32_t root = p.roots[ri];
+    const uint32_t skip = FLAGS ? (PRUNE ? p.prefix : 0u) : root;
+    uint32_t* F = p.qa + slot * n;
+    uint32_t* N = p.qb + slot * n;
+
+    // Root label (all completed hubs) into the hash table.
+    uint32_t* tkey = s_key;
+    uint32_t* tval = s_val;
+    uint32_t tshift = 32 - kLdsHashBits, tmask = kLdsHash - 1;
+    if (PRUNE) {
+      const uint64_t rb = p.root_off[root], re = p.root_off[root + 1];
+      if (2 * (re - rb) > kLdsHash) {
+        tkey = p.gkey + slot * p.gcap;
+        tval = p.gval + slot * p.gcap;
+        tmask = p.gcap - 1;
+        tshift = 32 - (31 - __clz(p.gcap));
+      }
+      for (uint32_t i = tid; i <= tmask; i += kThreads) tkey[i] = kEmpty;
+      __syncthreads();
+      for (uint64_t k = rb + tid; k < re; k += kThreads) {
+        uint64_t e = p.root_ent[k];
+        table_insert(tkey, tval, tshift, tmask, (uint32_t)(e >> 32), (uint32_t)e);
+      }
+    }
+    if (tid == 0) {
+      st[root] = (uint64_t)root;  // dist 0, low = root rank
+      stamp[root] = 1;
+      F[0] = root;
+      vis[0] = root;
+      s_fsize = 1;
+      s_nvis = 1;
+      s_below = 1;
+      s_clean = 1;
+      s_minnext = 0;
+    }
+    uint32_t B = p.delta;
+    uint32_t round = 1;
+    __syncthreads();
+
+    for (;;) {
+      const uint32_t fsize = s_fsize;
+      if (fsize == 0) break;
+      if (s_below == 0) {
+        // bucket [.., B) is final; every queued vertex is at >= B
+        if (FLAGS && s_clean == 0) break;
+        B = s_minnext + p.delta;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        s_nsize = 0;
+        s_below = 0;
+        s_clean = 0;
+        s_minnext = kEmpty;
+      }
+      __syncthreads();
+      const uint32_t nstamp = round + 1;
+
+      if (!PRUNE) {
+        // one lane per frontier entry
+        for (uint32_t k = tid; k < fsize; k += kThreads) {
+          const uint32_t u = F[k];
+          const uint64_t x = st[u];
+          const uint32_t d = (uint32_t)(x >> 32);
+          const uint32_t lo = (uint32_t)x;
+          if (d >= B) {
+            if (atomicExch(&stamp[u], nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = u;
+            continue;
+          }
+          if (u < skip) continue;
+          ++c_proc;
+          const uint32_t ab = p.off[u], ae = p.off[u + 1];
+          c_arcs += ae - ab;
+          for (uint32_t a = ab; a < ae; ++a) {
+            const uint64_t arc = p.arc[a];
+            const uint32_t v = (uint32_t)(arc >> 32);
+            const uint32_t nd = d + (uint32_t)arc;
+            const uint32_t nlo = FLAGS ? min(lo, v) : lo;
+            const uint64_t nv = ((uint64_t)nd << 32) | nlo;
+            if (nv < st[v]) {
+              const uint64_t old = atomicMin((unsigned long long*)&st[v], (unsigned long long)nv);
+              if (nv < old) {
+                if (old == kStateInf) vis[atomicAdd(&s_nvis, 1u)] = v;
+                if (atomicExch(&stamp[v], nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = v;
+              }
+            }
+          }
+        }
+      } else {
+        // one wavefront per frontier entry: prune check, then relax
+        for (uint32_t k = wave; k < fsize; k += kWaves) {
+          const uint32_t u = F[k];
+          const uint64_t x = st[u];
+          const uint32_t d = (uint32_t)(x >> 32);
+          const uint32_t lo = (uint32_t)x;
+          if (d >= B) {
+            if (lane == 0 && atomicExch(&stamp[u], nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = u;
+            continue;
+          }
+          if (u < skip || lo == 0) continue;  // lo == 0: pruned at this distance already
+          if (lane == 0) ++c_proc;
+          bool pruned = false;
+          const uint64_t rb = p.scan_off[u], re = p.scan_off[u + 1];
+          uint64_t base = rb;
+          for (; base < re; base += 64) {
+            const uint64_t kk = base + lane;
+            bool hit = false;
+            if (kk < re) {
+              const uint64_t e = p.scan_ent[kk];
+              const uint32_t rd = table_find(tkey, tval, tshift, tmask, (uint32_t)(e >> 32));
+              hit = rd != kEmpty && rd + (uint32_t)e <= d;
+            }
+            if (__any(hit)) {
+              pruned = true;
+              break;
+            }
+          }
+          if (lane == 0) c_prune += (pruned ? base + 64 : re) - rb;
+          if (pruned) {
+            if (lane == 0) atomicMin((unsigned long long*)&st[u], (unsigned long long)((uint64_t)d << 32));
+            continue;
+          }
+          const uint32_t ab = p.off[u], ae = p.off[u + 1];
+          if (lane == 0) c_arcs += ae - ab;
+          for (uint32_t a = ab + lane; a < ae; a += 64) {
+            const uint64_t arc = p.arc[a];
+            const uint32_t v = (uint32_t)(arc >> 32);
+            const uint32_t nd = d + (uint32_t)arc;
+            const uint32_t nlo = FLAGS ? min(lo, v) : lo;
+            const uint64_t nv = ((uint64_t)nd << 32) | nlo;
+            if (nv < st[v]) {
+              const uint64_t old = atomicMin((unsigned long long*)&st[v], (unsigned long long)nv);
+              if (nv < old) {
+                if (old == kStateInf) vis[atomicAdd(&s_nvis, 1u)] = v;
+                if (atomicExch(&stamp[v], nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = v;
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
+
+      // Bucket bookkeeping over the next frontier, read at its current state.
+      const uint32_t nsize = s_nsize;
+      uint32_t below = 0, clean = 0, mn = kEmpty;
+      for (uint32_t k = tid; k < nsize; k += kThreads) {
+        const uint64_t x = st[N[k]];
+        const uint32_t d = (uint32_t)(x >> 32);
+        below |= d < B;
+        clean |= FLAGS && (uint32_t)x == root;
+        mn = min(mn, d);
+      }
+      if (below) s_below = 1;
+      if (clean) s_clean = 1;
+      if (mn != kEmpty) atomicMin(&s_minnext, mn);
+      if (tid == 0) s_fsize = nsize;
+      uint32_t* t = F;
+      F = N;
+      N = t;
+      ++round;
+      __syncthreads();
+    }
+
+    // Emit (root, d) at every touched vertex whose final low is the root, then
+    // restore the slot's scratch to its unreached state.
+    const uint32_t nvis = s_nvis;
+    if (tid == 0) {
+      s_emit = 0;
+      s_cursor = 0;
+    }
+    __syncthreads();
+    uint32_t mine = 0;
+    for (uint32_t k = tid; k < nvis; k += kThreads) {
+      const uint32_t v = vis[k];
+      mine += ((uint32_t)st[v] == root && v >= root) ? 1u : 0u;
+    }
+    if (mine) atomicAdd(&s_emit, mine);
+    __syncthreads();
+    const uint32_t emit = s_emit;
+    if (tid == 0) {
+      unsigned long long b = atomicAdd(&p.ctr[CTR_LABELS], (unsigned long long)emit);
+      if (b + emit > p.out_cap) {
+        atomicOr(&p.ctr[CTR_OVERFLOW], 1ull);
+        b = ~0ull;
+      }
+      s_base = b;
+      p.label_count[ri] = emit;
+    }
+    __syncthreads();
+    const unsigned long long obase = s_base;
+    for (uint32_t k = tid; k < nvis; k += kThreads) {
+      const uint32_t v = vis[k];
+      const uint64_t x = st[v];
+      if ((uint32_t)x == root && v >= root && obase != ~0ull) {
+        const unsigned long long o = obase + atomicAdd(&s_cursor, 1u);
+        p.out_key[o] = ((uint64_t)v << 32) | root;
+        p.out_ent[o] = ((uint64_t)root << 32) | (x >> 32);
+      }
+      st[v] = kStateInf;
+      stamp[v] = 0;
+    }
+    // tree size = processed entries of this root
+    if (c_proc) atomicAdd(&s_ctr[0], c_proc);
+    __syncthreads();
+    if (tid == 0) {
+      p.tree_size[ri] = (uint32_t)s_ctr[0];
+      atomicAdd(&s_ctr[1], s_ctr[0]);
+      s_ctr[0] = 0;
+    }
+    c_proc = 0;
+  }
+  if (c_arcs) atomicAdd(&s_ctr[2], c_arcs);
+  if (c_prune) atomicAdd(&s_ctr[3], c_prune);
+  __syncthreads();
+  if (tid == 0) {
+    if (s_ctr[1]) atomicAdd(&p.ctr[CTR_PROCESSED], s_ctr[1]);
+    if (s_ctr[2]) atomicAdd(&p.ctr[CTR_ARCS], s_ctr[2]);
+    if (s_ctr[3]) atomicAdd(&p.ctr[CTR_PRUNE_ENT], s_ctr[3]);
+  }
+}
+
+__global__ void fill_u64_kernel(uint64_t* p, uint64_t v, size_t count) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t count) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+static unsigned fill_blocks(size_t count) {
+  size_t b = (count + 255) / 256;
+  return (unsigned)(b < 8192 ? (b ? b : 1) : 8192);
+}
+
+}  // namespace
+
+hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, hipStream_t s) {
+  dim3 grid(slots), block(kThreads);
+  switch (flags & (TREE_FLAGS | TREE_PRUNE)) {
+    case TREE_FLAGS: hipLaunchKernelGGL((tree_kernel<true, false>), grid, block, 0, s, p); break;
+    case TREE_FLAGS | TREE_PRUNE: hipLaunchKernelGGL((tree_kernel<true, true>), grid, block, 0, s, p); break;
+    case TREE_PRUNE: hipLaunchKernelGGL((tree_kernel<false, true>), grid, block, 0, s, p); break;
+    default: hipLaunchKernelGGL((tree_kernel<false, false>), grid, block, 0, s, p); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t count, hipStream_t s) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(fill_blocks(count)), dim3(256), 0, s, p, v, count);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(fill_blocks(count)), dim3(256), 0, s, p, v, count);
+  return hipGetLastError();
+}
+
+}  // namespace chl

@@ -1,0 +1,148 @@
+"""GPU label construction and queries against the CPU oracle (bit-exact).
+
+The oracle's run_paraPLL restatement (oracle/chl_oracle.cpp) is itself pinned to
+the reference's own hhl build and to the CHL definition in test_oracle.py.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_same_labels(gpu, ref, sides=(0, 1)):
+    for side in sides:
+        a = gpu.copy(side)
+        b = ref.copy(side)
+        for name, x, y in zip(("off", "hub", "dist"), a, b):
+            if not np.array_equal(x, y):
+                bad = np.flatnonzero(np.asarray(x[: len(y)]) != np.asarray(y[: len(x)]))
+                raise AssertionError(f"side {side} {name}: sizes {len(x)} vs {len(y)}, first diff at {bad[:5]}")
+
+
+def oracle_of(chl, orc, g, order, raw=None):
+    if raw is None:
+        src, dst, w = g.arc_list(1)
+        og = orc.OGraph.from_arcs(g.n, src, dst, w)
+    else:
+        og = orc.OGraph.from_arcs(g.n, *raw)
+    ref, _ = orc.gll(og, order, threads=8)
+    return og, ref
+
+
+@pytest.fixture(scope="module")
+def grid(chl, orc):
+    g = chl.Graph.grid(30, 30, 1, 100, seed=11)
+    order = g.order_betweenness(32, seed=3)
+    og, ref = oracle_of(chl, orc, g, order)
+    return g, order, og, ref
+
+
+@pytest.mark.parametrize("mode,prune", [(1, 0), (1, 1), (0, 0), (2, 0)])
+def test_grid_modes(chl, grid, mode, prune):
+    g, order, _, ref = grid
+    L = chl.build(g, order, mode=mode, plant_prune=prune, batch_roots=128)
+    assert_same_labels(L, ref)
+
+
+@pytest.mark.parametrize("batch,delta,slots", [(1, 0, 0), (7, 1, 3), (900, 1000, 64), (64, 37, 1)])
+def test_grid_schedule_independent(chl, grid, batch, delta, slots):
+    g, order, _, ref = grid
+    for mode, prune in ((1, 0), (0, 0), (1, 1)):
+        L = chl.build(g, order, mode=mode, plant_prune=prune, batch_roots=batch, delta=delta, slots=slots)
+        assert_same_labels(L, ref)
+
+
+def test_hybrid_switches(chl, grid):
+    g, order, _, ref = grid
+    L = chl.build(g, order, mode=chl.MODE_HYBRID, switch_ratio=1.5, batch_roots=50)
+    assert L.stats.switch_rank > 0
+    assert L.stats.trees_pruned > 0
+    assert_same_labels(L, ref)
+
+
+def test_query_all_pairs_grid(chl, grid):
+    g, order, og, ref = grid
+    L = chl.build(g, order, batch_roots=256)
+    n = g.n
+    src = np.arange(0, n, 7, dtype=np.uint32)
+    D = og.dijkstra_multi(src)
+    pairs = np.stack(np.meshgrid(src, np.arange(n, dtype=np.uint32), indexing="ij"), -1).reshape(-1, 2)
+    got = L.query(pairs)
+    assert np.array_equal(got, D.reshape(-1))
+    assert np.array_equal(got, ref.query(pairs))
+
+
+def test_knn_road_like(chl, orc):
+    g = chl.Graph.knn(4000, 3, seed=5)
+    order = g.order_betweenness(64, seed=9)
+    _, ref = oracle_of(chl, orc, g, order)
+    for mode in (1, 0):
+        L = chl.build(g, order, mode=mode, batch_roots=512)
+        assert_same_labels(L, ref)
+
+
+def test_directed_random(chl, orc):
+    rng = np.random.default_rng(7)
+    n = 500
+    m = 3000
+    src = rng.integers(0, n, m).astype(np.uint32)
+    dst = rng.integers(0, n, m).astype(np.uint32)
+    w = rng.integers(1, 50, m).astype(np.int32)
+    g = chl.Graph.from_arcs(n, src, dst, w)
+    assert not g.symmetric
+    order = rng.permutation(n).astype(np.uint32)
+    _, ref = oracle_of(chl, orc, g, order, raw=(src, dst, w))
+    for mode, prune in ((1, 0), (1, 1), (0, 0), (2, 0)):
+        L = chl.build(g, order, mode=mode, plant_prune=prune, batch_roots=37)
+        assert_same_labels(L, ref)
+    og = orc.OGraph.from_arcs(n, src, dst, w)
+    pairs = rng.integers(0, n, (5000, 2)).astype(np.uint32)
+    D = og.dijkstra_multi(np.arange(n, dtype=np.uint32))
+    assert np.array_equal(L.query(pairs), D[pairs[:, 0], pairs[:, 1]])
+
+
+def test_rmat_unit_weights_degree_order(chl, orc):
+    g = chl.Graph.rmat(11, 3 * 2048, seed=4)
+    order = g.order_degree(seed=2)
+    _, ref = oracle_of(chl, orc, g, order)
+    for mode in (1, 0, 2):
+        L = chl.build(g, order, mode=mode, batch_roots=300)
+        assert_same_labels(L, ref)
+
+
+def test_disconnected_and_tiny(chl, orc):
+    # two components, an isolated vertex and a self-loop
+    src = np.array([0, 1, 2, 4, 5, 5], np.uint32)
+    dst = np.array([1, 2, 0, 5, 6, 5], np.uint32)
+    w = np.array([3, 4, 5, 1, 2, 9], np.int32)
+    g = chl.Graph.from_arcs(8, src, dst, w, undirected=True)
+    order = np.array([5, 2, 7, 0, 3, 1, 6, 4], np.uint32)
+    _, ref = oracle_of(chl, orc, g, order, raw=(np.r_[src, dst], np.r_[dst, src], np.r_[w, w]))
+    L = chl.build(g, order)
+    assert_same_labels(L, ref)
+    q = L.query(np.array([[0, 4], [3, 3], [4, 6], [6, 4], [0, 2]], np.uint32))
+    assert list(q) == [chl.INFTY, 0, 3, 3, 5]
+    g1 = chl.Graph.from_arcs(1, np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.int32))
+    L1 = chl.build(g1, np.array([0], np.uint32))
+    off, hub, dist = L1.copy(0)
+    assert list(off) == [0, 1] and list(hub) == [0] and list(dist) == [0]
+
+
+def test_query_rejects_out_of_range(chl, grid):
+    g, order, _, _ = grid
+    L = chl.build(g, order)
+    with pytest.raises(chl.ChlError):
+        L.query(np.array([[0, g.n]], np.uint32))
+
+
+def test_label_file_roundtrip(chl, grid, tmp_path):
+    g, order, _, ref = grid
+    L = chl.build(g, order)
+    p = tmp_path / "l.txt"
+    L.write(str(p))
+    ref.write(str(tmp_path / "r.txt"))
+    assert p.read_bytes() == (tmp_path / "r.txt").read_bytes()
+    L2 = chl.Labels.read(str(p))
+    assert_same_labels(L2, ref)
+    pairs = np.random.default_rng(1).integers(0, g.n, (3000, 2)).astype(np.uint32)
+    assert np.array_equal(L2.query(pairs), ref.query(pairs))
