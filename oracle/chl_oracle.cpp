@@ -519,11 +519,13 @@ static unsigned iteration_gll(Search& S, size_t i, bool forward, Locks& lck,
   return count;
 }
 
-struct GllTimes { double lt, st, ct; int64_t supersteps; };
+struct GllTimes { double lt, st, ct; int64_t supersteps; int64_t roots_done; };
 
 // shared_memory/akiba.hpp:132-217: GLL supersteps.
+// max_seconds > 0 stops after the first superstep at which lt + st + ct passes it
+// (a bounded sample of the build: the completed rank prefix and its labels).
 static void run_gll(const Graph& g, const std::vector<Vertex>& order, const std::vector<Vertex>& rev,
-                    Labeling& glob, int P, size_t label_limit, GllTimes* tm) {
+                    Labeling& glob, int P, size_t label_limit, GllTimes* tm, double max_seconds = 0) {
   const int64_t N = (int64_t)order.size();
   Labeling loc(N);
   Locks lck(2 * N + 2);
@@ -572,9 +574,13 @@ static void run_gll(const Graph& g, const std::vector<Vertex>& order, const std:
     double ce = omp_get_wtime();
     ct += ce - se;
     steps++;
+    if (max_seconds > 0 && lt + st + ct > max_seconds) break;
   }
   for (auto p : ak) delete p;
-  if (tm) { tm->lt = lt; tm->st = st; tm->ct = ct; tm->supersteps = steps; }
+  if (tm) {
+    tm->lt = lt; tm->st = st; tm->ct = ct; tm->supersteps = steps;
+    tm->roots_done = std::min<int64_t>(N, (int64_t)cnt.load() / 2);
+  }
 }
 
 // distributed_memory/akiba.hpp:52-104: PLaNT tree from root order[i]. `common`
@@ -918,6 +924,23 @@ int64_t orc_order_read(const char* path, uint32_t* out, int64_t cap) {
 }
 
 // GLL build (shared_memory/akiba.hpp:132-217) with P threads and sf (hhl.cpp:25,65).
+// Bounded GLL sample: supersteps until max_seconds of lt+st+ct; stats5 =
+// lt, st, ct, supersteps, roots claimed (a root is claimed by the superstep that
+// builds both its trees, so the labels cover ranks [0, roots)).
+void* orc_gll_bounded(void* gp, const uint32_t* order, int64_t n, int P, int sf, double max_seconds, double* stats5) {
+  Graph* g = (Graph*)gp;
+  std::vector<Vertex> ord, rev;
+  if (!make_order(*g, order, n, ord, rev)) return nullptr;
+  Labeling L(n);
+  GllTimes tm;
+  run_gll(*g, ord, rev, L, P, (size_t)sf * (size_t)n, &tm, max_seconds);
+  if (stats5) {
+    stats5[0] = tm.lt; stats5[1] = tm.st; stats5[2] = tm.ct; stats5[3] = (double)tm.supersteps;
+    stats5[4] = (double)tm.roots_done;
+  }
+  return to_flat(L);
+}
+
 void* orc_gll(void* gp, const uint32_t* order, int64_t n, int P, int sf, double* times4) {
   Graph* g = (Graph*)gp;
   std::vector<Vertex> ord, rev;

