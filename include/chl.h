@@ -84,6 +84,8 @@ int chl_order_load(const char* path, uint32_t* order, int64_t cap, int64_t* n);
 int chl_order_write(const char* path, const uint32_t* order, int64_t n);
 /* Sampled-betweenness order (external tool in the reference, README.md:24). */
 int chl_order_betweenness(const chl_graph* g, int64_t roots, uint64_t seed, uint32_t* order);
+/* Same order, plus each vertex's summed descendant count (the score) if score != NULL. */
+int chl_order_betweenness_scored(const chl_graph* g, int64_t roots, uint64_t seed, uint32_t* order, float* score);
 /* Degree order, distributed_memory/degree.cpp:62-67 when reference_ties=1. */
 int chl_order_degree(const chl_graph* g, uint64_t seed, int reference_ties, uint32_t* order);
 
@@ -102,6 +104,9 @@ typedef struct chl_build_opts {
   int plant_prune;      /* 1: PLaNT trees also prune by the completed rank-prefix labels */
   int device;           /* HIP device ordinal */
   int verbose;          /* 1: per-batch lines on stderr */
+  int64_t out_capacity; /* initial batch output buffer in labels (0 = automatic; it grows as needed) */
+  const float* vertex_score; /* optional per-vertex importance score (e.g. sampled betweenness), a
+                                feature of the learned batching predictor; NULL = not used */
   int reserved[8];
 } chl_build_opts;
 
@@ -121,6 +126,11 @@ typedef struct chl_build_stats {
   double relax_ms, clean_ms;    /* summed device time of the tree kernels / cleaning + append */
   double relax_bytes;           /* algorithmic bytes of the tree kernels */
   double mlp_ms;                /* device time of MLP training/inference */
+  /* learned batching: mean absolute error of log1p(tree size) per root, of the MLP
+     prediction and of the static estimate (the previous batch's mean), over the
+     batches the MLP predicted */
+  double pred_mae_learned, pred_mae_static;
+  int64_t pred_roots;           /* roots those errors cover */
   double reserved[8];
 } chl_build_stats;
 
@@ -155,6 +165,20 @@ void chl_labels_free(chl_labels* l);
 int chl_query(const chl_labels* l, const uint32_t* pairs, int64_t q, int32_t* out);
 /* Same on device pointers, enqueued on `stream` (a hipStream_t; NULL = default). */
 int chl_query_device(const chl_labels* l, const uint32_t* d_pairs, int64_t q, int32_t* d_out, void* stream);
+
+/* ---- tree-size predictor of learned batching (no reference counterpart; it
+ * predicts the (label_count, tree_size) pair of distributed_memory/akiba.hpp:99-103).
+ * An 8 -> 64 -> 64 -> 2 ReLU MLP, BF16 weights, FP32 accumulation, on MFMA.
+ * params: FP32 vector of chl_mlp_param_count() values laid out W1[8][64], b1[64],
+ * W2[64][64], b2[64], W3[64][2], b3[2]; x: rows of 8 features; y/t: rows of 2. */
+int chl_mlp_param_count(void);
+int chl_mlp_forward(const float* params, const float* x, int64_t n, float* y);
+/* grad = d/dparams of mean((MLP(x) - t)^2) over the n x 2 outputs; *loss = that mean */
+int chl_mlp_grad(const float* params, const float* x, const float* t, int64_t n, float* grad, float* loss);
+/* full-batch Adam (0.9, 0.999, 1e-8) for `steps` steps; params updated in place */
+int chl_mlp_train(float* params, const float* x, const float* t, int64_t n, int steps, float lr, float* loss_curve);
+/* one 32x32x8 BF16 matrix-core product D = A B (row-major; A 32x8, B 8x32 as BF16 bits) */
+int chl_mfma_probe(const uint16_t* a, const uint16_t* b, float* d);
 
 /* ---- serving (continuous batching over resident labels; extends the QLSN
  * latency/throughput modes of distributed_memory/query.hpp:62-91) ------------- */

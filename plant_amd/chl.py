@@ -36,7 +36,8 @@ class ChlError(RuntimeError):
 class BuildOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("batching", C.c_int), ("autotune", C.c_int), ("batch_roots", C.c_int64),
                 ("delta", C.c_int32), ("slots", C.c_int32), ("switch_ratio", C.c_double),
-                ("plant_prune", C.c_int), ("device", C.c_int), ("verbose", C.c_int), ("reserved", C.c_int * 8)]
+                ("plant_prune", C.c_int), ("device", C.c_int), ("verbose", C.c_int), ("out_capacity", C.c_int64),
+                ("vertex_score", C.c_void_p), ("reserved", C.c_int * 8)]
 
 
 class BuildStats(C.Structure):
@@ -47,6 +48,7 @@ class BuildStats(C.Structure):
                 ("prune_entries", C.c_int64), ("clean_entries", C.c_int64), ("delta_used", C.c_int64),
                 ("slots_used", C.c_int64), ("batch_used", C.c_int64), ("relax_ms", C.c_double),
                 ("clean_ms", C.c_double), ("relax_bytes", C.c_double), ("mlp_ms", C.c_double),
+                ("pred_mae_learned", C.c_double), ("pred_mae_static", C.c_double), ("pred_roots", C.c_int64),
                 ("reserved", C.c_double * 8)]
 
     def as_dict(self):
@@ -84,6 +86,7 @@ def lib():
             "chl_order_load": ([C.c_char_p, vp, i64, C.POINTER(i64)], C.c_int),
             "chl_order_write": ([C.c_char_p, vp, i64], C.c_int),
             "chl_order_betweenness": ([vp, i64, u64, vp], C.c_int),
+            "chl_order_betweenness_scored": ([vp, i64, u64, vp, vp], C.c_int),
             "chl_order_degree": ([vp, u64, C.c_int, vp], C.c_int),
             "chl_build_opts_default": ([C.POINTER(BuildOpts)], None),
             "chl_build": ([vp, vp, i64, C.POINTER(BuildOpts), pp, C.POINTER(BuildStats)], C.c_int),
@@ -97,6 +100,11 @@ def lib():
             "chl_labels_free": ([vp], None),
             "chl_query": ([vp, vp, i64, vp], C.c_int),
             "chl_query_device": ([vp, vp, i64, vp, vp], C.c_int),
+            "chl_mlp_param_count": ([], C.c_int),
+            "chl_mlp_forward": ([vp, vp, i64, vp], C.c_int),
+            "chl_mlp_grad": ([vp, vp, vp, i64, vp, vp], C.c_int),
+            "chl_mlp_train": ([vp, vp, vp, i64, C.c_int, C.c_float, vp], C.c_int),
+            "chl_mfma_probe": ([vp, vp, vp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -201,10 +209,14 @@ class Graph:
     def write(self, path):
         _check(lib().chl_graph_write(self._h, os.fsencode(path)))
 
-    def order_betweenness(self, roots, seed=7):
+    def order_betweenness(self, roots, seed=7, with_score=False):
         o = np.zeros(self.n, np.uint32)
-        _check(lib().chl_order_betweenness(self._h, roots, seed, _ptr(o)))
-        return o
+        if not with_score:
+            _check(lib().chl_order_betweenness(self._h, roots, seed, _ptr(o)))
+            return o
+        sc = np.zeros(self.n, np.float32)
+        _check(lib().chl_order_betweenness_scored(self._h, roots, seed, _ptr(o), _ptr(sc)))
+        return o, sc
 
     def order_degree(self, seed=7, reference_ties=False):
         o = np.zeros(self.n, np.uint32)
@@ -307,3 +319,46 @@ def build(graph, order, **opts):
 
 def device_count():
     return lib().chl_device_count()
+
+
+# ---- tree-size predictor (MFMA kernels) -------------------------------------
+MLP_F, MLP_H, MLP_O = 8, 64, 2
+
+
+def mlp_param_count():
+    return lib().chl_mlp_param_count()
+
+
+def mlp_forward(params, x):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_F)
+    y = np.zeros((x.shape[0], MLP_O), np.float32)
+    _check(lib().chl_mlp_forward(_ptr(params), _ptr(x), x.shape[0], _ptr(y)))
+    return y
+
+
+def mlp_grad(params, x, t):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_F)
+    t = np.ascontiguousarray(t, np.float32).reshape(-1, MLP_O)
+    g = np.zeros(mlp_param_count(), np.float32)
+    loss = C.c_float(0)
+    _check(lib().chl_mlp_grad(_ptr(params), _ptr(x), _ptr(t), x.shape[0], _ptr(g), C.byref(loss)))
+    return g, loss.value
+
+
+def mlp_train(params, x, t, steps, lr=3e-3):
+    params = np.array(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_F)
+    t = np.ascontiguousarray(t, np.float32).reshape(-1, MLP_O)
+    curve = np.zeros(max(steps, 1), np.float32)
+    _check(lib().chl_mlp_train(_ptr(params), _ptr(x), _ptr(t), x.shape[0], steps, lr, _ptr(curve)))
+    return params, curve[:steps]
+
+
+def mfma_probe(a_bits, b_bits):
+    a = np.ascontiguousarray(a_bits, np.uint16).reshape(32, 8)
+    b = np.ascontiguousarray(b_bits, np.uint16).reshape(8, 32)
+    d = np.zeros((32, 32), np.float32)
+    _check(lib().chl_mfma_probe(_ptr(a), _ptr(b), _ptr(d)))
+    return d

@@ -147,12 +147,19 @@ int chl_order_write(const char* path, const uint32_t* order, int64_t n) {
   return r == CHL_OK ? r : set_error(r, "cannot write order %s", path);
 }
 
-int chl_order_betweenness(const chl_graph* g, int64_t roots, uint64_t seed, uint32_t* order) {
+int chl_order_betweenness_scored(const chl_graph* g, int64_t roots, uint64_t seed, uint32_t* order, float* score) {
   if (!g || !order || roots <= 0) return set_error(CHL_E_INVAL, "bad arguments");
   std::vector<uint32_t> o;
-  order_betweenness(g->g, roots, seed, host_threads(), o, nullptr);
+  std::vector<uint64_t> sc;
+  order_betweenness(g->g, roots, seed, host_threads(), o, score ? &sc : nullptr);
   memcpy(order, o.data(), sizeof(uint32_t) * o.size());
+  if (score)
+    for (size_t v = 0; v < sc.size(); ++v) score[v] = (float)sc[v];
   return CHL_OK;
+}
+
+int chl_order_betweenness(const chl_graph* g, int64_t roots, uint64_t seed, uint32_t* order) {
+  return chl_order_betweenness_scored(g, roots, seed, order, nullptr);
 }
 
 int chl_order_degree(const chl_graph* g, uint64_t seed, int reference_ties, uint32_t* order) {

@@ -61,14 +61,20 @@ enum : uint32_t {
 static constexpr uint64_t kStateInf = ~0ull;
 
 // Counters of algorithmic work, summed over a launch.
+// CTR_LABELS: labels written; CTR_OVERFLOW: roots whose labels did not fit (their
+// indices go to spill[]); CTR_SPILL: labels those roots still have to write.
+// CTR_SPILL_START: smallest output offset reserved by a root that did not fit
+// (all ones when none), so [0, CTR_SPILL_START) is the written prefix.
 enum { CTR_PROCESSED = 0, CTR_ARCS = 1, CTR_PRUNE_ENT = 2, CTR_LABELS = 3, CTR_OVERFLOW = 4, CTR_NEXT = 5,
-       CTR_COUNT = 8 };
+       CTR_SPILL = 6, CTR_ROUNDS = 7, CTR_SPILL_START = 8, CTR_COUNT = 10 };
 
 struct TreeParams {
-  // graph in rank ids: arcs of the range walked by the tree (out-arcs for a
-  // forward tree), arc = (head << 32) | length
+  // graph in device ids (a locality order): arcs of the range walked by the tree
+  // (out-arcs for a forward tree), arc = (head << 32) | length; rank[id] = the
+  // vertex's position in the vertex order
   const uint32_t* off;
   const uint64_t* arc;
+  const uint32_t* rank;
   uint32_t n;
   // label store (rows in rank ids, hub-sorted, all hubs < prefix)
   const uint64_t* scan_off;  // side read at a visited vertex
@@ -76,13 +82,13 @@ struct TreeParams {
   const uint64_t* root_off;  // side read at the root
   const uint64_t* root_ent;
   uint32_t prefix;
-  // batch
+  // batch (roots as device ids)
   const uint32_t* roots;
   uint32_t R;
   uint32_t delta;
-  // per-slot scratch, slot = workgroup index; each array holds n entries per slot
-  uint64_t* st;
-  uint32_t* stamp;
+  // per-slot scratch, slot = workgroup index: cell holds 2 n words per slot
+  // (state word, stamp), the lists n entries per slot
+  uint64_t* cell;
   uint32_t* qa;
   uint32_t* qb;
   uint32_t* vis;
@@ -93,6 +99,7 @@ struct TreeParams {
   uint64_t* out_key;
   uint64_t* out_ent;
   uint64_t out_cap;
+  uint32_t* spill;  // [R] indices into roots whose labels did not fit
   // per root statistics and launch counters
   uint32_t* tree_size;
   uint32_t* label_count;
@@ -100,6 +107,8 @@ struct TreeParams {
 };
 
 hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, hipStream_t s);
+// count cells set to the unreached state (state all ones, stamp 0)
+hipError_t launch_init_cells(uint64_t* cell, size_t count, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t count, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s);
 

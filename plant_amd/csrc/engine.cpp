@@ -47,21 +47,50 @@ int ensure_device(int device) {
   return CHL_OK;
 }
 
-// Builds the rank-id CSR of one arc range.
-static void relabel_range(const Graph& g, int range, const std::vector<uint32_t>& order,
-                          const std::vector<uint32_t>& rank, std::vector<uint32_t>& off, std::vector<uint64_t>& arc) {
+// Locality order of the vertices: breadth-first over the arcs of both ranges,
+// starting from the most important vertex (then from the next unvisited one in
+// rank order), so that vertices close in the graph get close device ids and a
+// search front touches few cache lines. inv[id] = vertex.
+static void locality_order(const Graph& g, const std::vector<uint32_t>& order, std::vector<uint32_t>& inv) {
+  const int64_t n = g.n;
+  inv.clear();
+  inv.reserve(n);
+  std::vector<uint8_t> seen(n, 0);
+  for (int64_t r = 0; r < n; ++r) {
+    const uint32_t s = order[r];
+    if (seen[s]) continue;
+    seen[s] = 1;
+    size_t head = inv.size();
+    inv.push_back(s);
+    while (head < inv.size()) {
+      const uint32_t u = inv[head++];
+      for (int rg = 1; rg >= 0; --rg)
+        for (uint64_t a = g.off[rg][u]; a < g.off[rg][u + 1]; ++a) {
+          const uint32_t v = g.head[rg][a];
+          if (!seen[v]) {
+            seen[v] = 1;
+            inv.push_back(v);
+          }
+        }
+    }
+  }
+}
+
+// CSR of one arc range in device ids.
+static void relabel_range(const Graph& g, int range, const std::vector<uint32_t>& inv, const std::vector<uint32_t>& pos,
+                          std::vector<uint32_t>& off, std::vector<uint64_t>& arc) {
   const int64_t n = g.n;
   off.assign(n + 1, 0);
   for (int64_t i = 0; i < n; ++i) {
-    uint32_t v = order[i];
+    uint32_t v = inv[i];
     off[i + 1] = off[i] + (uint32_t)(g.off[range][v + 1] - g.off[range][v]);
   }
   arc.resize(off[n]);
   parallel_for(n, host_threads(), 4096, [&](int64_t i) {
-    uint32_t v = order[i];
+    uint32_t v = inv[i];
     uint64_t o = off[i];
     for (uint64_t a = g.off[range][v]; a < g.off[range][v + 1]; ++a)
-      arc[o++] = ((uint64_t)rank[g.head[range][a]] << 32) | (uint32_t)g.len[range][a];
+      arc[o++] = ((uint64_t)pos[g.head[range][a]] << 32) | (uint32_t)g.len[range][a];
   });
 }
 
@@ -110,10 +139,14 @@ struct Builder {
 
   DBuf<uint32_t> goff[2];
   DBuf<uint64_t> garc[2];
-  DBuf<uint64_t> st;
-  DBuf<uint32_t> stamp, qa, qb, vis, gkey, gval;
+  DBuf<uint64_t> cell;
+  DBuf<uint32_t> qa, qb, vis, gkey, gval, grank;
+  std::vector<uint32_t> id_of_rank;  // rank -> device id
+  std::vector<uint32_t> hids;        // staging of root ids
   uint32_t slots = 0, gcap = 0;
-  DBuf<uint32_t> roots, tree_size, label_count;
+  DBuf<uint32_t> roots, roots2, spill, tree_size, label_count;
+  uint32_t W = 1;            // widest batch (scratch offset of the statistics arrays)
+  int64_t stats_batches = 0;
   DBuf<unsigned long long> ctr;
   DBuf<unsigned long long> misc;  // [0] switch flag, [1] max row
   DBuf<uint64_t> noff, nent;
@@ -135,12 +168,23 @@ struct Builder {
   }
 
   int upload_graph(const std::vector<uint32_t>& rank) {
+    std::vector<uint32_t> inv, pos(n);
+    locality_order(g, order, inv);
+    for (int64_t i = 0; i < n; ++i) pos[inv[i]] = (uint32_t)i;
+    std::vector<uint32_t> rank_of_id(n);
+    id_of_rank.resize(n);
+    for (int64_t i = 0; i < n; ++i) {
+      rank_of_id[i] = rank[inv[i]];
+      id_of_rank[rank[inv[i]]] = (uint32_t)i;
+    }
+    CHL_HIP(grank.reserve(n));
+    CHL_HIP(hipMemcpy(grank.p, rank_of_id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
     for (int r = 0; r < 2; ++r) {
       if (sym && r == 0) continue;  // in-arcs equal out-arcs
       if (g.off[r][n] >= 0xFFFFFFFFull) return set_error(CHL_E_RANGE, "more than 2^32 arcs in a range");
       std::vector<uint32_t> off;
       std::vector<uint64_t> arc;
-      relabel_range(g, r, order, rank, off, arc);
+      relabel_range(g, r, inv, pos, off, arc);
       CHL_HIP(goff[r].reserve(n + 1));
       CHL_HIP(garc[r].reserve(arc.size()));
       CHL_HIP(hipMemcpy(goff[r].p, off.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
@@ -153,18 +197,16 @@ struct Builder {
   int alloc_slots(uint32_t want) {
     size_t freeb = 0, totalb = 0;
     CHL_HIP(hipMemGetInfo(&freeb, &totalb));
-    const size_t per = (size_t)n * 24;
+    const size_t per = (size_t)n * 28;  // 16-byte cell + three u32 lists per vertex
     size_t budget = freeb / 3;
     uint32_t fit = (uint32_t)std::max<size_t>(1, budget / std::max<size_t>(per, 1));
     slots = std::max<uint32_t>(1, std::min(want, fit));
     const size_t cells = (size_t)slots * n;
-    CHL_HIP(st.reserve(cells));
-    CHL_HIP(stamp.reserve(cells));
+    CHL_HIP(cell.reserve(2 * cells));
     CHL_HIP(qa.reserve(cells));
     CHL_HIP(qb.reserve(cells));
     CHL_HIP(vis.reserve(cells));
-    CHL_HIP(launch_fill_u64(st.p, kStateInf, cells, stream));
-    CHL_HIP(launch_fill_u32(stamp.p, 0, cells, stream));
+    CHL_HIP(launch_init_cells(cell.p, cells, stream));
     return CHL_OK;
   }
 
@@ -184,67 +226,111 @@ struct Builder {
     return CHL_OK;
   }
 
-  // Runs the trees of roots[0..R) for one side; returns labels emitted in b.m.
-  int run_trees(int side, uint32_t R, uint32_t prefix, uint32_t flags, LabelStore& L, double& ms) {
+  // Grows a batch output buffer to cap entries keeping its first `used` entries.
+  int grow_keep(DBuf<uint64_t>& buf, uint64_t cap, uint64_t used) {
+    if (buf.n >= cap) return CHL_OK;
+    DBuf<uint64_t> nb;
+    CHL_HIP(nb.reserve(cap));
+    if (used) CHL_HIP(hipMemcpyAsync(nb.p, buf.p, sizeof(uint64_t) * used, hipMemcpyDeviceToDevice, stream));
+    buf.swap(nb);
+    CHL_HIP(hipStreamSynchronize(stream));
+    return CHL_OK;
+  }
+
+  // Runs the trees of roots[0..R) for one side into the side's output buffer;
+  // b.m = labels written. Roots whose labels did not fit are run again, alone,
+  // after the buffer has grown by exactly what they reported.
+  int run_trees(int side, uint32_t R, const uint32_t* hroots, uint32_t prefix, uint32_t flags, LabelStore& L,
+                double& ms) {
     SideBatch& b = sb[side];
     const int range = side == 0 ? 1 : 0;  // forward tree walks out-arcs and fills side 0
-    for (;;) {
-      int rc = ensure_out(b, out_cap);
-      if (rc) return rc;
-      TreeParams p;
-      memset(&p, 0, sizeof(p));
-      p.off = goff[sym ? 1 : range].p;
-      p.arc = garc[sym ? 1 : range].p;
-      p.n = (uint32_t)n;
-      p.scan_off = L.d_off(side);
-      p.scan_ent = L.d_ent(side);
-      p.root_off = L.d_off(1 - side);
-      p.root_ent = L.d_ent(1 - side);
-      p.prefix = prefix;
-      p.roots = roots.p;
-      p.R = R;
-      p.delta = (uint32_t)o.delta;
-      p.st = st.p;
-      p.stamp = stamp.p;
-      p.qa = qa.p;
-      p.qb = qb.p;
-      p.vis = vis.p;
-      p.gkey = gkey.p;
-      p.gval = gval.p;
-      p.gcap = gcap;
-      p.out_key = b.key.p;
-      p.out_ent = b.ent.p;
-      p.out_cap = out_cap;
-      p.tree_size = tree_size.p;
-      p.label_count = label_count.p;
-      p.ctr = ctr.p;
-      CHL_HIP(launch_fill_u64((uint64_t*)ctr.p, 0, CTR_COUNT, stream));
+    int rc = ensure_out(b, out_cap);
+    if (rc) return rc;
+    TreeParams p;
+    memset(&p, 0, sizeof(p));
+    p.off = goff[sym ? 1 : range].p;
+    p.arc = garc[sym ? 1 : range].p;
+    p.n = (uint32_t)n;
+    p.scan_off = L.d_off(side);
+    p.scan_ent = L.d_ent(side);
+    p.root_off = L.d_off(1 - side);
+    p.root_ent = L.d_ent(1 - side);
+    p.prefix = prefix;
+    p.roots = roots.p;
+    p.R = R;
+    p.delta = (uint32_t)o.delta;
+    p.rank = grank.p;
+    p.cell = cell.p;
+    p.qa = qa.p;
+    p.qb = qb.p;
+    p.vis = vis.p;
+    p.gkey = gkey.p;
+    p.gval = gval.p;
+    p.gcap = gcap;
+    p.out_key = b.key.p;
+    p.out_ent = b.ent.p;
+    p.out_cap = std::min<uint64_t>(b.key.n, b.ent.n);
+    p.spill = spill.p;
+    p.tree_size = tree_size.p;
+    p.label_count = label_count.p;
+    p.ctr = ctr.p;
+    CHL_HIP(launch_fill_u64((uint64_t*)ctr.p, 0, CTR_COUNT, stream));
+    CHL_HIP(launch_fill_u64((uint64_t*)ctr.p + CTR_SPILL_START, ~0ull, 1, stream));
+    std::vector<uint32_t> cur(hroots, hroots + R), idx;
+    unsigned long long hc[CTR_COUNT];
+    for (int pass = 0;; ++pass) {
       CHL_HIP(hipEventRecord(ev[0], stream));
-      CHL_HIP(launch_tree(p, flags, std::min<uint32_t>(slots, R), stream));
+      CHL_HIP(launch_tree(p, flags, std::min<uint32_t>(slots, p.R), stream));
       CHL_HIP(hipEventRecord(ev[1], stream));
-      unsigned long long hc[CTR_COUNT];
       CHL_HIP(hipMemcpyAsync(hc, ctr.p, sizeof(hc), hipMemcpyDeviceToHost, stream));
       CHL_HIP(hipStreamSynchronize(stream));
       float t = 0;
       CHL_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
       ms += t;
-      if (hc[CTR_OVERFLOW]) {
-        out_cap = std::max<uint64_t>(out_cap * 2, hc[CTR_LABELS] + hc[CTR_LABELS] / 4);
-        continue;
-      }
-      b.m = hc[CTR_LABELS];
-      if (stats) {
-        stats->settled += (int64_t)hc[CTR_PROCESSED];
-        stats->arcs_scanned += (int64_t)hc[CTR_ARCS];
-        stats->prune_entries += (int64_t)hc[CTR_PRUNE_ENT];
-        stats->candidates += (int64_t)hc[CTR_LABELS];
-        stats->relax_ms += t;
-        stats->relax_bytes += 16.0 * hc[CTR_PROCESSED] + 16.0 * hc[CTR_ARCS] + 8.0 * hc[CTR_PRUNE_ENT] +
-                              16.0 * hc[CTR_LABELS];
-      }
-      if (hooks && hooks->on_tree_launch) hooks->on_tree_launch(hooks->ctx, side, R, t, hc);
-      return CHL_OK;
+      if (stats) stats->relax_ms += t;
+      if (o.verbose)
+        fprintf(stderr, "  side %d pass %d: %u roots, %.3f ms, written %llu, spilled %llu (%llu labels), processed %llu arcs %llu rounds %llu\n", side, pass, p.R, t, hc[CTR_LABELS], hc[CTR_OVERFLOW], hc[CTR_SPILL], hc[CTR_PROCESSED], hc[CTR_ARCS], hc[CTR_ROUNDS]);
+
+      if (!hc[CTR_OVERFLOW]) break;
+      // second launch for the spilled roots (their statistics are already recorded)
+      const uint32_t ns = (uint32_t)hc[CTR_OVERFLOW];
+      idx.resize(ns);
+      CHL_HIP(hipMemcpy(idx.data(), spill.p, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost));
+      std::vector<uint32_t> next(ns);
+      for (uint32_t j = 0; j < ns; ++j) next[j] = cur[idx[j]];
+      cur.swap(next);
+      const uint64_t written = hc[CTR_SPILL_START];  // contiguous prefix of fitting roots
+      const uint64_t need = written + hc[CTR_SPILL];
+      out_cap = std::max<uint64_t>(out_cap, need + need / 8);
+      rc = grow_keep(b.key, out_cap, written);
+      if (rc) return rc;
+      rc = grow_keep(b.ent, out_cap, written);
+      if (rc) return rc;
+      hids.resize(ns);
+      for (uint32_t j = 0; j < ns; ++j) hids[j] = id_of_rank[cur[j]];
+      CHL_HIP(hipMemcpy(roots2.p, hids.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
+      p.roots = roots2.p;
+      p.R = ns;
+      p.out_key = b.key.p;
+      p.out_ent = b.ent.p;
+      p.out_cap = out_cap;
+      p.tree_size = tree_size.p + W;  // scratch: the first pass holds these roots' statistics
+      p.label_count = label_count.p + W;
+      CHL_HIP(launch_fill_u64((uint64_t*)ctr.p + CTR_LABELS, written, 1, stream));
+      CHL_HIP(launch_fill_u64((uint64_t*)ctr.p + CTR_OVERFLOW, 0, 3, stream));  // OVERFLOW, NEXT, SPILL
+      CHL_HIP(launch_fill_u64((uint64_t*)ctr.p + CTR_SPILL_START, ~0ull, 1, stream));
     }
+    b.m = hc[CTR_LABELS];
+    if (stats) {
+      stats->settled += (int64_t)hc[CTR_PROCESSED];
+      stats->arcs_scanned += (int64_t)hc[CTR_ARCS];
+      stats->prune_entries += (int64_t)hc[CTR_PRUNE_ENT];
+      stats->candidates += (int64_t)hc[CTR_LABELS];
+      stats->relax_bytes +=
+          16.0 * hc[CTR_PROCESSED] + 16.0 * hc[CTR_ARCS] + 8.0 * hc[CTR_PRUNE_ENT] + 16.0 * hc[CTR_LABELS];
+    }
+    if (hooks && hooks->on_tree_launch) hooks->on_tree_launch(hooks->ctx, side, R, (float)ms, hc);
+    return CHL_OK;
   }
 
   int sort_side(int side, double& ms) {
@@ -358,16 +444,36 @@ struct Builder {
       double mean = g.len[1].empty() ? 1.0 : sum / (double)g.len[1].size();
       o.delta = (int32_t)std::max(1.0, std::min(1e9, std::floor(2.0 * mean)));
     }
-    const uint32_t W = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(n, o.batch_roots > 0 ? o.batch_roots : 8192));
     rc = alloc_slots(o.slots > 0 ? (uint32_t)o.slots : 1024u);
     if (rc) return rc;
+    // Static schedule: a fixed width when batch_roots is given; otherwise widths
+    // doubling from the slot count up to 65536 roots (later roots have small
+    // trees, and fewer batches mean fewer appends to the store).
+    const bool fixed = o.batch_roots > 0;
+    // (a batching policy may ask for batches of any width)
+    const uint32_t static_cap = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(n, fixed ? o.batch_roots : 65536));
+    W = (uint32_t)(hooks && hooks->next_width && !fixed ? n : static_cap);
+    uint32_t width = fixed ? static_cap : std::min<uint32_t>(static_cap, std::max<uint32_t>(slots, 1024));
     CHL_HIP(roots.reserve(W));
-    CHL_HIP(tree_size.reserve(W));
-    CHL_HIP(label_count.reserve(W));
+    CHL_HIP(roots2.reserve(W));
+    CHL_HIP(spill.reserve(W));
+    CHL_HIP(tree_size.reserve(2 * (size_t)W));
+    CHL_HIP(label_count.reserve(2 * (size_t)W));
     CHL_HIP(ctr.reserve(CTR_COUNT));
     CHL_HIP(misc.reserve(4));
     CHL_HIP(launch_fill_u64((uint64_t*)misc.p, 0, 4, stream));
-    out_cap = std::max<uint64_t>((uint64_t)1 << 20, (uint64_t)W * 64);
+    // first batch: the top roots label a large part of the graph
+    if (o.out_capacity > 0) {
+      out_cap = (uint64_t)o.out_capacity;
+    } else {
+      // the top roots label a large part of the graph: width * n / 4, bounded
+      // by an eighth of the free memory (16 bytes per label)
+      size_t freeb = 0, totalb = 0;
+      CHL_HIP(hipMemGetInfo(&freeb, &totalb));
+      const uint64_t want = (uint64_t)width * (uint64_t)n / 4;
+      out_cap = std::max<uint64_t>((uint64_t)1 << 20, std::min<uint64_t>(want, freeb / 8 / 16));
+    }
+    if (hooks) hooks->slots = slots;
     rc = L.init(n, sym, o.device);
     if (rc) return rc;
     if (stats) {
@@ -380,12 +486,37 @@ struct Builder {
     bool plant_phase = o.mode != CHL_MODE_GLL;
     std::vector<uint32_t> hroots(W), hts(W), hlc(W);
     double lt = 0, stt = 0, ct = 0;
+    if (plant_phase && hooks && hooks->pilot && hooks->on_pilot) {
+      // measured PLaNT trees of sampled roots (labels discarded; timed as tree time)
+      const uint32_t R = hooks->pilot(hooks->ctx, hroots.data(), std::min<uint32_t>(W, 4096));
+      if (R) {
+        hids.resize(R);
+        for (uint32_t k = 0; k < R; ++k) hids[k] = id_of_rank[hroots[k]];
+        CHL_HIP(hipMemcpyAsync(roots.p, hids.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, stream));
+        rc = run_trees(0, R, hroots.data(), 0, TREE_FLAGS, L, lt);
+        if (rc) return rc;
+        CHL_HIP(hipMemcpy(hts.data(), tree_size.p, sizeof(uint32_t) * R, hipMemcpyDeviceToHost));
+        CHL_HIP(hipMemcpy(hlc.data(), label_count.p, sizeof(uint32_t) * R, hipMemcpyDeviceToHost));
+        hooks->on_pilot(hooks->ctx, hroots.data(), hts.data(), hlc.data(), R);
+      }
+    }
     for (int64_t r0 = 0; r0 < n;) {
-      const uint32_t R = (uint32_t)std::min<int64_t>(W, n - r0);
+      uint32_t bw = width;
+      if (hooks && hooks->next_width) {
+        const uint32_t lw = hooks->next_width(hooks->ctx, r0, width, W);
+        if (lw) bw = std::min<uint32_t>(lw, W);
+      }
+      const uint32_t R = (uint32_t)std::min<int64_t>(bw, n - r0);
       const uint32_t prefix = (uint32_t)r0;
       for (uint32_t k = 0; k < R; ++k) hroots[k] = (uint32_t)(r0 + k);
       if (hooks && hooks->order_roots) hooks->order_roots(hooks->ctx, hroots.data(), R);
-      CHL_HIP(hipMemcpyAsync(roots.p, hroots.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, stream));
+      if (hooks && hooks->out_estimate) {
+        const uint64_t e = hooks->out_estimate(hooks->ctx, hroots.data(), R);
+        if (e) out_cap = std::max<uint64_t>(e, (uint64_t)1 << 16);
+      }
+      hids.resize(R);
+      for (uint32_t k = 0; k < R; ++k) hids[k] = id_of_rank[hroots[k]];
+      CHL_HIP(hipMemcpyAsync(roots.p, hids.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, stream));
       uint32_t flags = 0;
       if (plant_phase) flags = TREE_FLAGS | ((o.plant_prune && prefix > 0) ? TREE_PRUNE : 0u);
       else flags = prefix > 0 ? TREE_PRUNE : 0u;
@@ -403,7 +534,7 @@ struct Builder {
         if (rc) return rc;
       }
       for (int s = 0; s < nsides; ++s) {
-        rc = run_trees(s, R, prefix, flags, L, lt);
+        rc = run_trees(s, R, hroots.data(), prefix, flags, L, lt);
         if (rc) return rc;
         if (s == 0 && nsides == 2) {
           // tree statistics are per side; keep the forward side for the hooks
@@ -450,6 +581,21 @@ struct Builder {
           if (stats) stats->switch_rank = r0 + R;
         }
       }
+      if (o.verbose) {
+        uint64_t emitted = sb[0].kept + (nsides == 2 ? sb[1].kept : 0);
+        fprintf(stderr, "batch %lld: ranks [%lld, %lld) flags %u labels %llu store %llu\n", (long long)stats_batches,
+                (long long)r0, (long long)(r0 + R), flags, (unsigned long long)emitted,
+                (unsigned long long)L.total[0]);
+      }
+      ++stats_batches;
+      // output estimate for the next batch from this one's labels per root
+      {
+        const uint32_t nextR = fixed ? static_cap : std::min<uint32_t>(static_cap, width * 2);
+        uint64_t emitted = std::max(sb[0].kept, nsides == 2 ? sb[1].kept : 0);
+        uint64_t est = emitted / std::max<uint32_t>(R, 1) * nextR;
+        out_cap = std::max<uint64_t>(out_cap, est + est / 4);
+        width = nextR;
+      }
       r0 += R;
     }
     CHL_HIP(hipStreamSynchronize(stream));
@@ -471,11 +617,6 @@ int build_labels(const Graph& g, const std::vector<uint32_t>& order, const chl_b
                  chl_build_stats* stats, BuildHooks* hooks, LabelStore& L) {
   Builder b(g, order, opts, stats, hooks);
   return b.run(L);
-}
-
-int run_build(const Graph& g, const std::vector<uint32_t>& order, const chl_build_opts& opts, chl_build_stats* stats,
-              LabelStore& L) {
-  return build_labels(g, order, opts, stats, nullptr, L);
 }
 
 }  // namespace chl

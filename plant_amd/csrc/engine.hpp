@@ -34,6 +34,16 @@ struct BuildHooks {
                    uint32_t R, uint32_t flags) = nullptr;
   // One tree launch finished: device time and the launch counters (CTR_*).
   void (*on_tree_launch)(void* ctx, int side, uint32_t R, float ms, const unsigned long long* ctr) = nullptr;
+  // Width of the batch starting at rank r0 (at most max_width); 0 keeps the static width.
+  uint32_t (*next_width)(void* ctx, int64_t r0, uint32_t static_width, uint32_t max_width) = nullptr;
+  // Output buffer estimate (labels) for roots[0..R); 0 keeps the static estimate.
+  uint64_t (*out_estimate)(void* ctx, const uint32_t* roots, uint32_t R) = nullptr;
+  // Before the first batch (PLaNT trees only): ranks whose trees are measured and
+  // whose labels are discarded (returns the count, at most cap), and their sizes.
+  uint32_t (*pilot)(void* ctx, uint32_t* roots, uint32_t cap) = nullptr;
+  void (*on_pilot)(void* ctx, const uint32_t* roots, const uint32_t* tree_size, const uint32_t* label_count,
+                   uint32_t R) = nullptr;
+  uint32_t slots = 0;  // set by the builder before the first batch
 };
 
 int ensure_device(int device);

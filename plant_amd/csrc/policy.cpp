@@ -1,0 +1,487 @@
+// Batching policies of the label build.
+//
+// Static (CHL_BATCH_STATIC): the builder's own schedule, contiguous rank ranges
+// of doubling width taken in rank order, the GPU analogue of the reference's
+// supersteps (shared_memory/akiba.hpp:152-185, distributed_memory/akiba.hpp:259,480).
+//
+// Learned (CHL_BATCH_LEARNED): a small MLP (mlp.hip) predicts each remaining
+// root's tree size and label count from cheap per-root features and is trained
+// online on the sizes every finished batch measures. The predictions choose the
+// batch boundary (enough predicted work that the largest tree is a small part
+// of it), the order of roots inside the batch (largest predicted tree first, so
+// the persistent workgroups finish together) and the batch output capacity
+// (predicted labels, so batches rarely spill). Batches stay contiguous rank
+// ranges, so the labels are the same under either policy. Until the first batch
+// has been measured the static schedule is used, and the static estimate (the
+// previous batch's mean) is kept beside the MLP's to report both errors.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "dev.hpp"
+#include "engine.hpp"
+#include "internal.hpp"
+#include "mlp.hpp"
+
+namespace chl {
+namespace {
+
+struct Learned {
+  const Graph& g;
+  const std::vector<uint32_t>& order;
+  chl_build_opts o;
+  chl_build_stats* stats;
+  int64_t n;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  DBuf<float> X, P, M, V, G, Y, loss, T;
+  DBuf<uint16_t> wb;
+  DBuf<uint32_t> tidx, mb;
+  std::vector<float> hT;
+  std::vector<uint32_t> hidx;
+  uint64_t tcap = 65536, tcount = 0;
+  int step = 0;
+  bool trained = false;
+  int64_t pred_r0 = -1;
+  uint32_t pred_cnt = 0;
+  std::vector<float> pred;  // [pred_cnt][2]
+  double last_mean_log = -1.0;
+  double mae_l = 0, mae_s = 0;
+  int64_t mae_n = 0;
+  double mlp_ms = 0;
+  std::mt19937_64 rng{12345};
+  const BuildHooks* hooks = nullptr;  // the builder writes its slot count there
+  // labels one batch may write: an eighth of the free device memory at 32 bytes
+  // per label (output key and entry plus their sorted copies), at most 2^31 - 1
+  double label_budget = 268435456.0;
+
+  Learned(const Graph& gg, const std::vector<uint32_t>& ord, const chl_build_opts& opts, chl_build_stats* s)
+      : g(gg), order(ord), o(opts), stats(s), n(gg.n) {}
+  ~Learned() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int init() {
+    CHL_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    CHL_HIP(hipEventCreate(&e0));
+    CHL_HIP(hipEventCreate(&e1));
+    {
+      size_t freeb = 0, totalb = 0;
+      CHL_HIP(hipMemGetInfo(&freeb, &totalb));
+      label_budget = std::min<double>(2147483647.0, std::max<double>(1 << 22, (double)freeb / 8.0 / 32.0));
+    }
+    // features by rank: log rank, log degree, log 2-hop degree sum, log score
+    std::vector<double> f1(n), f2(n), f3(n, 0.0);
+    std::vector<uint64_t> deg(n);
+    for (int64_t v = 0; v < n; ++v)
+      deg[v] = (g.off[1][v + 1] - g.off[1][v]) + (g.off[0][v + 1] - g.off[0][v]);
+    parallel_for(n, host_threads(), 4096, [&](int64_t v) {
+      uint64_t two = 0;
+      for (uint64_t a = g.off[1][v]; a < g.off[1][v + 1]; ++a) two += deg[g.head[1][a]];
+      f1[v] = std::log1p((double)deg[v]);
+      f2[v] = std::log1p((double)two);
+      if (o.vertex_score) f3[v] = std::log1p(std::max(0.0, (double)o.vertex_score[v]));
+    });
+    auto standardize = [&](std::vector<double>& f) {
+      double mu = 0, var = 0;
+      for (double x : f) mu += x;
+      mu /= (double)n;
+      for (double x : f) var += (x - mu) * (x - mu);
+      const double sd = std::sqrt(var / (double)n);
+      for (double& x : f) x = sd > 0 ? (x - mu) / sd : 0.0;
+    };
+    standardize(f1);
+    standardize(f2);
+    standardize(f3);
+    std::vector<float> hx((size_t)n * kMlpF, 0.0f);
+    const double lr = std::log1p((double)n);
+    for (int64_t r = 0; r < n; ++r) {
+      const uint32_t v = order[r];
+      float* row = &hx[(size_t)r * kMlpF];
+      row[0] = (float)(std::log1p((double)r) / lr);
+      row[1] = (float)f1[v];
+      row[2] = (float)f2[v];
+      row[3] = (float)f3[v];
+    }
+    CHL_HIP(X.reserve(hx.size()));
+    CHL_HIP(hipMemcpy(X.p, hx.data(), sizeof(float) * hx.size(), hipMemcpyHostToDevice));
+    // parameters: uniform(+-sqrt(6/(fan_in+fan_out))) weights, zero biases
+    std::vector<float> hp(kMlpParams, 0.0f);
+    auto fill = [&](int off, int fin, int fout) {
+      std::uniform_real_distribution<float> u(-std::sqrt(6.0f / (fin + fout)), std::sqrt(6.0f / (fin + fout)));
+      for (int i = 0; i < fin * fout; ++i) hp[off + i] = u(rng);
+    };
+    fill(kMlpOffW1, 4, kMlpH);
+    for (int k = 4; k < kMlpF; ++k)
+      for (int c = 0; c < kMlpH; ++c) hp[kMlpOffW1 + k * kMlpH + c] = 0.0f;
+    fill(kMlpOffW2, kMlpH, kMlpH);
+    fill(kMlpOffW3, kMlpH, kMlpO);
+    CHL_HIP(P.reserve(kMlpParams));
+    CHL_HIP(M.reserve(kMlpParams));
+    CHL_HIP(V.reserve(kMlpParams));
+    CHL_HIP(G.reserve(kMlpParams));
+    CHL_HIP(wb.reserve(kMlpParams));
+    CHL_HIP(loss.reserve(1));
+    CHL_HIP(hipMemcpy(P.p, hp.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    CHL_HIP(hipMemset(M.p, 0, sizeof(float) * kMlpParams));
+    CHL_HIP(hipMemset(V.p, 0, sizeof(float) * kMlpParams));
+    CHL_HIP(mlp_sync(P.p, wb.p, stream));
+    CHL_HIP(tidx.reserve(tcap));
+    CHL_HIP(T.reserve(tcap * kMlpO));
+    CHL_HIP(mb.reserve(1024));
+    CHL_HIP(Y.reserve(65536 * kMlpO));
+    hT.resize(tcap * kMlpO);
+    hidx.resize(tcap);
+    CHL_HIP(hipStreamSynchronize(stream));
+    return CHL_OK;
+  }
+
+  void timed_begin() { (void)hipEventRecord(e0, stream); }
+  void timed_end() {
+    (void)hipEventRecord(e1, stream);
+    (void)hipEventSynchronize(e1);
+    float t = 0;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    mlp_ms += t;
+  }
+
+  // Predictions for ranks [r0, r0 + cnt).
+  int predict(int64_t r0, uint32_t cnt) {
+    if (pred_r0 == r0 && pred_cnt >= cnt) return CHL_OK;
+    CHL_HIP(Y.reserve((size_t)cnt * kMlpO));
+    timed_begin();
+    CHL_HIP(mlp_forward(X.p + (size_t)r0 * kMlpF, (int)cnt, wb.p, P.p, Y.p, stream));
+    pred.resize((size_t)cnt * kMlpO);
+    CHL_HIP(hipMemcpyAsync(pred.data(), Y.p, sizeof(float) * cnt * kMlpO, hipMemcpyDeviceToHost, stream));
+    timed_end();
+    for (uint32_t k = 0; k < cnt; ++k)
+      for (int o = 0; o < kMlpO; ++o) pred[(size_t)k * kMlpO + o] += (float)trend(o, (uint32_t)(r0 + k));
+    pred_r0 = r0;
+    pred_cnt = cnt;
+    return CHL_OK;
+  }
+
+  float pred_tree(uint32_t rank) const { return pred[(size_t)(rank - pred_r0) * kMlpO]; }
+  float pred_labels(uint32_t rank) const { return pred[(size_t)(rank - pred_r0) * kMlpO + 1]; }
+
+  uint32_t next_width(int64_t r0, uint32_t static_width, uint32_t max_width) {
+    (void)static_width;
+    if (!trained) return 0;
+    const uint32_t cnt = (uint32_t)std::min<int64_t>(max_width, n - r0);
+    if (predict(r0, cnt)) return 0;
+    // the widest batch whose predicted labels fit the output budget (every batch
+    // boundary is a device-wide wait for the batch's last tree), and at least one
+    // root per slot
+    const uint32_t slots = std::max<uint32_t>(1, hooks ? hooks->slots : 1);
+    double labels = 0;
+    uint32_t R = 0;
+    while (R < cnt) {
+      labels += std::expm1((double)std::max(0.0f, pred_labels((uint32_t)(r0 + R))));
+      if (2.0 * labels > label_budget && R >= slots) break;  // 2x margin on the prediction
+      ++R;
+    }
+    return R;
+  }
+
+  void order_roots(uint32_t* roots, uint32_t R) {
+    if (!trained || pred_r0 < 0 || roots[0] < pred_r0 || roots[0] + R > pred_r0 + pred_cnt) return;
+    std::stable_sort(roots, roots + R, [&](uint32_t a, uint32_t b) { return pred_tree(a) > pred_tree(b); });
+  }
+
+  // The whole budget once the batch is predicted to need more than a few million
+  // labels (the buffer is kept across batches), else twice the prediction.
+  uint64_t out_estimate(const uint32_t* roots, uint32_t R) {
+    if (!trained || pred_r0 < 0) return 0;
+    double s = 0;
+    for (uint32_t k = 0; k < R; ++k) {
+      if (roots[k] < pred_r0 || roots[k] >= pred_r0 + pred_cnt) return 0;
+      s += std::expm1((double)std::max(0.0f, pred_labels(roots[k])));
+    }
+    return (uint64_t)std::min(label_budget, std::max(2.0 * s, s > 4e6 ? label_budget : 0.0)) + 4096;
+  }
+
+  // Pilot sample: 512 ranks log-uniform over [max(1024, n/128), n) (trees there are
+  // cheap), so the first real batch is already packed by prediction; the top
+  // ranks are left to the trend.
+  uint32_t pilot(uint32_t* roots, uint32_t cap) {
+    const int64_t lo = std::min<int64_t>(std::max<int64_t>(1024, n / 128), n / 2);
+    if (n - lo < 64) return 0;
+    const uint32_t R = (uint32_t)std::min<int64_t>(std::min<int64_t>(cap, 512), n - lo);
+    std::uniform_real_distribution<double> u(std::log((double)lo + 1.0), std::log((double)n));
+    for (uint32_t k = 0; k < R; ++k) {
+      int64_t r = (int64_t)std::floor(std::exp(u(rng))) - 1;
+      roots[k] = (uint32_t)std::min<int64_t>(n - 1, std::max<int64_t>(lo, r));
+    }
+    return R;
+  }
+
+  int on_pilot(const uint32_t* roots, const uint32_t* ts, const uint32_t* lc, uint32_t R) {
+    // the static estimate of the first batch: the sample mean
+    double mean_log = 0;
+    for (uint32_t k = 0; k < R; ++k) mean_log += std::log1p((double)ts[k]);
+    last_mean_log = R ? mean_log / R : -1.0;
+    for (uint32_t k = 0; k < R; ++k) {
+      const uint64_t pos = tcount % tcap;
+      hidx[pos] = roots[k];
+      hT[pos * kMlpO] = (float)std::log1p((double)ts[k]);
+      hT[pos * kMlpO + 1] = (float)std::log1p((double)lc[k]);
+      ++tcount;
+    }
+    fit_trend();
+    for (int rep = 0; rep < 3; ++rep) {
+      int rc = train();
+      if (rc) return rc;
+    }
+    return CHL_OK;
+  }
+
+  int on_batch(const uint32_t* roots, const uint32_t* ts, const uint32_t* lc, uint32_t R) {
+    // prediction error before learning from this batch
+    double mean_log = 0;
+    for (uint32_t k = 0; k < R; ++k) mean_log += std::log1p((double)ts[k]);
+    mean_log /= std::max<uint32_t>(R, 1);
+    if (trained && last_mean_log >= 0) {
+      bool covered = true;
+      for (uint32_t k = 0; k < R && covered; ++k)
+        covered = roots[k] >= pred_r0 && roots[k] < pred_r0 + pred_cnt;
+      if (covered) {
+        for (uint32_t k = 0; k < R; ++k) {
+          const double y = std::log1p((double)ts[k]);
+          mae_l += std::fabs(pred_tree(roots[k]) - y);
+          mae_s += std::fabs(last_mean_log - y);
+        }
+        mae_n += R;
+      }
+    }
+    last_mean_log = mean_log;
+    // append (rank, targets) to the sample ring
+    for (uint32_t k = 0; k < R; ++k) {
+      const uint64_t pos = tcount % tcap;
+      hidx[pos] = roots[k];
+      hT[pos * kMlpO] = (float)std::log1p((double)ts[k]);
+      hT[pos * kMlpO + 1] = (float)std::log1p((double)lc[k]);
+      ++tcount;
+    }
+    fit_trend();
+    return train();
+  }
+
+  // Least-squares line of each target against log1p(rank) over the most recent
+  // samples. The MLP learns the residual from this trend, so that predictions for
+  // ranks beyond the measured ones follow the trend instead of extrapolating the
+  // network.
+  double ta[kMlpO] = {0, 0}, tb[kMlpO] = {0, 0};
+  double trend(int o, uint32_t rank) const { return ta[o] + tb[o] * std::log1p((double)rank); }
+  void fit_trend() {
+    const uint64_t avail = std::min<uint64_t>(tcount, tcap);
+    const uint64_t recent = std::min<uint64_t>(avail, 16384);
+    for (int o = 0; o < kMlpO; ++o) {
+      double sx = 0, sy = 0, sxx = 0, sxy = 0;
+      for (uint64_t b = 0; b < recent; ++b) {
+        const uint64_t pos = (tcount - 1 - b) % tcap;
+        const double x = std::log1p((double)hidx[pos]), y = hT[pos * kMlpO + o];
+        sx += x;
+        sy += y;
+        sxx += x * x;
+        sxy += x * y;
+      }
+      const double m = (double)recent, den = m * sxx - sx * sx;
+      tb[o] = den > 1e-9 ? (m * sxy - sx * sy) / den : 0.0;
+      ta[o] = (sy - tb[o] * sx) / m;
+    }
+  }
+
+  // kSteps Adam steps on minibatches of up to 512 samples drawn from the most
+  // recent 16384; the minibatches (sample ranks and targets) are drawn on the
+  // host from the mirror of the sample ring and uploaded in one copy each.
+  static constexpr int kSteps = 60;
+  int train() {
+    const uint64_t avail = std::min<uint64_t>(tcount, tcap);
+    const uint64_t recent = std::min<uint64_t>(avail, 16384);
+    const int B = (int)std::min<uint64_t>(512, avail);
+    std::vector<uint32_t> rk((size_t)kSteps * B);
+    std::vector<float> tv((size_t)kSteps * B * kMlpO);
+    for (size_t k = 0; k < rk.size(); ++k) {
+      const uint64_t back = rng() % recent;  // 0 = newest
+      const uint64_t pos = (tcount - 1 - back) % tcap;
+      rk[k] = hidx[pos];
+      for (int o = 0; o < kMlpO; ++o) tv[k * kMlpO + o] = (float)(hT[pos * kMlpO + o] - trend(o, hidx[pos]));
+    }
+    CHL_HIP(mb.reserve(rk.size()));
+    CHL_HIP(T.reserve(std::max<size_t>(tv.size(), tcap * kMlpO)));
+    CHL_HIP(hipMemcpyAsync(mb.p, rk.data(), sizeof(uint32_t) * rk.size(), hipMemcpyHostToDevice, stream));
+    CHL_HIP(hipMemcpyAsync(T.p, tv.data(), sizeof(float) * tv.size(), hipMemcpyHostToDevice, stream));
+    timed_begin();
+    for (int s = 0; s < kSteps; ++s) {
+      CHL_HIP(launch_fill_u32((uint32_t*)G.p, 0, kMlpParams, stream));
+      CHL_HIP(launch_fill_u32((uint32_t*)loss.p, 0, 1, stream));
+      CHL_HIP(mlp_grad(X.p, T.p + (size_t)s * B * kMlpO, mb.p + (size_t)s * B, B, wb.p, P.p, G.p, loss.p, stream));
+      CHL_HIP(mlp_adam(P.p, G.p, M.p, V.p, wb.p, 3e-3f, ++step, stream));
+    }
+    timed_end();
+    trained = true;
+    pred_r0 = -1;  // weights changed
+    return CHL_OK;
+  }
+};
+
+uint32_t hk_next_width(void* c, int64_t r0, uint32_t sw, uint32_t mw) {
+  return ((Learned*)c)->next_width(r0, sw, mw);
+}
+void hk_order(void* c, uint32_t* roots, uint32_t R) { ((Learned*)c)->order_roots(roots, R); }
+uint64_t hk_out(void* c, const uint32_t* roots, uint32_t R) { return ((Learned*)c)->out_estimate(roots, R); }
+uint32_t hk_pilot(void* c, uint32_t* roots, uint32_t cap) { return ((Learned*)c)->pilot(roots, cap); }
+void hk_on_pilot(void* c, const uint32_t* roots, const uint32_t* ts, const uint32_t* lc, uint32_t R) {
+  (void)((Learned*)c)->on_pilot(roots, ts, lc, R);
+}
+void hk_batch(void* c, const uint32_t* roots, const uint32_t* ts, const uint32_t* lc, uint32_t R, uint32_t) {
+  Learned* L = (Learned*)c;
+  (void)L->on_batch(roots, ts, lc, R);
+}
+
+}  // namespace
+
+int run_build(const Graph& g, const std::vector<uint32_t>& order, const chl_build_opts& opts, chl_build_stats* stats,
+              LabelStore& L) {
+  if (opts.batching != CHL_BATCH_LEARNED) return build_labels(g, order, opts, stats, nullptr, L);
+  Learned pol(g, order, opts, stats);
+  int rc = pol.init();
+  if (rc) return rc;
+  BuildHooks h;
+  h.ctx = &pol;
+  h.next_width = hk_next_width;
+  h.order_roots = hk_order;
+  h.out_estimate = hk_out;
+  h.on_batch = hk_batch;
+  h.pilot = hk_pilot;
+  h.on_pilot = hk_on_pilot;
+  pol.hooks = &h;
+  rc = build_labels(g, order, opts, stats, &h, L);
+  if (stats) {
+    stats->mlp_ms = pol.mlp_ms;
+    stats->pred_roots = pol.mae_n;
+    stats->pred_mae_learned = pol.mae_n ? pol.mae_l / pol.mae_n : 0.0;
+    stats->pred_mae_static = pol.mae_n ? pol.mae_s / pol.mae_n : 0.0;
+  }
+  return rc;
+}
+
+}  // namespace chl
+
+// ---- C ABI of the predictor kernels (host pointers; used by the tests) --------
+using namespace chl;
+
+namespace {
+struct MlpDev {
+  DBuf<float> P, X, T, Y, G, loss, M, V;
+  DBuf<uint16_t> wb;
+  DBuf<uint32_t> idx;
+  int load(const float* params, const float* x, int64_t N) {
+    CHL_HIP(P.reserve(kMlpParams));
+    CHL_HIP(wb.reserve(kMlpParams));
+    CHL_HIP(hipMemcpy(P.p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    CHL_HIP(mlp_sync(P.p, wb.p, nullptr));
+    CHL_HIP(X.reserve((size_t)std::max<int64_t>(N, 1) * kMlpF));
+    if (N) CHL_HIP(hipMemcpy(X.p, x, sizeof(float) * N * kMlpF, hipMemcpyHostToDevice));
+    return CHL_OK;
+  }
+};
+}  // namespace
+
+extern "C" {
+
+int chl_mlp_param_count(void) { return kMlpParams; }
+
+int chl_mlp_forward(const float* params, const float* x, int64_t n, float* y) {
+  if (!params || n < 0 || (n && (!x || !y)) || n > (1 << 30)) return set_error(CHL_E_INVAL, "bad arguments");
+  int rc = ensure_device(0);
+  if (rc) return rc;
+  MlpDev d;
+  if ((rc = d.load(params, x, n))) return rc;
+  CHL_HIP(d.Y.reserve((size_t)std::max<int64_t>(n, 1) * kMlpO));
+  CHL_HIP(mlp_forward(d.X.p, (int)n, d.wb.p, d.P.p, d.Y.p, nullptr));
+  if (n) CHL_HIP(hipMemcpy(y, d.Y.p, sizeof(float) * n * kMlpO, hipMemcpyDeviceToHost));
+  return CHL_OK;
+}
+
+int chl_mlp_grad(const float* params, const float* x, const float* t, int64_t n, float* grad, float* loss) {
+  if (!params || !x || !t || !grad || n <= 0 || n > (1 << 20)) return set_error(CHL_E_INVAL, "bad arguments");
+  int rc = ensure_device(0);
+  if (rc) return rc;
+  MlpDev d;
+  if ((rc = d.load(params, x, n))) return rc;
+  std::vector<uint32_t> id(n);
+  std::iota(id.begin(), id.end(), 0u);
+  CHL_HIP(d.idx.reserve(n));
+  CHL_HIP(hipMemcpy(d.idx.p, id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+  CHL_HIP(d.T.reserve((size_t)n * kMlpO));
+  CHL_HIP(hipMemcpy(d.T.p, t, sizeof(float) * n * kMlpO, hipMemcpyHostToDevice));
+  CHL_HIP(d.G.reserve(kMlpParams));
+  CHL_HIP(d.loss.reserve(1));
+  CHL_HIP(hipMemset(d.G.p, 0, sizeof(float) * kMlpParams));
+  CHL_HIP(hipMemset(d.loss.p, 0, sizeof(float)));
+  CHL_HIP(mlp_grad(d.X.p, d.T.p, d.idx.p, (int)n, d.wb.p, d.P.p, d.G.p, d.loss.p, nullptr));
+  CHL_HIP(hipMemcpy(grad, d.G.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+  float l = 0;
+  CHL_HIP(hipMemcpy(&l, d.loss.p, sizeof(float), hipMemcpyDeviceToHost));
+  if (loss) *loss = l / (float)(n * kMlpO);
+  return CHL_OK;
+}
+
+// Full-batch Adam for `steps` steps at learning rate lr; params updated in place,
+// loss_curve[s] = mean squared error before step s (if non-null).
+int chl_mlp_train(float* params, const float* x, const float* t, int64_t n, int steps, float lr, float* loss_curve) {
+  if (!params || !x || !t || n <= 0 || n > (1 << 20) || steps < 0) return set_error(CHL_E_INVAL, "bad arguments");
+  int rc = ensure_device(0);
+  if (rc) return rc;
+  MlpDev d;
+  if ((rc = d.load(params, x, n))) return rc;
+  std::vector<uint32_t> id(n);
+  std::iota(id.begin(), id.end(), 0u);
+  CHL_HIP(d.idx.reserve(n));
+  CHL_HIP(hipMemcpy(d.idx.p, id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+  CHL_HIP(d.T.reserve((size_t)n * kMlpO));
+  CHL_HIP(hipMemcpy(d.T.p, t, sizeof(float) * n * kMlpO, hipMemcpyHostToDevice));
+  CHL_HIP(d.G.reserve(kMlpParams));
+  CHL_HIP(d.M.reserve(kMlpParams));
+  CHL_HIP(d.V.reserve(kMlpParams));
+  CHL_HIP(d.loss.reserve(1));
+  CHL_HIP(hipMemset(d.M.p, 0, sizeof(float) * kMlpParams));
+  CHL_HIP(hipMemset(d.V.p, 0, sizeof(float) * kMlpParams));
+  for (int s = 0; s < steps; ++s) {
+    CHL_HIP(hipMemset(d.G.p, 0, sizeof(float) * kMlpParams));
+    CHL_HIP(hipMemset(d.loss.p, 0, sizeof(float)));
+    CHL_HIP(mlp_grad(d.X.p, d.T.p, d.idx.p, (int)n, d.wb.p, d.P.p, d.G.p, d.loss.p, nullptr));
+    CHL_HIP(mlp_adam(d.P.p, d.G.p, d.M.p, d.V.p, d.wb.p, lr, s + 1, nullptr));
+    if (loss_curve) {
+      float l = 0;
+      CHL_HIP(hipMemcpy(&l, d.loss.p, sizeof(float), hipMemcpyDeviceToHost));
+      loss_curve[s] = l / (float)(n * kMlpO);
+    }
+  }
+  CHL_HIP(hipMemcpy(params, d.P.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+  return CHL_OK;
+}
+
+int chl_mfma_probe(const uint16_t* a, const uint16_t* b, float* d) {
+  if (!a || !b || !d) return set_error(CHL_E_INVAL, "null argument");
+  int rc = ensure_device(0);
+  if (rc) return rc;
+  DBuf<uint16_t> A, B;
+  DBuf<float> D;
+  CHL_HIP(A.reserve(32 * 8));
+  CHL_HIP(B.reserve(8 * 32));
+  CHL_HIP(D.reserve(32 * 32));
+  CHL_HIP(hipMemcpy(A.p, a, sizeof(uint16_t) * 256, hipMemcpyHostToDevice));
+  CHL_HIP(hipMemcpy(B.p, b, sizeof(uint16_t) * 256, hipMemcpyHostToDevice));
+  CHL_HIP(mfma_probe(A.p, B.p, D.p, nullptr));
+  CHL_HIP(hipMemcpy(d, D.p, sizeof(float) * 1024, hipMemcpyDeviceToHost));
+  return CHL_OK;
+}
+
+}  // extern "C"

@@ -2,14 +2,18 @@
 // (distributed_memory/akiba.hpp:52-104) and of GLL's pruned Dijkstra
 // (shared_memory/akiba.hpp:65-125), run for many roots per persistent launch.
 //
+// Vertices carry two ids: the device id (a locality order of the graph, so that
+// neighbours share cache lines) and the rank (position in the vertex order,
+// rank 0 most important). Hubs, label rows and the flag rule use ranks.
+//
 // One workgroup owns one root at a time (it pulls the next root index from a
-// device cursor) and a private slot of scratch arrays sized n: the packed
-// state word of every vertex, a round stamp, two frontier lists and the list of
-// touched vertices. The search is a bucketed label-correcting Dijkstra (delta
-// stepping): each round processes the frontier entries whose distance is below
-// the bucket bound B and re-queues the rest; a vertex whose state improves is
-// queued once per round (stamp) and processed again, so every vertex is finally
-// processed at its final state.
+// device cursor) and a private slot of scratch sized n: a 16-byte cell per
+// vertex holding the packed state word and a round stamp, two frontier lists
+// and the list of touched vertices. The search is a bucketed label-correcting
+// Dijkstra (delta stepping): each round processes the frontier entries whose
+// distance is below the bucket bound B and re-queues the rest; a vertex whose
+// state improves is queued once per round (stamp) and processed again, so every
+// vertex is finally processed at its final state.
 //
 // State word = (dist << 32) | low. With TREE_FLAGS, low is the smallest rank on
 // the best path found so far, including the vertex itself; a 64-bit atomicMin
@@ -64,6 +68,30 @@ __device__ __forceinline__ void table_insert(uint32_t* key, uint32_t* val, uint3
   }
 }
 
+// cell[2 v] = state word, low half of cell[2 v + 1] = round stamp
+__device__ __forceinline__ uint32_t* stamp_of(uint64_t* cell, uint32_t v) {
+  return reinterpret_cast<uint32_t*>(cell + 2 * (size_t)v + 1);
+}
+
+// Relaxes arc (u -> v, w) from state (d, lo); queues v in N once per round.
+template <bool FLAGS>
+__device__ __forceinline__ void relax(const TreeParams& p, uint64_t* cell, uint32_t* N, uint32_t* vis,
+                                      uint32_t* s_nsize, uint32_t* s_nvis, uint32_t nstamp, uint64_t arc, uint32_t d,
+                                      uint32_t lo) {
+  const uint32_t v = (uint32_t)(arc >> 32);
+  const uint32_t nd = d + (uint32_t)arc;
+  const uint32_t nlo = FLAGS ? min(lo, p.rank[v]) : lo;
+  const uint64_t nv = ((uint64_t)nd << 32) | nlo;
+  uint64_t* sv = cell + 2 * (size_t)v;
+  if (nv < *sv) {
+    const uint64_t old = atomicMin((unsigned long long*)sv, (unsigned long long)nv);
+    if (nv < old) {
+      if (old == kStateInf) vis[atomicAdd(s_nvis, 1u)] = v;
+      if (atomicExch(stamp_of(cell, v), nstamp) != nstamp) N[atomicAdd(s_nsize, 1u)] = v;
+    }
+  }
+}
+
 template <bool FLAGS, bool PRUNE>
 __global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
   __shared__ uint32_t s_key[PRUNE ? kLdsHash : 1];
@@ -77,11 +105,10 @@ __global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
   const uint32_t wave = tid >> 6;
   const size_t n = p.n;
   const size_t slot = blockIdx.x;
-  uint64_t* st = p.st + slot * n;
-  uint32_t* stamp = p.stamp + slot * n;
+  uint64_t* cell = p.cell + slot * 2 * n;
   uint32_t* vis = p.vis + slot * n;
 
-  unsigned long long c_proc = 0, c_arcs = 0, c_prune = 0;
+  unsigned long long c_proc = 0, c_arcs = 0, c_prune = 0, c_rounds = 0;
   if (tid < 4) s_ctr[tid] = 0;
 
   for (;;) {
@@ -92,8 +119,9 @@ __global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
     __syncthreads();
     const uint32_t ri = s_ri;
     if (ri == kEmpty) break;
-    const uint32_t root = p.roots[ri];
-    const uint32_t skip = FLAGS ? (PRUNE ? p.prefix : 0u) : root;
+    const uint32_t root = p.roots[ri];  // device id
+    const uint32_t rr = p.rank[root];   // its rank = the hub of its labels
+    const uint32_t skip = FLAGS ? (PRUNE ? p.prefix : 0u) : rr;  // ranks below are not expanded
     uint32_t* F = p.qa + slot * n;
     uint32_t* N = p.qb + slot * n;
 
@@ -102,7 +130,7 @@ __global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
     uint32_t* tval = s_val;
     uint32_t tshift = 32 - kLdsHashBits, tmask = kLdsHash - 1;
     if (PRUNE) {
-      const uint64_t rb = p.root_off[root], re = p.root_off[root + 1];
+      const uint64_t rb = p.root_off[rr], re = p.root_off[rr + 1];
       if (2 * (re - rb) > kLdsHash) {
         tkey = p.gkey + slot * p.gcap;
         tval = p.gval + slot * p.gcap;
@@ -117,8 +145,8 @@ __global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
       }
     }
     if (tid == 0) {
-      st[root] = (uint64_t)root;  // dist 0, low = root rank
-      stamp[root] = 1;
+      cell[2 * (size_t)root] = (uint64_t)rr;  // dist 0, low = root rank
+      *stamp_of(cell, root) = 1;
       F[0] = root;
       vis[0] = root;
       s_fsize = 1;
@@ -153,47 +181,36 @@ __global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
         // one lane per frontier entry
         for (uint32_t k = tid; k < fsize; k += kThreads) {
           const uint32_t u = F[k];
-          const uint64_t x = st[u];
+          const uint64_t x = cell[2 * (size_t)u];
           const uint32_t d = (uint32_t)(x >> 32);
           const uint32_t lo = (uint32_t)x;
           if (d >= B) {
-            if (atomicExch(&stamp[u], nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = u;
+            if (atomicExch(stamp_of(cell, u), nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = u;
             continue;
           }
-          if (u < skip) continue;
+          if (!FLAGS && p.rank[u] < skip) continue;
           ++c_proc;
           const uint32_t ab = p.off[u], ae = p.off[u + 1];
           c_arcs += ae - ab;
-          for (uint32_t a = ab; a < ae; ++a) {
-            const uint64_t arc = p.arc[a];
-            const uint32_t v = (uint32_t)(arc >> 32);
-            const uint32_t nd = d + (uint32_t)arc;
-            const uint32_t nlo = FLAGS ? min(lo, v) : lo;
-            const uint64_t nv = ((uint64_t)nd << 32) | nlo;
-            if (nv < st[v]) {
-              const uint64_t old = atomicMin((unsigned long long*)&st[v], (unsigned long long)nv);
-              if (nv < old) {
-                if (old == kStateInf) vis[atomicAdd(&s_nvis, 1u)] = v;
-                if (atomicExch(&stamp[v], nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = v;
-              }
-            }
-          }
+          for (uint32_t a = ab; a < ae; ++a)
+            relax<FLAGS>(p, cell, N, vis, &s_nsize, &s_nvis, nstamp, p.arc[a], d, lo);
         }
       } else {
         // one wavefront per frontier entry: prune check, then relax
         for (uint32_t k = wave; k < fsize; k += kWaves) {
           const uint32_t u = F[k];
-          const uint64_t x = st[u];
+          const uint64_t x = cell[2 * (size_t)u];
           const uint32_t d = (uint32_t)(x >> 32);
           const uint32_t lo = (uint32_t)x;
           if (d >= B) {
-            if (lane == 0 && atomicExch(&stamp[u], nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = u;
+            if (lane == 0 && atomicExch(stamp_of(cell, u), nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = u;
             continue;
           }
-          if (u < skip || lo == 0) continue;  // lo == 0: pruned at this distance already
+          const uint32_t ru = p.rank[u];
+          if (ru < skip || lo == 0) continue;  // lo == 0: pruned at this distance already
           if (lane == 0) ++c_proc;
           bool pruned = false;
-          const uint64_t rb = p.scan_off[u], re = p.scan_off[u + 1];
+          const uint64_t rb = p.scan_off[ru], re = p.scan_off[ru + 1];
           uint64_t base = rb;
           for (; base < re; base += 64) {
             const uint64_t kk = base + lane;
@@ -210,25 +227,14 @@ __global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
           }
           if (lane == 0) c_prune += (pruned ? base + 64 : re) - rb;
           if (pruned) {
-            if (lane == 0) atomicMin((unsigned long long*)&st[u], (unsigned long long)((uint64_t)d << 32));
+            if (lane == 0)
+              atomicMin((unsigned long long*)&cell[2 * (size_t)u], (unsigned long long)((uint64_t)d << 32));
             continue;
           }
           const uint32_t ab = p.off[u], ae = p.off[u + 1];
           if (lane == 0) c_arcs += ae - ab;
-          for (uint32_t a = ab + lane; a < ae; a += 64) {
-            const uint64_t arc = p.arc[a];
-            const uint32_t v = (uint32_t)(arc >> 32);
-            const uint32_t nd = d + (uint32_t)arc;
-            const uint32_t nlo = FLAGS ? min(lo, v) : lo;
-            const uint64_t nv = ((uint64_t)nd << 32) | nlo;
-            if (nv < st[v]) {
-              const uint64_t old = atomicMin((unsigned long long*)&st[v], (unsigned long long)nv);
-              if (nv < old) {
-                if (old == kStateInf) vis[atomicAdd(&s_nvis, 1u)] = v;
-                if (atomicExch(&stamp[v], nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = v;
-              }
-            }
-          }
+          for (uint32_t a = ab + lane; a < ae; a += 64)
+            relax<FLAGS>(p, cell, N, vis, &s_nsize, &s_nvis, nstamp, p.arc[a], d, lo);
         }
       }
       __syncthreads();
@@ -237,10 +243,10 @@ __global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
       const uint32_t nsize = s_nsize;
       uint32_t below = 0, clean = 0, mn = kEmpty;
       for (uint32_t k = tid; k < nsize; k += kThreads) {
-        const uint64_t x = st[N[k]];
+        const uint64_t x = cell[2 * (size_t)N[k]];
         const uint32_t d = (uint32_t)(x >> 32);
         below |= d < B;
-        clean |= FLAGS && (uint32_t)x == root;
+        clean |= FLAGS && (uint32_t)x == rr;
         mn = min(mn, d);
       }
       if (below) s_below = 1;
@@ -251,11 +257,12 @@ __global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
       F = N;
       N = t;
       ++round;
+      ++c_rounds;
       __syncthreads();
     }
 
-    // Emit (root, d) at every touched vertex whose final low is the root, then
-    // restore the slot's scratch to its unreached state.
+    // Emit (root rank, d) at every touched vertex whose final low is the root's
+    // rank, then restore the slot's scratch to its unreached state.
     const uint32_t nvis = s_nvis;
     if (tid == 0) {
       s_emit = 0;
@@ -265,15 +272,21 @@ __global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
     uint32_t mine = 0;
     for (uint32_t k = tid; k < nvis; k += kThreads) {
       const uint32_t v = vis[k];
-      mine += ((uint32_t)st[v] == root && v >= root) ? 1u : 0u;
+      mine += ((uint32_t)cell[2 * (size_t)v] == rr && p.rank[v] >= rr) ? 1u : 0u;
     }
     if (mine) atomicAdd(&s_emit, mine);
     __syncthreads();
     const uint32_t emit = s_emit;
     if (tid == 0) {
+      // reserve the root's output range (wait-free); a range that does not fit
+      // is not written: the root is recorded for a second launch, and the
+      // smallest such start marks the end of the contiguous written prefix
       unsigned long long b = atomicAdd(&p.ctr[CTR_LABELS], (unsigned long long)emit);
       if (b + emit > p.out_cap) {
-        atomicOr(&p.ctr[CTR_OVERFLOW], 1ull);
+        const unsigned long long k = atomicAdd(&p.ctr[CTR_OVERFLOW], 1ull);
+        p.spill[k] = ri;
+        atomicAdd(&p.ctr[CTR_SPILL], (unsigned long long)emit);
+        atomicMin(&p.ctr[CTR_SPILL_START], b);
         b = ~0ull;
       }
       s_base = b;
@@ -283,14 +296,15 @@ __global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
     const unsigned long long obase = s_base;
     for (uint32_t k = tid; k < nvis; k += kThreads) {
       const uint32_t v = vis[k];
-      const uint64_t x = st[v];
-      if ((uint32_t)x == root && v >= root && obase != ~0ull) {
+      const uint64_t x = cell[2 * (size_t)v];
+      const uint32_t rv = p.rank[v];
+      if ((uint32_t)x == rr && rv >= rr && obase != ~0ull) {
         const unsigned long long o = obase + atomicAdd(&s_cursor, 1u);
-        p.out_key[o] = ((uint64_t)v << 32) | root;
-        p.out_ent[o] = ((uint64_t)root << 32) | (x >> 32);
+        p.out_key[o] = ((uint64_t)rv << 32) | rr;
+        p.out_ent[o] = ((uint64_t)rr << 32) | (x >> 32);
       }
-      st[v] = kStateInf;
-      stamp[v] = 0;
+      cell[2 * (size_t)v] = kStateInf;
+      *stamp_of(cell, v) = 0;
     }
     // tree size = processed entries of this root
     if (c_proc) atomicAdd(&s_ctr[0], c_proc);
@@ -309,9 +323,17 @@ __global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
     if (s_ctr[1]) atomicAdd(&p.ctr[CTR_PROCESSED], s_ctr[1]);
     if (s_ctr[2]) atomicAdd(&p.ctr[CTR_ARCS], s_ctr[2]);
     if (s_ctr[3]) atomicAdd(&p.ctr[CTR_PRUNE_ENT], s_ctr[3]);
+    if (c_rounds) atomicAdd(&p.ctr[CTR_ROUNDS], c_rounds);
   }
 }
 
+// Cells start unreached: state all ones, stamp 0.
+__global__ void init_cells_kernel(uint64_t* cell, size_t count) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    cell[2 * i] = kStateInf;
+    cell[2 * i + 1] = 0;
+  }
+}
 __global__ void fill_u64_kernel(uint64_t* p, uint64_t v, size_t count) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
     p[i] = v;
@@ -336,6 +358,12 @@ hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, hipS
     case TREE_PRUNE: hipLaunchKernelGGL((tree_kernel<false, true>), grid, block, 0, s, p); break;
     default: hipLaunchKernelGGL((tree_kernel<false, false>), grid, block, 0, s, p); break;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_init_cells(uint64_t* cell, size_t count, hipStream_t s) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(init_cells_kernel, dim3(fill_blocks(count)), dim3(256), 0, s, cell, count);
   return hipGetLastError();
 }
 

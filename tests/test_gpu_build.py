@@ -52,6 +52,14 @@ def test_grid_schedule_independent(chl, grid, batch, delta, slots):
         assert_same_labels(L, ref)
 
 
+def test_output_spill_relaunch(chl, grid):
+    # a batch output buffer of 100 labels: nearly every root spills and is run again
+    g, order, _, ref = grid
+    for mode in (1, 0):
+        L = chl.build(g, order, mode=mode, batch_roots=300, out_capacity=100)
+        assert_same_labels(L, ref)
+
+
 def test_hybrid_switches(chl, grid):
     g, order, _, ref = grid
     L = chl.build(g, order, mode=chl.MODE_HYBRID, switch_ratio=1.5, batch_roots=50)
