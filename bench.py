@@ -100,6 +100,22 @@ def query_bench(chl, L, n, batches=200, batch=4096, seed=11):
             "direct_1M_pairs_queries_per_s": big.shape[0] / direct}
 
 
+def copy_calibration(nbytes=1 << 30):
+    """One device-to-device copy of nbytes (a plain copy kernel of known size, the
+    scale for FETCH_SIZE / WRITE_SIZE counters)."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {nbytes} bytes device to device")
+
+
 def cpu_baseline(g, order, seconds):
     """The oracle's GLL (run_paraPLL restated) on all host cores over the rank prefix
     it completes within `seconds` of labeling+sorting+cleaning time."""
@@ -123,7 +139,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--mode", default="plant", choices=sorted(MODES))
-    ap.add_argument("--learned", action="store_true")
+    ap.add_argument("--static", action="store_true", help="time static rank-order batching (default: learned)")
+    ap.add_argument("--no-compare", action="store_true", help="skip the timed build under the other batching")
+    ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--delta", type=int, default=0)
     ap.add_argument("--slots", type=int, default=0)
@@ -142,10 +160,13 @@ def main():
 
     t = time.perf_counter()
     g = chl.Graph.knn(args.n, 3, seed=1)
-    order = g.order_betweenness(256, seed=7)
+    order, score = g.order_betweenness(256, seed=7, with_score=True)
     log(f"graph n={g.n} arcs={g.arcs(1)} + order in {time.perf_counter() - t:.1f} s")
-    opts = dict(mode=MODES[args.mode], batching=1 if args.learned else 0, batch_roots=args.batch,
-                delta=args.delta, slots=args.slots)
+    if args.copy_calibration:
+        copy_calibration()
+    learned = not args.static
+    opts = dict(mode=MODES[args.mode], batching=1 if learned else 0, batch_roots=args.batch,
+                delta=args.delta, slots=args.slots, vertex_score=score.ctypes.data)
     for k in range(args.warmup):
         L = chl.build(g, order, **opts)
         log(f"warmup {k}: {L.stats.total_s:.3f} s")
@@ -172,14 +193,28 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"road-like 3-NN, n={g.n}, {g.arcs(1) // 2} undirected edges, w=round(1e4*d)+1, "
                                f"256-root sampled-betweenness order, "
-                               f"{'learned' if args.learned else 'static rank-order'} batches",
+                               f"{'learned' if learned else 'static rank-order'} batches",
                    "mode": args.mode, "build_s": ms / 1e3, "labels": labels, "avg_label": s["avg_label"],
                    "batches": s["batches"], "batch_roots": s["batch_used"], "delta": s["delta_used"],
-                   "slots": s["slots_used"], "lt_s": s["lt"], "st_s": s["st"], "ct_s": s["ct"]},
+                   "slots": s["slots_used"], "lt_s": s["lt"], "st_s": s["st"], "ct_s": s["ct"],
+                   "mlp_ms": s["mlp_ms"], "pred_mae_learned": s["pred_mae_learned"],
+                   "pred_mae_static": s["pred_mae_static"], "pred_roots": s["pred_roots"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "tree_kernel",
                      "bytes_per_build": s["relax_bytes"], "kernel_ms_per_build": s["relax_ms"]},
     }
+    if not args.no_compare:
+        other = dict(opts, batching=0 if learned else 1)
+        t0 = time.perf_counter()
+        L2 = chl.build(g, order, **other)
+        t_other = time.perf_counter() - t0
+        sides = (0,) if g.symmetric else (0, 1)  # a symmetric graph stores one side
+        same = all(np.array_equal(a, b) for side in sides for a, b in zip(L.copy(side), L2.copy(side)))
+        del L2
+        t_static, t_learned = (t_other, ms / 1e3) if learned else (ms / 1e3, t_other)
+        line["batching"] = {"static_build_s": t_static, "learned_build_s": t_learned,
+                            "learned_speedup": t_static / t_learned, "labels_identical": same}
+        log(f"other batching: {t_other:.3f} s, labels identical: {same}")
     if not args.no_query:
         line["query"] = query_bench(chl, L, g.n)
     if not args.no_cpu:

@@ -100,6 +100,7 @@ typedef struct chl_build_opts {
   int64_t batch_roots;  /* roots per batch (0 = default) */
   int32_t delta;        /* bucket width of the delta-stepping relaxation (0 = twice the mean arc length) */
   int32_t slots;        /* concurrent root slots = workgroups of the tree kernel (0 = default) */
+  int granularity;      /* per-root parallelism: 0 = default (workgroup), 1 = wavefront, 2 = workgroup */
   double switch_ratio;  /* ps of distributed_memory/hhl.cpp:23 (default 1024) */
   int plant_prune;      /* 1: PLaNT trees also prune by the completed rank-prefix labels */
   int device;           /* HIP device ordinal */
@@ -131,6 +132,8 @@ typedef struct chl_build_stats {
      batches the MLP predicted */
   double pred_mae_learned, pred_mae_static;
   int64_t pred_roots;           /* roots those errors cover */
+  int64_t granularity_used;     /* 1 wavefront, 2 workgroup per root */
+  double autotune_ms;           /* wall time spent by the autotuner (its trial trees included) */
   double reserved[8];
 } chl_build_stats;
 
@@ -177,6 +180,10 @@ int chl_mlp_forward(const float* params, const float* x, int64_t n, float* y);
 int chl_mlp_grad(const float* params, const float* x, const float* t, int64_t n, float* grad, float* loss);
 /* full-batch Adam (0.9, 0.999, 1e-8) for `steps` steps; params updated in place */
 int chl_mlp_train(float* params, const float* x, const float* t, int64_t n, int steps, float lr, float* loss_curve);
+/* one training step captured as a HIP graph, replayed `replays` times, each replay
+   checked against the same step run eagerly: *max_rel = largest relative difference */
+int chl_mlp_graph_check(const float* params, const float* x, const float* t, int64_t n, int replays, float lr,
+                        float* max_rel);
 /* one 32x32x8 BF16 matrix-core product D = A B (row-major; A 32x8, B 8x32 as BF16 bits) */
 int chl_mfma_probe(const uint16_t* a, const uint16_t* b, float* d);
 

@@ -35,7 +35,7 @@ class ChlError(RuntimeError):
 
 class BuildOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("batching", C.c_int), ("autotune", C.c_int), ("batch_roots", C.c_int64),
-                ("delta", C.c_int32), ("slots", C.c_int32), ("switch_ratio", C.c_double),
+                ("delta", C.c_int32), ("slots", C.c_int32), ("granularity", C.c_int), ("switch_ratio", C.c_double),
                 ("plant_prune", C.c_int), ("device", C.c_int), ("verbose", C.c_int), ("out_capacity", C.c_int64),
                 ("vertex_score", C.c_void_p), ("reserved", C.c_int * 8)]
 
@@ -49,7 +49,7 @@ class BuildStats(C.Structure):
                 ("slots_used", C.c_int64), ("batch_used", C.c_int64), ("relax_ms", C.c_double),
                 ("clean_ms", C.c_double), ("relax_bytes", C.c_double), ("mlp_ms", C.c_double),
                 ("pred_mae_learned", C.c_double), ("pred_mae_static", C.c_double), ("pred_roots", C.c_int64),
-                ("reserved", C.c_double * 8)]
+                ("granularity_used", C.c_int64), ("autotune_ms", C.c_double), ("reserved", C.c_double * 8)]
 
     def as_dict(self):
         return {k: (getattr(self, k) if k != "reserved" else None) for k, _ in self._fields_ if k != "reserved"}
@@ -105,6 +105,7 @@ def lib():
             "chl_mlp_grad": ([vp, vp, vp, i64, vp, vp], C.c_int),
             "chl_mlp_train": ([vp, vp, vp, i64, C.c_int, C.c_float, vp], C.c_int),
             "chl_mfma_probe": ([vp, vp, vp], C.c_int),
+            "chl_mlp_graph_check": ([vp, vp, vp, i64, C.c_int, C.c_float, vp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -354,6 +355,16 @@ def mlp_train(params, x, t, steps, lr=3e-3):
     curve = np.zeros(max(steps, 1), np.float32)
     _check(lib().chl_mlp_train(_ptr(params), _ptr(x), _ptr(t), x.shape[0], steps, lr, _ptr(curve)))
     return params, curve[:steps]
+
+
+def mlp_graph_check(params, x, t, replays, lr=3e-3):
+    """Largest relative parameter difference between graph-replayed and eager training steps."""
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_F)
+    t = np.ascontiguousarray(t, np.float32).reshape(-1, MLP_O)
+    worst = C.c_float(0)
+    _check(lib().chl_mlp_graph_check(_ptr(params), _ptr(x), _ptr(t), x.shape[0], replays, lr, C.byref(worst)))
+    return worst.value
 
 
 def mfma_probe(a_bits, b_bits):

@@ -153,7 +153,9 @@ int chl_build(const chl_graph* g, const uint32_t* order, int64_t n, const chl_bu
   if (rc) return rc;
   if (stats) memset(stats, 0, sizeof(*stats));
   chl_labels* l = new chl_labels();
-  rc = run_build(g->g, ord, o, stats, l->store);
+  chl_graph* gm = const_cast<chl_graph*>(g);  // the device copy is a cache inside the handle
+  if (!gm->dev) gm->dev.reset(new DeviceGraph());
+  rc = run_build(g->g, *gm->dev, ord, o, stats, l->store);
   if (rc == CHL_OK) rc = make_handle(l, rank);
   if (rc) {
     delete l;

@@ -19,33 +19,48 @@ namespace chl {
                               hipGetErrorString(e_));                                               \
   } while (0)
 
-// Device buffer that frees itself.
+// Process-wide cache of device allocations: a released block is kept (up to a
+// cap) and handed to a later request of at least its size and at most twice
+// it, so repeated builds and query batches do not pay hipMalloc/hipFree. On an
+// allocation failure the cache is emptied and the allocation retried.
+hipError_t dev_alloc(void** p, size_t bytes, size_t* got);
+void dev_free(void* p, size_t bytes);
+void dev_cache_flush();
+
+// Device buffer that returns its memory to the cache.
 template <class T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  size_t bytes = 0;
   DBuf() = default;
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) dev_free(p, bytes);
     p = nullptr;
     n = 0;
+    bytes = 0;
   }
-  // Allocates exactly `count` elements (contents undefined); keeps a larger buffer.
+  // At least `count` elements (contents undefined); keeps a large enough buffer.
   hipError_t reserve(size_t count) {
     if (count <= n && p) return hipSuccess;
     release();
-    size_t bytes = (count ? count : 1) * sizeof(T);
-    hipError_t e = hipMalloc((void**)&p, bytes);
-    if (e == hipSuccess) n = count;
-    else p = nullptr;
+    size_t got = 0;
+    hipError_t e = dev_alloc((void**)&p, (count ? count : 1) * sizeof(T), &got);
+    if (e == hipSuccess) {
+      bytes = got;
+      n = got / sizeof(T);
+    } else {
+      p = nullptr;
+    }
     return e;
   }
   void swap(DBuf& o) {
     T* tp = p; p = o.p; o.p = tp;
     size_t tn = n; n = o.n; o.n = tn;
+    size_t tb = bytes; bytes = o.bytes; o.bytes = tb;
   }
 };
 
@@ -106,7 +121,9 @@ struct TreeParams {
   unsigned long long* ctr;  // CTR_* (CTR_NEXT is the work-queue cursor)
 };
 
-hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, hipStream_t s);
+// One workgroup per slot: 256 threads (workgroup per root) or, with
+// wave_per_root, 64 threads (wavefront per root).
+hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, int wave_per_root, hipStream_t s);
 // count cells set to the unreached state (state all ones, stamp 0)
 hipError_t launch_init_cells(uint64_t* cell, size_t count, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t count, hipStream_t s);

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -48,16 +49,25 @@ int ensure_device(int device) {
 }
 
 // Locality order of the vertices: breadth-first over the arcs of both ranges,
-// starting from the most important vertex (then from the next unvisited one in
-// rank order), so that vertices close in the graph get close device ids and a
-// search front touches few cache lines. inv[id] = vertex.
-static void locality_order(const Graph& g, const std::vector<uint32_t>& order, std::vector<uint32_t>& inv) {
+// starting from a vertex of maximum degree (then from the next unvisited vertex
+// by id), so that vertices close in the graph get close device ids and a search
+// front touches few cache lines. inv[id] = vertex.
+static void locality_order(const Graph& g, std::vector<uint32_t>& inv) {
   const int64_t n = g.n;
   inv.clear();
   inv.reserve(n);
   std::vector<uint8_t> seen(n, 0);
-  for (int64_t r = 0; r < n; ++r) {
-    const uint32_t s = order[r];
+  int64_t best = 0;
+  uint64_t bdeg = 0;
+  for (int64_t v = 0; v < n; ++v) {
+    const uint64_t d = (g.off[1][v + 1] - g.off[1][v]) + (g.off[0][v + 1] - g.off[0][v]);
+    if (d > bdeg) {
+      bdeg = d;
+      best = v;
+    }
+  }
+  for (int64_t k = -1; k < n; ++k) {
+    const uint32_t s = (uint32_t)(k < 0 ? best : k);
     if (seen[s]) continue;
     seen[s] = 1;
     size_t head = inv.size();
@@ -94,6 +104,30 @@ static void relabel_range(const Graph& g, int range, const std::vector<uint32_t>
   });
 }
 
+int DeviceGraph::build(const Graph& g, int dev) {
+  device = dev;
+  n = g.n;
+  sym = g.symmetric;
+  locality_order(g, inv);
+  pos.assign(n, 0);
+  for (int64_t i = 0; i < n; ++i) pos[inv[i]] = (uint32_t)i;
+  for (int r = 0; r < 2; ++r) {
+    if (sym && r == 0) continue;  // in-arcs equal out-arcs
+    if (g.off[r][n] >= 0xFFFFFFFFull) {
+      device = -1;
+      return set_error(CHL_E_RANGE, "more than 2^32 arcs in a range");
+    }
+    std::vector<uint32_t> o;
+    std::vector<uint64_t> a;
+    relabel_range(g, r, inv, pos, o, a);
+    CHL_HIP(off[r].reserve(n + 1));
+    CHL_HIP(arc[r].reserve(std::max<size_t>(a.size(), 1)));
+    CHL_HIP(hipMemcpy(off[r].p, o.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
+    if (!a.empty()) CHL_HIP(hipMemcpy(arc[r].p, a.data(), sizeof(uint64_t) * a.size(), hipMemcpyHostToDevice));
+  }
+  return CHL_OK;
+}
+
 int LabelStore::init(int64_t nn, bool shared, int dev) {
   n = nn;
   shared_sides = shared;
@@ -128,6 +162,7 @@ struct SideBatch {
 
 struct Builder {
   const Graph& g;
+  DeviceGraph& dg;
   const std::vector<uint32_t>& order;
   chl_build_opts o;
   chl_build_stats* stats;
@@ -137,13 +172,13 @@ struct Builder {
   int nsides;
   hipStream_t stream = nullptr;
 
-  DBuf<uint32_t> goff[2];
-  DBuf<uint64_t> garc[2];
   DBuf<uint64_t> cell;
   DBuf<uint32_t> qa, qb, vis, gkey, gval, grank;
   std::vector<uint32_t> id_of_rank;  // rank -> device id
   std::vector<uint32_t> hids;        // staging of root ids
   uint32_t slots = 0, gcap = 0;
+  uint32_t active = 0;  // slots used by launches (at most slots)
+  int wave = 0;         // 1: wavefront per root
   DBuf<uint32_t> roots, roots2, spill, tree_size, label_count;
   uint32_t W = 1;            // widest batch (scratch offset of the statistics arrays)
   int64_t stats_batches = 0;
@@ -155,9 +190,9 @@ struct Builder {
   uint64_t out_cap = 0;
   hipEvent_t ev[8];
 
-  Builder(const Graph& gg, const std::vector<uint32_t>& ord, const chl_build_opts& opts, chl_build_stats* s,
-          BuildHooks* h)
-      : g(gg), order(ord), o(opts), stats(s), hooks(h), n(gg.n), sym(gg.symmetric) {
+  Builder(const Graph& gg, DeviceGraph& d, const std::vector<uint32_t>& ord, const chl_build_opts& opts,
+          chl_build_stats* s, BuildHooks* h)
+      : g(gg), dg(d), order(ord), o(opts), stats(s), hooks(h), n(gg.n), sym(gg.symmetric) {
     nsides = sym ? 1 : 2;
     for (auto& e : ev) e = nullptr;
   }
@@ -167,30 +202,22 @@ struct Builder {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
+  // The device graph (built once per graph handle and device) and this build's
+  // rank of every device id.
   int upload_graph(const std::vector<uint32_t>& rank) {
-    std::vector<uint32_t> inv, pos(n);
-    locality_order(g, order, inv);
-    for (int64_t i = 0; i < n; ++i) pos[inv[i]] = (uint32_t)i;
+    if (dg.device != o.device || dg.n != n) {
+      int rc = dg.build(g, o.device);
+      if (rc) return rc;
+    }
     std::vector<uint32_t> rank_of_id(n);
     id_of_rank.resize(n);
     for (int64_t i = 0; i < n; ++i) {
-      rank_of_id[i] = rank[inv[i]];
-      id_of_rank[rank[inv[i]]] = (uint32_t)i;
+      rank_of_id[i] = rank[dg.inv[i]];
+      id_of_rank[rank[dg.inv[i]]] = (uint32_t)i;
     }
     CHL_HIP(grank.reserve(n));
-    CHL_HIP(hipMemcpy(grank.p, rank_of_id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-    for (int r = 0; r < 2; ++r) {
-      if (sym && r == 0) continue;  // in-arcs equal out-arcs
-      if (g.off[r][n] >= 0xFFFFFFFFull) return set_error(CHL_E_RANGE, "more than 2^32 arcs in a range");
-      std::vector<uint32_t> off;
-      std::vector<uint64_t> arc;
-      relabel_range(g, r, inv, pos, off, arc);
-      CHL_HIP(goff[r].reserve(n + 1));
-      CHL_HIP(garc[r].reserve(arc.size()));
-      CHL_HIP(hipMemcpy(goff[r].p, off.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
-      if (!arc.empty())
-        CHL_HIP(hipMemcpy(garc[r].p, arc.data(), sizeof(uint64_t) * arc.size(), hipMemcpyHostToDevice));
-    }
+    CHL_HIP(hipMemcpyAsync(grank.p, rank_of_id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream));
+    CHL_HIP(hipStreamSynchronize(stream));
     return CHL_OK;
   }
 
@@ -248,8 +275,8 @@ struct Builder {
     if (rc) return rc;
     TreeParams p;
     memset(&p, 0, sizeof(p));
-    p.off = goff[sym ? 1 : range].p;
-    p.arc = garc[sym ? 1 : range].p;
+    p.off = dg.off[sym ? 1 : range].p;
+    p.arc = dg.arc[sym ? 1 : range].p;
     p.n = (uint32_t)n;
     p.scan_off = L.d_off(side);
     p.scan_ent = L.d_ent(side);
@@ -280,7 +307,7 @@ struct Builder {
     unsigned long long hc[CTR_COUNT];
     for (int pass = 0;; ++pass) {
       CHL_HIP(hipEventRecord(ev[0], stream));
-      CHL_HIP(launch_tree(p, flags, std::min<uint32_t>(slots, p.R), stream));
+      CHL_HIP(launch_tree(p, flags, std::min<uint32_t>(active, p.R), wave, stream));
       CHL_HIP(hipEventRecord(ev[1], stream));
       CHL_HIP(hipMemcpyAsync(hc, ctr.p, sizeof(hc), hipMemcpyDeviceToHost, stream));
       CHL_HIP(hipStreamSynchronize(stream));
@@ -330,6 +357,60 @@ struct Builder {
           16.0 * hc[CTR_PROCESSED] + 16.0 * hc[CTR_ARCS] + 8.0 * hc[CTR_PRUNE_ENT] + 16.0 * hc[CTR_LABELS];
     }
     if (hooks && hooks->on_tree_launch) hooks->on_tree_launch(hooks->ctx, side, R, (float)ms, hc);
+    return CHL_OK;
+  }
+
+  // Autotuner: times the PLaNT trees of a fixed sample of roots (ranks
+  // log-uniform over [max(1024, n/64), n); labels discarded) under each candidate
+  // per-root granularity (workgroup, wavefront) and bucket width (1x, 2x, 4x the
+  // default) and keeps the fastest. Labels depend on neither.
+  int autotune(LabelStore& L, double& lt, uint32_t base_slots) {
+    const double t0 = now_s();
+    std::mt19937_64 rng(0x5EEDu);
+    const int64_t lo = std::min<int64_t>(std::max<int64_t>(1024, n / 64), n / 2);
+    const uint32_t R = (uint32_t)std::min<int64_t>(256, n - lo);
+    std::vector<uint32_t> sample(R);
+    std::uniform_real_distribution<double> u(std::log((double)lo + 1.0), std::log((double)n));
+    for (auto& r : sample) r = (uint32_t)std::min<int64_t>(n - 1, std::max<int64_t>(lo, (int64_t)std::exp(u(rng)) - 1));
+    hids.resize(R);
+    for (uint32_t k = 0; k < R; ++k) hids[k] = id_of_rank[sample[k]];
+    CHL_HIP(hipMemcpyAsync(roots.p, hids.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, stream));
+    const int32_t base_delta = o.delta;
+    double best = 1e300;
+    int best_wave = wave;
+    int32_t best_delta = o.delta;
+    uint32_t best_active = active;
+    double warm = 0;
+    int rc = run_trees(0, R, sample.data(), 0, TREE_FLAGS, L, warm);  // first launch of the kernel
+    if (rc) return rc;
+    lt += warm;
+    for (int gran = 0; gran < 2; ++gran)
+      for (int32_t mult : {1, 2, 4}) {
+        wave = gran;
+        active = std::min<uint32_t>(slots, gran ? 4 * base_slots : base_slots);
+        o.delta = std::max<int32_t>(1, base_delta * mult);
+        double ms = 0;
+        rc = run_trees(0, R, sample.data(), 0, TREE_FLAGS, L, ms);
+        if (rc) return rc;
+        lt += ms;
+        if (o.verbose) fprintf(stderr, "autotune: %s delta %d slots %u: %.3f ms\n", gran ? "wave" : "workgroup",
+                               o.delta, active, ms);
+        if (ms < best) {
+          best = ms;
+          best_wave = wave;
+          best_delta = o.delta;
+          best_active = active;
+        }
+      }
+    wave = best_wave;
+    o.delta = best_delta;
+    active = best_active;
+    if (stats) {
+      stats->autotune_ms = (now_s() - t0) * 1e3;
+      stats->delta_used = o.delta;
+      stats->slots_used = active;
+      stats->granularity_used = wave ? 1 : 2;
+    }
     return CHL_OK;
   }
 
@@ -444,7 +525,11 @@ struct Builder {
       double mean = g.len[1].empty() ? 1.0 : sum / (double)g.len[1].size();
       o.delta = (int32_t)std::max(1.0, std::min(1e9, std::floor(2.0 * mean)));
     }
-    rc = alloc_slots(o.slots > 0 ? (uint32_t)o.slots : 1024u);
+    // wavefront granularity runs four roots where a workgroup runs one
+    wave = o.granularity == 1 ? 1 : 0;
+    const uint32_t base_slots = o.slots > 0 ? (uint32_t)o.slots : 1024u;
+    rc = alloc_slots(o.autotune && o.slots <= 0 ? 4 * base_slots : (wave ? 4 * base_slots : base_slots));
+    active = std::min<uint32_t>(slots, wave ? 4 * base_slots : base_slots);
     if (rc) return rc;
     // Static schedule: a fixed width when batch_roots is given; otherwise widths
     // doubling from the slot count up to 65536 roots (later roots have small
@@ -479,13 +564,18 @@ struct Builder {
     if (stats) {
       stats->switch_rank = -1;
       stats->delta_used = o.delta;
-      stats->slots_used = slots;
+      stats->slots_used = active;
+      stats->granularity_used = wave ? 1 : 2;
       stats->batch_used = W;
     }
 
     bool plant_phase = o.mode != CHL_MODE_GLL;
     std::vector<uint32_t> hroots(W), hts(W), hlc(W);
     double lt = 0, stt = 0, ct = 0;
+    if (o.autotune && plant_phase && n >= 4096) {
+      rc = autotune(L, lt, base_slots);
+      if (rc) return rc;
+    }
     if (plant_phase && hooks && hooks->pilot && hooks->on_pilot) {
       // measured PLaNT trees of sampled roots (labels discarded; timed as tree time)
       const uint32_t R = hooks->pilot(hooks->ctx, hroots.data(), std::min<uint32_t>(W, 4096));
@@ -613,9 +703,9 @@ struct Builder {
   }
 };
 
-int build_labels(const Graph& g, const std::vector<uint32_t>& order, const chl_build_opts& opts,
+int build_labels(const Graph& g, DeviceGraph& dg, const std::vector<uint32_t>& order, const chl_build_opts& opts,
                  chl_build_stats* stats, BuildHooks* hooks, LabelStore& L) {
-  Builder b(g, order, opts, stats, hooks);
+  Builder b(g, dg, order, opts, stats, hooks);
   return b.run(L);
 }
 

@@ -24,6 +24,21 @@ struct LabelStore {
   int download(int side, std::vector<uint64_t>& off, std::vector<uint64_t>& ent) const;
 };
 
+// The graph resident on the device, kept in the graph handle across builds:
+// vertices renumbered into a breadth-first locality order (device ids), one CSR
+// per arc range with arcs packed as (head id << 32) | length. A symmetric graph
+// keeps only the out-arc range.
+struct DeviceGraph {
+  int device = -1;
+  int64_t n = 0;
+  bool sym = false;
+  std::vector<uint32_t> inv;  // device id -> vertex
+  std::vector<uint32_t> pos;  // vertex -> device id
+  DBuf<uint32_t> off[2];
+  DBuf<uint64_t> arc[2];
+  int build(const Graph& g, int device);
+};
+
 // Callbacks into the batching policy (static order, learned packing).
 struct BuildHooks {
   void* ctx = nullptr;
@@ -47,10 +62,11 @@ struct BuildHooks {
 };
 
 int ensure_device(int device);
-int build_labels(const Graph& g, const std::vector<uint32_t>& order, const chl_build_opts& opts,
+// dg: the graph on the device (built into *dg first when dg->device differs).
+int build_labels(const Graph& g, DeviceGraph& dg, const std::vector<uint32_t>& order, const chl_build_opts& opts,
                  chl_build_stats* stats, BuildHooks* hooks, LabelStore& L);
 // Build under the batching policy of opts.batching (static rank order or learned packing).
-int run_build(const Graph& g, const std::vector<uint32_t>& order, const chl_build_opts& opts, chl_build_stats* stats,
-              LabelStore& L);
+int run_build(const Graph& g, DeviceGraph& dg, const std::vector<uint32_t>& order, const chl_build_opts& opts,
+              chl_build_stats* stats, LabelStore& L);
 
 }  // namespace chl

@@ -198,9 +198,13 @@ __global__ __launch_bounds__(64) void mlp_grad_kernel(const float* X, const floa
   }
 }
 
-// Adam on the FP32 master parameters, then the BF16 copy of the weights.
+// Adam on the FP32 master parameters, then the BF16 copy of the weights. The
+// step number t >= 1 is read from device memory (so a captured graph can be
+// replayed); bias corrections 1 - beta^t.
 __global__ void mlp_adam_kernel(float* P, const float* G, float* M, float* V, uint16_t* wb, float lr, float b1, float b2,
-                                float eps, float c1, float c2) {
+                                float eps, const int* step) {
+  const float t = (float)*step;
+  const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kMlpParams; i += gridDim.x * blockDim.x) {
     const float g = G[i];
     const float m = b1 * M[i] + (1.0f - b1) * g;
@@ -212,6 +216,8 @@ __global__ void mlp_adam_kernel(float* P, const float* G, float* M, float* V, ui
     wb[i] = to_bf16(p);
   }
 }
+
+__global__ void step_bump_kernel(int* step) { *step += 1; }
 
 __global__ void mlp_sync_kernel(const float* P, uint16_t* wb) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kMlpParams; i += gridDim.x * blockDim.x) wb[i] = to_bf16(P[i]);
@@ -251,11 +257,11 @@ hipError_t mlp_grad(const float* X, const float* T, const uint32_t* idx, int B, 
   return hipGetLastError();
 }
 
-hipError_t mlp_adam(float* P, const float* G, float* M, float* V, uint16_t* wb, float lr, int step, hipStream_t s) {
-  const float b1 = 0.9f, b2 = 0.999f;
-  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
-  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, P, G, M, V, wb, lr, b1, b2,
-                     1e-8f, c1, c2);
+hipError_t mlp_adam(float* P, const float* G, float* M, float* V, uint16_t* wb, float lr, int* step, hipStream_t s) {
+  // step counter first (t = previous + 1), then the update
+  hipLaunchKernelGGL(step_bump_kernel, dim3(1), dim3(1), 0, s, step);
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, P, G, M, V, wb, lr, 0.9f,
+                     0.999f, 1e-8f, (const int*)step);
   return hipGetLastError();
 }
 

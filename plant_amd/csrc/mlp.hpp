@@ -23,8 +23,9 @@ hipError_t mlp_forward(const float* X, int N, const uint16_t* wb, const float* P
 // G += d/dP of mean over (B rows of idx) x O of (MLP(X[idx]) - T)^2; loss += sum of squared errors.
 hipError_t mlp_grad(const float* X, const float* T, const uint32_t* idx, int B, const uint16_t* wb, const float* P,
                     float* G, float* loss, hipStream_t s);
-// One Adam step (beta 0.9/0.999) at step >= 1; refreshes the BF16 weights.
-hipError_t mlp_adam(float* P, const float* G, float* M, float* V, uint16_t* wb, float lr, int step, hipStream_t s);
+// One Adam step (beta 0.9/0.999); *step (device memory, 0 before the first step)
+// is incremented first and used as t. Refreshes the BF16 weights.
+hipError_t mlp_adam(float* P, const float* G, float* M, float* V, uint16_t* wb, float lr, int* step, hipStream_t s);
 hipError_t mlp_sync(const float* P, uint16_t* wb, hipStream_t s);
 // D = A B for one 32x32x8 BF16 tile (row-major operands).
 hipError_t mfma_probe(const uint16_t* A, const uint16_t* B, float* D, hipStream_t s);

@@ -60,7 +60,14 @@ struct Learned {
 
   Learned(const Graph& gg, const std::vector<uint32_t>& ord, const chl_build_opts& opts, chl_build_stats* s)
       : g(gg), order(ord), o(opts), stats(s), n(gg.n) {}
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_B = -1;
+  DBuf<int> stepd;  // Adam step counter in device memory
+
   ~Learned() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -131,6 +138,8 @@ struct Learned {
     CHL_HIP(hipMemset(M.p, 0, sizeof(float) * kMlpParams));
     CHL_HIP(hipMemset(V.p, 0, sizeof(float) * kMlpParams));
     CHL_HIP(mlp_sync(P.p, wb.p, stream));
+    CHL_HIP(stepd.reserve(1));
+    CHL_HIP(launch_fill_u32((uint32_t*)stepd.p, 0, 1, stream));
     CHL_HIP(tidx.reserve(tcap));
     CHL_HIP(T.reserve(tcap * kMlpO));
     CHL_HIP(mb.reserve(1024));
@@ -312,18 +321,32 @@ struct Learned {
       rk[k] = hidx[pos];
       for (int o = 0; o < kMlpO; ++o) tv[k * kMlpO + o] = (float)(hT[pos * kMlpO + o] - trend(o, hidx[pos]));
     }
-    CHL_HIP(mb.reserve(rk.size()));
-    CHL_HIP(T.reserve(std::max<size_t>(tv.size(), tcap * kMlpO)));
+    CHL_HIP(mb.reserve((size_t)kSteps * 512));
+    CHL_HIP(T.reserve(std::max<size_t>((size_t)kSteps * 512 * kMlpO, tcap * kMlpO)));
     CHL_HIP(hipMemcpyAsync(mb.p, rk.data(), sizeof(uint32_t) * rk.size(), hipMemcpyHostToDevice, stream));
     CHL_HIP(hipMemcpyAsync(T.p, tv.data(), sizeof(float) * tv.size(), hipMemcpyHostToDevice, stream));
-    timed_begin();
-    for (int s = 0; s < kSteps; ++s) {
-      CHL_HIP(launch_fill_u32((uint32_t*)G.p, 0, kMlpParams, stream));
-      CHL_HIP(launch_fill_u32((uint32_t*)loss.p, 0, 1, stream));
-      CHL_HIP(mlp_grad(X.p, T.p + (size_t)s * B * kMlpO, mb.p + (size_t)s * B, B, wb.p, P.p, G.p, loss.p, stream));
-      CHL_HIP(mlp_adam(P.p, G.p, M.p, V.p, wb.p, 3e-3f, ++step, stream));
+    // the kSteps steps are one captured graph, replayed for every training
+    // call with the same minibatch size (inputs live in fixed device buffers)
+    if (!exec || graph_B != B) {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      exec = nullptr;
+      graph = nullptr;
+      CHL_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      for (int s = 0; s < kSteps; ++s) {
+        CHL_HIP(launch_fill_u32((uint32_t*)G.p, 0, kMlpParams, stream));
+        CHL_HIP(launch_fill_u32((uint32_t*)loss.p, 0, 1, stream));
+        CHL_HIP(mlp_grad(X.p, T.p + (size_t)s * B * kMlpO, mb.p + (size_t)s * B, B, wb.p, P.p, G.p, loss.p, stream));
+        CHL_HIP(mlp_adam(P.p, G.p, M.p, V.p, wb.p, 3e-3f, stepd.p, stream));
+      }
+      CHL_HIP(hipStreamEndCapture(stream, &graph));
+      CHL_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      graph_B = B;
     }
+    timed_begin();
+    CHL_HIP(hipGraphLaunch(exec, stream));
     timed_end();
+    step += kSteps;
     trained = true;
     pred_r0 = -1;  // weights changed
     return CHL_OK;
@@ -346,9 +369,9 @@ void hk_batch(void* c, const uint32_t* roots, const uint32_t* ts, const uint32_t
 
 }  // namespace
 
-int run_build(const Graph& g, const std::vector<uint32_t>& order, const chl_build_opts& opts, chl_build_stats* stats,
-              LabelStore& L) {
-  if (opts.batching != CHL_BATCH_LEARNED) return build_labels(g, order, opts, stats, nullptr, L);
+int run_build(const Graph& g, DeviceGraph& dg, const std::vector<uint32_t>& order, const chl_build_opts& opts,
+              chl_build_stats* stats, LabelStore& L) {
+  if (opts.batching != CHL_BATCH_LEARNED) return build_labels(g, dg, order, opts, stats, nullptr, L);
   Learned pol(g, order, opts, stats);
   int rc = pol.init();
   if (rc) return rc;
@@ -361,7 +384,7 @@ int run_build(const Graph& g, const std::vector<uint32_t>& order, const chl_buil
   h.pilot = hk_pilot;
   h.on_pilot = hk_on_pilot;
   pol.hooks = &h;
-  rc = build_labels(g, order, opts, stats, &h, L);
+  rc = build_labels(g, dg, order, opts, stats, &h, L);
   if (stats) {
     stats->mlp_ms = pol.mlp_ms;
     stats->pred_roots = pol.mae_n;
@@ -381,6 +404,7 @@ struct MlpDev {
   DBuf<float> P, X, T, Y, G, loss, M, V;
   DBuf<uint16_t> wb;
   DBuf<uint32_t> idx;
+  DBuf<int> step;
   int load(const float* params, const float* x, int64_t N) {
     CHL_HIP(P.reserve(kMlpParams));
     CHL_HIP(wb.reserve(kMlpParams));
@@ -450,6 +474,8 @@ int chl_mlp_train(float* params, const float* x, const float* t, int64_t n, int 
   CHL_HIP(d.G.reserve(kMlpParams));
   CHL_HIP(d.M.reserve(kMlpParams));
   CHL_HIP(d.V.reserve(kMlpParams));
+  CHL_HIP(d.step.reserve(1));
+  CHL_HIP(hipMemset(d.step.p, 0, sizeof(int)));
   CHL_HIP(d.loss.reserve(1));
   CHL_HIP(hipMemset(d.M.p, 0, sizeof(float) * kMlpParams));
   CHL_HIP(hipMemset(d.V.p, 0, sizeof(float) * kMlpParams));
@@ -457,7 +483,7 @@ int chl_mlp_train(float* params, const float* x, const float* t, int64_t n, int 
     CHL_HIP(hipMemset(d.G.p, 0, sizeof(float) * kMlpParams));
     CHL_HIP(hipMemset(d.loss.p, 0, sizeof(float)));
     CHL_HIP(mlp_grad(d.X.p, d.T.p, d.idx.p, (int)n, d.wb.p, d.P.p, d.G.p, d.loss.p, nullptr));
-    CHL_HIP(mlp_adam(d.P.p, d.G.p, d.M.p, d.V.p, d.wb.p, lr, s + 1, nullptr));
+    CHL_HIP(mlp_adam(d.P.p, d.G.p, d.M.p, d.V.p, d.wb.p, lr, d.step.p, nullptr));
     if (loss_curve) {
       float l = 0;
       CHL_HIP(hipMemcpy(&l, d.loss.p, sizeof(float), hipMemcpyDeviceToHost));
@@ -466,6 +492,75 @@ int chl_mlp_train(float* params, const float* x, const float* t, int64_t n, int 
   }
   CHL_HIP(hipMemcpy(params, d.P.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
   return CHL_OK;
+}
+
+// One full-batch training step captured as a HIP graph and replayed `replays`
+// times on a parameter copy, each replay followed by the same step run eagerly on
+// a second copy; *max_rel gets the largest |graph - eager| / (|eager| + 1e-3)
+// over all parameters after every replay.
+int chl_mlp_graph_check(const float* params, const float* x, const float* t, int64_t n, int replays, float lr,
+                        float* max_rel) {
+  if (!params || !x || !t || !max_rel || n <= 0 || n > (1 << 20) || replays < 1)
+    return set_error(CHL_E_INVAL, "bad arguments");
+  int rc = ensure_device(0);
+  if (rc) return rc;
+  MlpDev e, g;
+  if ((rc = e.load(params, x, n)) || (rc = g.load(params, x, n))) return rc;
+  std::vector<uint32_t> id(n);
+  std::iota(id.begin(), id.end(), 0u);
+  for (MlpDev* d : {&e, &g}) {
+    CHL_HIP(d->idx.reserve(n));
+    CHL_HIP(hipMemcpy(d->idx.p, id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    CHL_HIP(d->T.reserve((size_t)n * kMlpO));
+    CHL_HIP(hipMemcpy(d->T.p, t, sizeof(float) * n * kMlpO, hipMemcpyHostToDevice));
+    CHL_HIP(d->G.reserve(kMlpParams));
+    CHL_HIP(d->M.reserve(kMlpParams));
+    CHL_HIP(d->V.reserve(kMlpParams));
+    CHL_HIP(d->loss.reserve(1));
+    CHL_HIP(d->step.reserve(1));
+    CHL_HIP(hipMemset(d->M.p, 0, sizeof(float) * kMlpParams));
+    CHL_HIP(hipMemset(d->V.p, 0, sizeof(float) * kMlpParams));
+    CHL_HIP(hipMemset(d->step.p, 0, sizeof(int)));
+  }
+  hipStream_t s = nullptr;
+  CHL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  auto step = [&](MlpDev& d, hipStream_t st) -> int {
+    CHL_HIP(launch_fill_u32((uint32_t*)d.G.p, 0, kMlpParams, st));
+    CHL_HIP(launch_fill_u32((uint32_t*)d.loss.p, 0, 1, st));
+    CHL_HIP(mlp_grad(d.X.p, d.T.p, d.idx.p, (int)n, d.wb.p, d.P.p, d.G.p, d.loss.p, st));
+    CHL_HIP(mlp_adam(d.P.p, d.G.p, d.M.p, d.V.p, d.wb.p, lr, d.step.p, st));
+    return CHL_OK;
+  };
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  rc = CHL_OK;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = set_error(CHL_E_DEVICE, "capture");
+  if (!rc) rc = step(g, s);
+  if (!rc && hipStreamEndCapture(s, &graph) != hipSuccess) rc = set_error(CHL_E_DEVICE, "end capture");
+  if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+    rc = set_error(CHL_E_DEVICE, "instantiate");
+  std::vector<float> pg(kMlpParams), pe(kMlpParams);
+  float worst = 0.0f;
+  for (int r = 0; r < replays && !rc; ++r) {
+    if (hipGraphLaunch(exec, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+      rc = set_error(CHL_E_DEVICE, "graph replay");
+      break;
+    }
+    if ((rc = step(e, nullptr))) break;
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(pg.data(), g.P.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(pe.data(), e.P.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = set_error(CHL_E_DEVICE, "readback");
+      break;
+    }
+    for (int i = 0; i < kMlpParams; ++i)
+      worst = std::max(worst, std::fabs(pg[i] - pe[i]) / (std::fabs(pe[i]) + 1e-3f));
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipStreamDestroy(s);
+  *max_rel = worst;
+  return rc;
 }
 
 int chl_mfma_probe(const uint16_t* a, const uint16_t* b, float* d) {

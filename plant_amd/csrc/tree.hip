@@ -92,8 +92,12 @@ __device__ __forceinline__ void relax(const TreeParams& p, uint64_t* cell, uint3
   }
 }
 
-template <bool FLAGS, bool PRUNE>
-__global__ __launch_bounds__(kThreads) void tree_kernel(TreeParams p) {
+// THREADS = 256: a workgroup of four wavefronts per root; 64: one wavefront per
+// root (more roots in flight, no cross-wave barrier cost for small trees).
+template <bool FLAGS, bool PRUNE, int THREADS>
+__global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
+  constexpr int kThreads = THREADS;
+  constexpr int kWaves = THREADS / 64;
   __shared__ uint32_t s_key[PRUNE ? kLdsHash : 1];
   __shared__ uint32_t s_val[PRUNE ? kLdsHash : 1];
   __shared__ uint32_t s_ri, s_fsize, s_nsize, s_nvis, s_below, s_clean, s_minnext, s_emit, s_cursor;
@@ -350,14 +354,20 @@ static unsigned fill_blocks(size_t count) {
 
 }  // namespace
 
-hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, hipStream_t s) {
-  dim3 grid(slots), block(kThreads);
+template <int T>
+static void launch_tree_t(const TreeParams& p, uint32_t flags, uint32_t slots, hipStream_t s) {
+  dim3 grid(slots), block(T);
   switch (flags & (TREE_FLAGS | TREE_PRUNE)) {
-    case TREE_FLAGS: hipLaunchKernelGGL((tree_kernel<true, false>), grid, block, 0, s, p); break;
-    case TREE_FLAGS | TREE_PRUNE: hipLaunchKernelGGL((tree_kernel<true, true>), grid, block, 0, s, p); break;
-    case TREE_PRUNE: hipLaunchKernelGGL((tree_kernel<false, true>), grid, block, 0, s, p); break;
-    default: hipLaunchKernelGGL((tree_kernel<false, false>), grid, block, 0, s, p); break;
+    case TREE_FLAGS: hipLaunchKernelGGL((tree_kernel<true, false, T>), grid, block, 0, s, p); break;
+    case TREE_FLAGS | TREE_PRUNE: hipLaunchKernelGGL((tree_kernel<true, true, T>), grid, block, 0, s, p); break;
+    case TREE_PRUNE: hipLaunchKernelGGL((tree_kernel<false, true, T>), grid, block, 0, s, p); break;
+    default: hipLaunchKernelGGL((tree_kernel<false, false, T>), grid, block, 0, s, p); break;
   }
+}
+
+hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, int wave_per_root, hipStream_t s) {
+  if (wave_per_root) launch_tree_t<64>(p, flags, slots, s);
+  else launch_tree_t<kThreads>(p, flags, slots, s);
   return hipGetLastError();
 }
 

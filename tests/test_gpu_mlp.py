@@ -51,6 +51,16 @@ def test_gradient_matches_reference(chl):
         assert err <= 1e-2 * np.linalg.norm(ref[lo:hi]) + 1e-6, (lo, err, np.linalg.norm(ref[lo:hi]))
 
 
+def test_graph_replays_match_eager(chl):
+    # the captured training step, replayed 12 times with eager steps and
+    # synchronisations in between; FP32 atomics reorder sums, so 1e-3 relative
+    rng = np.random.default_rng(3)
+    x = features(rng, 700)
+    t = rng.normal(size=(700, M.O)).astype(np.float32)
+    worst = chl.mlp_graph_check(M.init_params(8), x, t, replays=12)
+    assert worst < 1e-3, worst
+
+
 def test_training_fits_tree_sizes(chl, orc):
     # targets: log tree size / label count of PLaNT trees measured by the oracle
     g = chl.Graph.knn(3000, 3, seed=4)
