@@ -1,11 +1,15 @@
-"""Benchmark: canonical hub label build of the 1M-vertex road-like graph on one GPU.
+"""Benchmark: canonical hub label build on one GPU.
 
-Workload: n = 1,000,000 points uniform in the unit square, each joined to its 3
-nearest neighbours, symmetrised, integer weight round(1e4 * euclidean) + 1;
-vertex order = sampled betweenness from 256 seeded roots; labels built in static
-rank-order batches. One step = one complete label build (all n roots) with the
-graph and order already resident in the library; the value is labels per second
-(entries of both sides, as Labeling::get_total counts them).
+Default workload (road): n = 1,000,000 points uniform in the unit square, each
+joined to its 3 nearest neighbours, symmetrised, integer weight
+round(1e4 * euclidean) + 1; vertex order = sampled betweenness from 256 seeded
+roots; PLaNT trees. Workload rmat / rmat-w16: R-MAT scale 20, edge factor 3,
+symmetrised, unit weights / weights in [1, 16], degree order, hybrid PLaNT to
+pruned trees. One step = one complete label build (all n roots) with the graph
+and order already resident in the library, under learned batching (--static:
+static rank-order batching); the value is labels per second (entries of both
+sides, as Labeling::get_total counts them). The other batching policy is timed
+once beside it and its labels compared.
 
 Extra fields: the dominant kernel's roofline (algorithmic bytes of the tree
 kernel over its device time, measured with HIP events on the launch stream), a
@@ -13,7 +17,7 @@ CPU baseline (the oracle's GLL on all host cores over a bounded rank prefix of
 the same graph), and the query path (merge-join kernel on 4096-pair batches and
 the continuous-batching server's p50/p99 latency).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--mode plant|gll|hybrid]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload road|rmat|rmat-w16] [--mode plant|gll|hybrid]
 """
 import argparse
 import ctypes as C
@@ -137,8 +141,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--mode", default="plant", choices=sorted(MODES))
+    ap.add_argument("--workload", default="road", choices=["road", "rmat", "rmat-w16"],
+                    help="road: 1M-vertex 3-NN road-like graph (default); rmat / rmat-w16: R-MAT scale 20, "
+                         "edge factor 3, unit weights / weights in [1, 16], degree order, hybrid mode")
+    ap.add_argument("--n", type=int, default=0, help="vertices (road) or 2^scale (rmat); 0 = the workload's own")
+    ap.add_argument("--mode", default=None, choices=sorted(MODES), help="default: plant (road), hybrid (rmat)")
     ap.add_argument("--static", action="store_true", help="time static rank-order batching (default: learned)")
     ap.add_argument("--no-compare", action="store_true", help="skip the timed build under the other batching")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -159,13 +166,26 @@ def main():
         sys.exit(1)
 
     t = time.perf_counter()
-    g = chl.Graph.knn(args.n, 3, seed=1)
-    order, score = g.order_betweenness(256, seed=7, with_score=True)
+    if args.workload == "road":
+        mode = args.mode or "plant"
+        g = chl.Graph.knn(args.n or 1_000_000, 3, seed=1)
+        order, score = g.order_betweenness(256, seed=7, with_score=True)
+        desc = (f"road-like 3-NN, n={g.n}, {g.arcs(1) // 2} undirected edges, w=round(1e4*d)+1, "
+                f"256-root sampled-betweenness order")
+    else:
+        mode = args.mode or "hybrid"
+        scale = int(np.log2(args.n)) if args.n else 20
+        wmax = 16 if args.workload == "rmat-w16" else 1
+        g = chl.Graph.rmat(scale, 3 << scale, wmax=wmax, seed=1)
+        order = g.order_degree(seed=2)
+        _, score = g.order_betweenness(64, seed=7, with_score=True)  # a predictor feature only
+        desc = (f"R-MAT scale {scale} (a=.57 b=.19 c=.19 d=.05), edge factor 3, symmetrised, n={g.n}, "
+                f"{g.arcs(1) // 2} undirected edges, weights in [1, {wmax}], degree order")
     log(f"graph n={g.n} arcs={g.arcs(1)} + order in {time.perf_counter() - t:.1f} s")
     if args.copy_calibration:
         copy_calibration()
     learned = not args.static
-    opts = dict(mode=MODES[args.mode], batching=1 if learned else 0, batch_roots=args.batch,
+    opts = dict(mode=MODES[mode], batching=1 if learned else 0, batch_roots=args.batch,
                 delta=args.delta, slots=args.slots, vertex_score=score.ctypes.data)
     for k in range(args.warmup):
         L = chl.build(g, order, **opts)
@@ -191,14 +211,13 @@ def main():
         "value": labels / (ms / 1e3), "unit": "labels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": f"road-like 3-NN, n={g.n}, {g.arcs(1) // 2} undirected edges, w=round(1e4*d)+1, "
-                               f"256-root sampled-betweenness order, "
-                               f"{'learned' if learned else 'static rank-order'} batches",
-                   "mode": args.mode, "build_s": ms / 1e3, "labels": labels, "avg_label": s["avg_label"],
+        "config": {"workload": f"{desc}, {'learned' if learned else 'static rank-order'} batches",
+                   "mode": mode, "build_s": ms / 1e3, "labels": labels, "avg_label": s["avg_label"],
                    "batches": s["batches"], "batch_roots": s["batch_used"], "delta": s["delta_used"],
                    "slots": s["slots_used"], "lt_s": s["lt"], "st_s": s["st"], "ct_s": s["ct"],
                    "mlp_ms": s["mlp_ms"], "pred_mae_learned": s["pred_mae_learned"],
-                   "pred_mae_static": s["pred_mae_static"], "pred_roots": s["pred_roots"]},
+                   "pred_mae_static": s["pred_mae_static"], "pred_roots": s["pred_roots"],
+                   "switch_rank": s["switch_rank"], "switch_trials": s["switch_trials"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "tree_kernel",
                      "bytes_per_build": s["relax_bytes"], "kernel_ms_per_build": s["relax_ms"]},

@@ -94,7 +94,10 @@ enum { CHL_MODE_GLL = 0, CHL_MODE_PLANT = 1, CHL_MODE_HYBRID = 2 };
 enum { CHL_BATCH_STATIC = 0, CHL_BATCH_LEARNED = 1 };
 
 typedef struct chl_build_opts {
-  int mode;             /* CHL_MODE_*; HYBRID = PLaNT until some root's tree/label ratio passes switch_ratio */
+  int mode;             /* CHL_MODE_*; HYBRID = PLaNT trees, then pruned trees: under static batching
+                           from the batch where some root's tree/label ratio passes switch_ratio (the
+                           device tests it); under learned batching where the trees predicted for the
+                           next roots would cost more than pruned trees measured on a trial batch */
   int batching;         /* CHL_BATCH_STATIC (rank order) or CHL_BATCH_LEARNED (MLP-predicted packing) */
   int autotune;         /* 1: pick batch width / delta / slots by measurement on the first batches */
   int64_t batch_roots;  /* roots per batch (0 = default) */
@@ -134,7 +137,9 @@ typedef struct chl_build_stats {
   int64_t pred_roots;           /* roots those errors cover */
   int64_t granularity_used;     /* 1 wavefront, 2 workgroup per root */
   double autotune_ms;           /* wall time spent by the autotuner (its trial trees included) */
-  double reserved[8];
+  int64_t switch_trials;        /* hybrid under learned batching: pruned trial batches run */
+  double policy_ms;             /* learned batching: wall time inside the policy (features, MLP, hooks) */
+  double reserved[6];
 } chl_build_stats;
 
 void chl_build_opts_default(chl_build_opts* o);

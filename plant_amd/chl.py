@@ -49,7 +49,8 @@ class BuildStats(C.Structure):
                 ("slots_used", C.c_int64), ("batch_used", C.c_int64), ("relax_ms", C.c_double),
                 ("clean_ms", C.c_double), ("relax_bytes", C.c_double), ("mlp_ms", C.c_double),
                 ("pred_mae_learned", C.c_double), ("pred_mae_static", C.c_double), ("pred_roots", C.c_int64),
-                ("granularity_used", C.c_int64), ("autotune_ms", C.c_double), ("reserved", C.c_double * 8)]
+                ("granularity_used", C.c_int64), ("autotune_ms", C.c_double), ("switch_trials", C.c_int64),
+                ("policy_ms", C.c_double), ("reserved", C.c_double * 6)]
 
     def as_dict(self):
         return {k: (getattr(self, k) if k != "reserved" else None) for k, _ in self._fields_ if k != "reserved"}

@@ -570,6 +570,7 @@ struct Builder {
     }
 
     bool plant_phase = o.mode != CHL_MODE_GLL;
+    const bool policy_switch = o.mode == CHL_MODE_HYBRID && hooks && hooks->plant_next;
     std::vector<uint32_t> hroots(W), hts(W), hlc(W);
     double lt = 0, stt = 0, ct = 0;
     if (o.autotune && plant_phase && n >= 4096) {
@@ -598,6 +599,17 @@ struct Builder {
       }
       const uint32_t R = (uint32_t)std::min<int64_t>(bw, n - r0);
       const uint32_t prefix = (uint32_t)r0;
+      const double t_batch = lt + stt + ct;
+      // hybrid under a policy: the policy picks each batch's tree kind until it
+      // makes the pruned kind final (a pruned batch is exact as any other)
+      bool plant_batch = plant_phase;
+      if (policy_switch && plant_phase && prefix > 0) {
+        int fin = 0;
+        plant_batch = hooks->plant_next(hooks->ctx, r0, R, &fin) != 0;
+        if (stats && !plant_batch && stats->switch_rank < 0) stats->switch_rank = r0;
+        if (stats && plant_batch) stats->switch_rank = -1;
+        if (!plant_batch && fin) plant_phase = false;
+      }
       for (uint32_t k = 0; k < R; ++k) hroots[k] = (uint32_t)(r0 + k);
       if (hooks && hooks->order_roots) hooks->order_roots(hooks->ctx, hroots.data(), R);
       if (hooks && hooks->out_estimate) {
@@ -608,8 +620,14 @@ struct Builder {
       for (uint32_t k = 0; k < R; ++k) hids[k] = id_of_rank[hroots[k]];
       CHL_HIP(hipMemcpyAsync(roots.p, hids.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, stream));
       uint32_t flags = 0;
-      if (plant_phase) flags = TREE_FLAGS | ((o.plant_prune && prefix > 0) ? TREE_PRUNE : 0u);
+      if (plant_batch) flags = TREE_FLAGS | ((o.plant_prune && prefix > 0) ? TREE_PRUNE : 0u);
       else flags = prefix > 0 ? TREE_PRUNE : 0u;
+      if (flags == 0 && o.out_capacity <= 0) {
+        // unpruned trees without flags label every vertex they reach
+        size_t freeb = 0, totalb = 0;
+        CHL_HIP(hipMemGetInfo(&freeb, &totalb));
+        out_cap = std::max<uint64_t>(out_cap, std::min<uint64_t>((uint64_t)R * (uint64_t)n + 4096, freeb / 4 / 16));
+      }
       if (flags & TREE_PRUNE) {
         uint64_t mr = 0;
         for (int s = 0; s < nsides; ++s) {
@@ -636,7 +654,8 @@ struct Builder {
         CHL_HIP(hipMemcpy(hts.data(), tree_size.p, sizeof(uint32_t) * R, hipMemcpyDeviceToHost));
         CHL_HIP(hipMemcpy(hlc.data(), label_count.p, sizeof(uint32_t) * R, hipMemcpyDeviceToHost));
       }
-      if (plant_phase && o.mode == CHL_MODE_HYBRID)
+      const bool device_switch = plant_phase && o.mode == CHL_MODE_HYBRID && !policy_switch;
+      if (device_switch)
         CHL_HIP(launch_switch_test(tree_size.p, label_count.p, R, o.switch_ratio, misc.p, stream));
       for (int s = 0; s < nsides; ++s) {
         rc = sort_side(s, stt);
@@ -661,8 +680,10 @@ struct Builder {
         if (flags & TREE_FLAGS) stats->trees_plant += (int64_t)R * nsides;
         else stats->trees_pruned += (int64_t)R * nsides;
       }
-      if (hooks && hooks->on_batch) hooks->on_batch(hooks->ctx, hroots.data(), hts.data(), hlc.data(), R, flags);
-      if (plant_phase && o.mode == CHL_MODE_HYBRID) {
+      if (hooks && hooks->on_batch)
+        hooks->on_batch(hooks->ctx, hroots.data(), hts.data(), hlc.data(), R, flags, lt + stt + ct - t_batch,
+                        sb[0].kept + (nsides == 2 ? sb[1].kept : 0));
+      if (device_switch) {
         unsigned long long f = 0;
         CHL_HIP(hipMemcpyAsync(&f, misc.p, sizeof(f), hipMemcpyDeviceToHost, stream));
         CHL_HIP(hipStreamSynchronize(stream));

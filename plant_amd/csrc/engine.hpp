@@ -44,9 +44,15 @@ struct BuildHooks {
   void* ctx = nullptr;
   // May permute roots[0..R) of the next batch (the set is fixed).
   void (*order_roots)(void* ctx, uint32_t* roots, uint32_t R) = nullptr;
-  // Per-root measured tree sizes and label counts of a finished batch.
+  // Per-root measured tree sizes and label counts of a finished batch (flags:
+  // TREE_*), its device time (trees, sort, clean and merge) and the labels it
+  // added to the store.
   void (*on_batch)(void* ctx, const uint32_t* roots, const uint32_t* tree_size, const uint32_t* label_count,
-                   uint32_t R, uint32_t flags) = nullptr;
+                   uint32_t R, uint32_t flags, double ms, uint64_t labels) = nullptr;
+  // Hybrid mode: 1 if the batch of R roots at rank r0 runs PLaNT trees, 0 if it
+  // runs pruned trees; *final = 1 makes every later batch pruned. nullptr: the
+  // device-side tree/label ratio test decides.
+  int (*plant_next)(void* ctx, int64_t r0, uint32_t R, int* final) = nullptr;
   // One tree launch finished: device time and the launch counters (CTR_*).
   void (*on_tree_launch)(void* ctx, int side, uint32_t R, float ms, const unsigned long long* ctr) = nullptr;
   // Width of the batch starting at rank r0 (at most max_width); 0 keeps the static width.

@@ -10,12 +10,16 @@
 // batch boundary (enough predicted work that the largest tree is a small part
 // of it), the order of roots inside the batch (largest predicted tree first, so
 // the persistent workgroups finish together) and the batch output capacity
-// (predicted labels, so batches rarely spill). Batches stay contiguous rank
+// (predicted labels, so batches rarely spill). In hybrid mode they also place
+// the PLaNT-to-pruned switch (predicted tree/label ratio, confirmed by a timed
+// trial batch). Batches stay contiguous rank
 // ranges, so the labels are the same under either policy. Until the first batch
 // has been measured the static schedule is used, and the static estimate (the
 // previous batch's mean) is kept beside the MLP's to report both errors.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <random>
@@ -58,8 +62,26 @@ struct Learned {
   // per label (output key and entry plus their sorted copies), at most 2^31 - 1
   double label_budget = 268435456.0;
 
+  // Hybrid switch point. PLaNT work per root grows with its tree/label ratio,
+  // while a pruned tree costs about its labels times the label merge-join; the
+  // GPU ratio of the two costs depends on the graph (road-like graphs keep PLaNT
+  // cheaper to the end, scale-free graphs want pruned trees after the first
+  // thousand roots). A batch whose predicted tree/label ratio (sum of predicted
+  // tree sizes over sum of predicted labels) reaches theta runs pruned trees as a
+  // trial; batches are cut where a block of kBlock roots crosses theta. The
+  // trial's measured device time against the PLaNT time predicted for its roots
+  // (measured time per tree vertex of the last PLaNT batch times the predicted
+  // tree sizes) makes the switch final, or doubles theta.
+  static constexpr uint32_t kBlock = 256;
+  bool hybrid = false, switched = false, trial_pending = false;
+  double theta = 16.0, ms_per_vertex = 0.0, trial_plant_ms = 0.0, trial_ratio = 0.0;
+  int64_t decided_r0 = -1;
+  bool decided_trial = false;
+  int64_t trials = 0;
+  double host_ms = 0;  // wall time inside the policy
+
   Learned(const Graph& gg, const std::vector<uint32_t>& ord, const chl_build_opts& opts, chl_build_stats* s)
-      : g(gg), order(ord), o(opts), stats(s), n(gg.n) {}
+      : g(gg), order(ord), o(opts), stats(s), n(gg.n), hybrid(opts.mode == CHL_MODE_HYBRID) {}
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_B = -1;
@@ -178,28 +200,81 @@ struct Learned {
   float pred_tree(uint32_t rank) const { return pred[(size_t)(rank - pred_r0) * kMlpO]; }
   float pred_labels(uint32_t rank) const { return pred[(size_t)(rank - pred_r0) * kMlpO + 1]; }
 
+  // Predicted sums of tree sizes and labels over ranks [a, b) (inside the
+  // predicted range).
+  void pred_sums(int64_t a, int64_t b, double& tree, double& labels) const {
+    tree = 0;
+    labels = 0;
+    for (int64_t r = a; r < b; ++r) {
+      tree += std::expm1((double)std::max(0.0f, pred_tree((uint32_t)r)));
+      labels += std::expm1((double)std::max(0.0f, pred_labels((uint32_t)r)));
+    }
+  }
+  double pred_ratio(int64_t a, int64_t b) const {
+    double t, l;
+    pred_sums(a, b, t, l);
+    return t / std::max(1.0, l);
+  }
+
   uint32_t next_width(int64_t r0, uint32_t static_width, uint32_t max_width) {
     (void)static_width;
-    if (!trained) return 0;
+    decided_r0 = -1;
+    if (!trained || switched) return 0;
     const uint32_t cnt = (uint32_t)std::min<int64_t>(max_width, n - r0);
     if (predict(r0, cnt)) return 0;
     // the widest batch whose predicted labels fit the output budget (every batch
     // boundary is a device-wide wait for the batch's last tree), and at least one
     // root per slot
     const uint32_t slots = std::max<uint32_t>(1, hooks ? hooks->slots : 1);
+    if (hybrid && r0 > 0 && ms_per_vertex > 0 &&
+        pred_ratio(r0, r0 + std::min<uint32_t>(kBlock, cnt)) >= theta) {
+      // a trial batch of pruned trees: two roots per slot
+      decided_r0 = r0;
+      decided_trial = true;
+      return std::min<uint32_t>(cnt, 2 * slots);
+    }
     double labels = 0;
     uint32_t R = 0;
     while (R < cnt) {
       labels += std::expm1((double)std::max(0.0f, pred_labels((uint32_t)(r0 + R))));
       if (2.0 * labels > label_budget && R >= slots) break;  // 2x margin on the prediction
+      // hybrid: end the batch before the first block predicted to cross theta
+      if (hybrid && R >= slots && R % kBlock == 0 && R + kBlock <= cnt &&
+          pred_ratio(r0 + R, r0 + R + kBlock) >= theta)
+        break;
       ++R;
     }
+    decided_r0 = r0;
+    decided_trial = false;
     return R;
+  }
+
+  // Hybrid: the kind of trees of the batch [r0, r0 + R).
+  int plant_next(int64_t r0, uint32_t R, int* final) {
+    *final = 0;
+    if (switched) {
+      *final = 1;
+      return 0;
+    }
+    if (!trained || ms_per_vertex <= 0) return 1;
+    if (predict(r0, R)) return 1;
+    double t, l;
+    pred_sums(r0, r0 + R, t, l);
+    const bool trial = decided_r0 == r0 ? decided_trial : t / std::max(1.0, l) >= theta;
+    if (!trial) return 1;
+    trial_pending = true;
+    trial_plant_ms = ms_per_vertex * t;
+    trial_ratio = t / std::max(1.0, l);
+    ++trials;
+    return 0;
   }
 
   void order_roots(uint32_t* roots, uint32_t R) {
     if (!trained || pred_r0 < 0 || roots[0] < pred_r0 || roots[0] + R > pred_r0 + pred_cnt) return;
-    std::stable_sort(roots, roots + R, [&](uint32_t a, uint32_t b) { return pred_tree(a) > pred_tree(b); });
+    if (switched || trial_pending)  // pruned trees: their work follows their labels
+      std::stable_sort(roots, roots + R, [&](uint32_t a, uint32_t b) { return pred_labels(a) > pred_labels(b); });
+    else
+      std::stable_sort(roots, roots + R, [&](uint32_t a, uint32_t b) { return pred_tree(a) > pred_tree(b); });
   }
 
   // The whole budget once the batch is predicted to need more than a few million
@@ -215,9 +290,13 @@ struct Learned {
   }
 
   // Pilot sample: 512 ranks log-uniform over [max(1024, n/128), n) (trees there are
-  // cheap), so the first real batch is already packed by prediction; the top
-  // ranks are left to the trend.
+  // cheap on road-like graphs), so the first real batch is already packed by
+  // prediction; the top ranks are left to the trend. Not in hybrid mode: on
+  // scale-free graphs low-ranked PLaNT trees cover most of the graph, and the
+  // pilot's slowest tree would cost as much as the first batch; there the first
+  // batch (the top ranks, static width) trains the network.
   uint32_t pilot(uint32_t* roots, uint32_t cap) {
+    if (hybrid) return 0;
     const int64_t lo = std::min<int64_t>(std::max<int64_t>(1024, n / 128), n / 2);
     if (n - lo < 64) return 0;
     const uint32_t R = (uint32_t)std::min<int64_t>(std::min<int64_t>(cap, 512), n - lo);
@@ -249,7 +328,24 @@ struct Learned {
     return CHL_OK;
   }
 
-  int on_batch(const uint32_t* roots, const uint32_t* ts, const uint32_t* lc, uint32_t R) {
+  int on_batch(const uint32_t* roots, const uint32_t* ts, const uint32_t* lc, uint32_t R, uint32_t flags, double ms) {
+    if (!(flags & TREE_FLAGS)) {
+      // pruned trees measure no PLaNT sizes; a trial decides the switch
+      if (trial_pending) {
+        trial_pending = false;
+        if (ms < trial_plant_ms) switched = true;
+        else theta = 2.0 * std::max(theta, trial_ratio);
+        if (o.verbose)
+          fprintf(stderr, "switch trial: pruned %.3f ms, PLaNT predicted %.3f ms (ratio %.1f): %s\n", ms,
+                  trial_plant_ms, trial_ratio, switched ? "switch" : "stay");
+      }
+      return CHL_OK;
+    }
+    {
+      double sum = 0;
+      for (uint32_t k = 0; k < R; ++k) sum += ts[k];
+      if (sum > 0) ms_per_vertex = ms / sum;
+    }
     // prediction error before learning from this batch
     double mean_log = 0;
     for (uint32_t k = 0; k < R; ++k) mean_log += std::log1p((double)ts[k]);
@@ -353,18 +449,42 @@ struct Learned {
   }
 };
 
+// wall time spent inside the policy (host work, copies and the MLP launches)
+struct HostTimer {
+  double& acc;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  explicit HostTimer(void* c) : acc(((Learned*)c)->host_ms) {}
+  ~HostTimer() { acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
 uint32_t hk_next_width(void* c, int64_t r0, uint32_t sw, uint32_t mw) {
+  HostTimer t(c);
   return ((Learned*)c)->next_width(r0, sw, mw);
 }
-void hk_order(void* c, uint32_t* roots, uint32_t R) { ((Learned*)c)->order_roots(roots, R); }
-uint64_t hk_out(void* c, const uint32_t* roots, uint32_t R) { return ((Learned*)c)->out_estimate(roots, R); }
-uint32_t hk_pilot(void* c, uint32_t* roots, uint32_t cap) { return ((Learned*)c)->pilot(roots, cap); }
+void hk_order(void* c, uint32_t* roots, uint32_t R) {
+  HostTimer t(c);
+  ((Learned*)c)->order_roots(roots, R);
+}
+uint64_t hk_out(void* c, const uint32_t* roots, uint32_t R) {
+  HostTimer t(c);
+  return ((Learned*)c)->out_estimate(roots, R);
+}
+uint32_t hk_pilot(void* c, uint32_t* roots, uint32_t cap) {
+  HostTimer t(c);
+  return ((Learned*)c)->pilot(roots, cap);
+}
 void hk_on_pilot(void* c, const uint32_t* roots, const uint32_t* ts, const uint32_t* lc, uint32_t R) {
+  HostTimer t(c);
   (void)((Learned*)c)->on_pilot(roots, ts, lc, R);
 }
-void hk_batch(void* c, const uint32_t* roots, const uint32_t* ts, const uint32_t* lc, uint32_t R, uint32_t) {
-  Learned* L = (Learned*)c;
-  (void)L->on_batch(roots, ts, lc, R);
+void hk_batch(void* c, const uint32_t* roots, const uint32_t* ts, const uint32_t* lc, uint32_t R, uint32_t flags,
+              double ms, uint64_t) {
+  HostTimer t(c);
+  (void)((Learned*)c)->on_batch(roots, ts, lc, R, flags, ms);
+}
+int hk_plant_next(void* c, int64_t r0, uint32_t R, int* final) {
+  HostTimer t(c);
+  return ((Learned*)c)->plant_next(r0, R, final);
 }
 
 }  // namespace
@@ -373,7 +493,11 @@ int run_build(const Graph& g, DeviceGraph& dg, const std::vector<uint32_t>& orde
               chl_build_stats* stats, LabelStore& L) {
   if (opts.batching != CHL_BATCH_LEARNED) return build_labels(g, dg, order, opts, stats, nullptr, L);
   Learned pol(g, order, opts, stats);
-  int rc = pol.init();
+  int rc;
+  {
+    HostTimer t(&pol);
+    rc = pol.init();
+  }
   if (rc) return rc;
   BuildHooks h;
   h.ctx = &pol;
@@ -383,6 +507,7 @@ int run_build(const Graph& g, DeviceGraph& dg, const std::vector<uint32_t>& orde
   h.on_batch = hk_batch;
   h.pilot = hk_pilot;
   h.on_pilot = hk_on_pilot;
+  if (opts.mode == CHL_MODE_HYBRID) h.plant_next = hk_plant_next;
   pol.hooks = &h;
   rc = build_labels(g, dg, order, opts, stats, &h, L);
   if (stats) {
@@ -390,6 +515,8 @@ int run_build(const Graph& g, DeviceGraph& dg, const std::vector<uint32_t>& orde
     stats->pred_roots = pol.mae_n;
     stats->pred_mae_learned = pol.mae_n ? pol.mae_l / pol.mae_n : 0.0;
     stats->pred_mae_static = pol.mae_n ? pol.mae_s / pol.mae_n : 0.0;
+    stats->switch_trials = pol.trials;
+    stats->policy_ms = pol.host_ms;
   }
   return rc;
 }

@@ -81,6 +81,27 @@ def test_hybrid_switches(chl, grid):
     assert_same_labels(L, ref)
 
 
+@pytest.mark.parametrize("kind", ["rmat", "knn"])
+def test_learned_hybrid_switch(chl, orc, kind):
+    # the learned policy places the switch from predicted tree/label ratios and a
+    # timed trial batch of pruned trees; every choice gives the same labels
+    if kind == "rmat":
+        g = chl.Graph.rmat(14, 3 * 16384, seed=6)
+        order = g.order_degree(seed=1)
+    else:
+        g = chl.Graph.knn(16000, 3, seed=6)
+        order = g.order_betweenness(32, seed=1)
+    _, ref = oracle_of(chl, orc, g, order)
+    L = chl.build(g, order, mode=chl.MODE_HYBRID, batching=chl.BATCH_LEARNED, verbose=1)
+    assert_same_labels(L, ref)
+    st = L.stats
+    assert st.trees_plant > 0
+    # pruned trees run only in trials and after a trial made the switch final
+    assert (st.trees_pruned > 0) == (st.switch_trials > 0)
+    if st.switch_rank >= 0:
+        assert st.switch_trials >= 1
+
+
 def test_query_all_pairs_grid(chl, grid):
     g, order, og, ref = grid
     L = chl.build(g, order, batch_roots=256)
