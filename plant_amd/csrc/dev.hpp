@@ -89,6 +89,7 @@ struct TreeParams {
   // vertex's position in the vertex order
   const uint32_t* off;
   const uint64_t* arc;
+  const uint32_t* arank;  // rank of each arc's head, parallel to arc
   const uint32_t* rank;
   uint32_t n;
   // label store (rows in rank ids, hub-sorted, all hubs < prefix)
@@ -128,6 +129,8 @@ hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, int 
 hipError_t launch_init_cells(uint64_t* cell, size_t count, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t count, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s);
+// arank[a] = rank[arc[a] >> 32] for a in [0, m)
+hipError_t launch_arc_ranks(const uint64_t* arc, uint64_t m, const uint32_t* rank, uint32_t* arank, hipStream_t s);
 
 // Label-store maintenance.
 // seg[v] = first index k of the sorted keys with key >> 32 >= v, for v in [0, n].

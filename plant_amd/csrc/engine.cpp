@@ -5,12 +5,15 @@
 // distributed_memory/query_funcs.hpp:23-30).
 //
 // Device layout (all in HBM for the whole build):
-//   * graph in rank ids (vertex i = order[i]); per arc range an offset array
-//     u32[n+1] and packed arcs u64[A] = (head << 32) | length. A symmetric graph
-//     keeps one range, since both trees of a root are then the same.
-//   * label store per side: off u64[n+1], ent u64[T] = (hub << 32) | dist, rows
-//     hub-ascending; a symmetric graph stores side 0 only (side 1 is identical).
-//   * per-slot search scratch (tree.hip), S slots x 24 n bytes.
+//   * graph in device ids, a breadth-first locality order kept with the graph
+//     handle across builds; per arc range an offset array u32[n+1] and packed
+//     arcs u64[A] = (head << 32) | length, plus this build's rank of every arc
+//     head u32[A] and of every device id u32[n]. A symmetric graph keeps one
+//     range, since both trees of a root are then the same.
+//   * label store per side, in rank ids: off u64[n+1], ent u64[T] =
+//     (hub << 32) | dist, rows hub-ascending; a symmetric graph stores side 0
+//     only (side 1 is identical).
+//   * per-slot search scratch (tree.hip), S slots x 28 n bytes.
 //   * batch output: key (vertex << 32 | hub), ent; sorted, cleaned (GLL) and
 //     appended to the store after every batch.
 //
@@ -174,6 +177,7 @@ struct Builder {
 
   DBuf<uint64_t> cell;
   DBuf<uint32_t> qa, qb, vis, gkey, gval, grank;
+  DBuf<uint32_t> arank[2];  // this build's rank of every arc head, parallel to dg.arc
   std::vector<uint32_t> id_of_rank;  // rank -> device id
   std::vector<uint32_t> hids;        // staging of root ids
   uint32_t slots = 0, gcap = 0;
@@ -217,6 +221,12 @@ struct Builder {
     }
     CHL_HIP(grank.reserve(n));
     CHL_HIP(hipMemcpyAsync(grank.p, rank_of_id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream));
+    for (int r = 0; r < 2; ++r) {
+      if (sym && r == 0) continue;
+      const uint64_t m = (uint64_t)g.off[r][n];
+      CHL_HIP(arank[r].reserve(std::max<uint64_t>(m, 1)));
+      CHL_HIP(launch_arc_ranks(dg.arc[r].p, m, grank.p, arank[r].p, stream));
+    }
     CHL_HIP(hipStreamSynchronize(stream));
     return CHL_OK;
   }
@@ -277,6 +287,7 @@ struct Builder {
     memset(&p, 0, sizeof(p));
     p.off = dg.off[sym ? 1 : range].p;
     p.arc = dg.arc[sym ? 1 : range].p;
+    p.arank = arank[sym ? 1 : range].p;
     p.n = (uint32_t)n;
     p.scan_off = L.d_off(side);
     p.scan_ent = L.d_ent(side);

@@ -68,26 +68,51 @@ __device__ __forceinline__ void table_insert(uint32_t* key, uint32_t* val, uint3
   }
 }
 
-// cell[2 v] = state word, low half of cell[2 v + 1] = round stamp
+// cell[2 v] = state word; cell[2 v + 1] = round stamp (low half) and the
+// vertex's rank (high half), both written by the one 64-bit exchange that
+// queues the vertex, so that processing and emission read one 16-byte cell and
+// no rank array. (A plain store of the rank beside the atomics on the same
+// word measured twice as slow on gfx950.)
 __device__ __forceinline__ uint32_t* stamp_of(uint64_t* cell, uint32_t v) {
   return reinterpret_cast<uint32_t*>(cell + 2 * (size_t)v + 1);
 }
+__device__ __forceinline__ uint32_t rank_of(const uint64_t* cell, uint32_t v) {
+  return reinterpret_cast<const uint32_t*>(cell + 2 * (size_t)v + 1)[1];
+}
 
-// Relaxes arc (u -> v, w) from state (d, lo); queues v in N once per round.
+// Bucket bookkeeping of the next frontier, gathered from every state change
+// of the round and from the re-queued entries: below = some queued distance is
+// under the bucket bound B, mn = the smallest queued distance. A vertex's final
+// state is the smallest of its recorded changes, so both are exact. (Whether a
+// queued vertex still has low == root rank needs the final states; it is read
+// at bucket boundaries only.)
+struct Book {
+  uint32_t below = 0, mn = 0xFFFFFFFFu;
+  __device__ __forceinline__ void note(uint32_t d, uint32_t B) {
+    below |= d < B;
+    mn = min(mn, d);
+  }
+};
+
+// Relaxes arc (u -> v, w), v of rank rv, from state (d, lo); queues v in N once
+// per round.
 template <bool FLAGS>
-__device__ __forceinline__ void relax(const TreeParams& p, uint64_t* cell, uint32_t* N, uint32_t* vis,
-                                      uint32_t* s_nsize, uint32_t* s_nvis, uint32_t nstamp, uint64_t arc, uint32_t d,
-                                      uint32_t lo) {
+__device__ __forceinline__ void relax(uint64_t* cell, uint32_t* N, uint32_t* vis, uint32_t* s_nsize,
+                                      uint32_t* s_nvis, uint32_t nstamp, uint64_t arc, uint32_t rv, uint32_t d,
+                                      uint32_t lo, uint32_t B, Book& bk) {
   const uint32_t v = (uint32_t)(arc >> 32);
   const uint32_t nd = d + (uint32_t)arc;
-  const uint32_t nlo = FLAGS ? min(lo, p.rank[v]) : lo;
+  const uint32_t nlo = FLAGS ? min(lo, rv) : lo;
   const uint64_t nv = ((uint64_t)nd << 32) | nlo;
   uint64_t* sv = cell + 2 * (size_t)v;
   if (nv < *sv) {
     const uint64_t old = atomicMin((unsigned long long*)sv, (unsigned long long)nv);
     if (nv < old) {
+      bk.note(nd, B);
       if (old == kStateInf) vis[atomicAdd(s_nvis, 1u)] = v;
-      if (atomicExch(stamp_of(cell, v), nstamp) != nstamp) N[atomicAdd(s_nsize, 1u)] = v;
+      const uint64_t os = atomicExch((unsigned long long*)(cell + 2 * (size_t)v + 1),
+                                     (unsigned long long)(((uint64_t)rv << 32) | nstamp));
+      if ((uint32_t)os != nstamp) N[atomicAdd(s_nsize, 1u)] = v;
     }
   }
 }
@@ -150,7 +175,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
     }
     if (tid == 0) {
       cell[2 * (size_t)root] = (uint64_t)rr;  // dist 0, low = root rank
-      *stamp_of(cell, root) = 1;
+      cell[2 * (size_t)root + 1] = ((uint64_t)rr << 32) | 1u;  // rank, stamp 1
       F[0] = root;
       vis[0] = root;
       s_fsize = 1;
@@ -180,6 +205,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
       }
       __syncthreads();
       const uint32_t nstamp = round + 1;
+      Book bk;
 
       if (!PRUNE) {
         // one lane per frontier entry
@@ -189,15 +215,16 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
           const uint32_t d = (uint32_t)(x >> 32);
           const uint32_t lo = (uint32_t)x;
           if (d >= B) {
+            bk.note(d, B);
             if (atomicExch(stamp_of(cell, u), nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = u;
             continue;
           }
-          if (!FLAGS && p.rank[u] < skip) continue;
+          if (!FLAGS && rank_of(cell, u) < skip) continue;
           ++c_proc;
           const uint32_t ab = p.off[u], ae = p.off[u + 1];
           c_arcs += ae - ab;
           for (uint32_t a = ab; a < ae; ++a)
-            relax<FLAGS>(p, cell, N, vis, &s_nsize, &s_nvis, nstamp, p.arc[a], d, lo);
+            relax<FLAGS>(cell, N, vis, &s_nsize, &s_nvis, nstamp, p.arc[a], p.arank[a], d, lo, B, bk);
         }
       } else {
         // one wavefront per frontier entry: prune check, then relax
@@ -207,10 +234,13 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
           const uint32_t d = (uint32_t)(x >> 32);
           const uint32_t lo = (uint32_t)x;
           if (d >= B) {
-            if (lane == 0 && atomicExch(stamp_of(cell, u), nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = u;
+            if (lane == 0) {
+              bk.note(d, B);
+              if (atomicExch(stamp_of(cell, u), nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = u;
+            }
             continue;
           }
-          const uint32_t ru = p.rank[u];
+          const uint32_t ru = rank_of(cell, u);
           if (ru < skip || lo == 0) continue;  // lo == 0: pruned at this distance already
           if (lane == 0) ++c_proc;
           bool pruned = false;
@@ -238,24 +268,28 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
           const uint32_t ab = p.off[u], ae = p.off[u + 1];
           if (lane == 0) c_arcs += ae - ab;
           for (uint32_t a = ab + lane; a < ae; a += 64)
-            relax<FLAGS>(p, cell, N, vis, &s_nsize, &s_nvis, nstamp, p.arc[a], d, lo);
+            relax<FLAGS>(cell, N, vis, &s_nsize, &s_nvis, nstamp, p.arc[a], p.arank[a], d, lo, B, bk);
         }
       }
-      __syncthreads();
-
-      // Bucket bookkeeping over the next frontier, read at its current state.
-      const uint32_t nsize = s_nsize;
-      uint32_t below = 0, clean = 0, mn = kEmpty;
-      for (uint32_t k = tid; k < nsize; k += kThreads) {
-        const uint64_t x = cell[2 * (size_t)N[k]];
-        const uint32_t d = (uint32_t)(x >> 32);
-        below |= d < B;
-        clean |= FLAGS && (uint32_t)x == rr;
-        mn = min(mn, d);
+      // bookkeeping of the next frontier (wavefront reductions, then LDS)
+      const bool any_below = __any(bk.below != 0);
+      uint32_t mn = bk.mn;
+      for (int off = 32; off > 0; off >>= 1) mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
+      if (lane == 0) {
+        if (any_below) s_below = 1;
+        if (mn != kEmpty) atomicMin(&s_minnext, mn);
       }
-      if (below) s_below = 1;
-      if (clean) s_clean = 1;
-      if (mn != kEmpty) atomicMin(&s_minnext, mn);
+      __syncthreads();
+      const uint32_t nsize = s_nsize;
+      if (FLAGS && s_below == 0) {
+        // bucket boundary: is some queued vertex still reached through the root
+        // alone? (stops at the first one found)
+        for (uint32_t k = tid; k < nsize; k += kThreads) {
+          if (s_clean) break;
+          if ((uint32_t)cell[2 * (size_t)N[k]] == rr) s_clean = 1;
+        }
+        __syncthreads();
+      }
       if (tid == 0) s_fsize = nsize;
       uint32_t* t = F;
       F = N;
@@ -276,7 +310,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
     uint32_t mine = 0;
     for (uint32_t k = tid; k < nvis; k += kThreads) {
       const uint32_t v = vis[k];
-      mine += ((uint32_t)cell[2 * (size_t)v] == rr && p.rank[v] >= rr) ? 1u : 0u;
+      mine += ((uint32_t)cell[2 * (size_t)v] == rr && rank_of(cell, v) >= rr) ? 1u : 0u;
     }
     if (mine) atomicAdd(&s_emit, mine);
     __syncthreads();
@@ -301,14 +335,14 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
     for (uint32_t k = tid; k < nvis; k += kThreads) {
       const uint32_t v = vis[k];
       const uint64_t x = cell[2 * (size_t)v];
-      const uint32_t rv = p.rank[v];
+      const uint32_t rv = rank_of(cell, v);
       if ((uint32_t)x == rr && rv >= rr && obase != ~0ull) {
         const unsigned long long o = obase + atomicAdd(&s_cursor, 1u);
         p.out_key[o] = ((uint64_t)rv << 32) | rr;
         p.out_ent[o] = ((uint64_t)rr << 32) | (x >> 32);
       }
       cell[2 * (size_t)v] = kStateInf;
-      *stamp_of(cell, v) = 0;
+      cell[2 * (size_t)v + 1] = 0;
     }
     // tree size = processed entries of this root
     if (c_proc) atomicAdd(&s_ctr[0], c_proc);
@@ -345,6 +379,11 @@ __global__ void fill_u64_kernel(uint64_t* p, uint64_t v, size_t count) {
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t count) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
     p[i] = v;
+}
+
+__global__ void arc_ranks_kernel(const uint64_t* arc, uint64_t m, const uint32_t* rank, uint32_t* arank) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x)
+    arank[i] = rank[arc[i] >> 32];
 }
 
 static unsigned fill_blocks(size_t count) {
@@ -385,6 +424,11 @@ hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t count, hipStream_t s)
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s) {
   if (!count) return hipSuccess;
   hipLaunchKernelGGL(fill_u32_kernel, dim3(fill_blocks(count)), dim3(256), 0, s, p, v, count);
+  return hipGetLastError();
+}
+hipError_t launch_arc_ranks(const uint64_t* arc, uint64_t m, const uint32_t* rank, uint32_t* arank, hipStream_t s) {
+  if (!m) return hipSuccess;
+  hipLaunchKernelGGL(arc_ranks_kernel, dim3(fill_blocks(m)), dim3(256), 0, s, arc, m, rank, arank);
   return hipGetLastError();
 }
 
