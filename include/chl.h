@@ -185,8 +185,10 @@ int chl_mlp_forward(const float* params, const float* x, int64_t n, float* y);
 int chl_mlp_grad(const float* params, const float* x, const float* t, int64_t n, float* grad, float* loss);
 /* full-batch Adam (0.9, 0.999, 1e-8) for `steps` steps; params updated in place */
 int chl_mlp_train(float* params, const float* x, const float* t, int64_t n, int steps, float lr, float* loss_curve);
-/* one training step captured as a HIP graph, replayed `replays` times, each replay
-   checked against the same step run eagerly: *max_rel = largest relative difference */
+/* one training step captured as a HIP graph, replayed `replays` times; before each
+   replay the eager run's state is copied in, and the replay is checked against one
+   eager step from that state: *max_rel = largest norm-relative difference of the
+   gradient or of the updated parameters */
 int chl_mlp_graph_check(const float* params, const float* x, const float* t, int64_t n, int replays, float lr,
                         float* max_rel);
 /* one 32x32x8 BF16 matrix-core product D = A B (row-major; A 32x8, B 8x32 as BF16 bits) */

@@ -119,6 +119,7 @@ struct TreeParams {
   // per root statistics and launch counters
   uint32_t* tree_size;
   uint32_t* label_count;
+  uint32_t* touched;  // optional: vertices reached per root
   unsigned long long* ctr;  // CTR_* (CTR_NEXT is the work-queue cursor)
 };
 

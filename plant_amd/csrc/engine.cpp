@@ -183,7 +183,7 @@ struct Builder {
   uint32_t slots = 0, gcap = 0;
   uint32_t active = 0;  // slots used by launches (at most slots)
   int wave = 0;         // 1: wavefront per root
-  DBuf<uint32_t> roots, roots2, spill, tree_size, label_count;
+  DBuf<uint32_t> roots, roots2, spill, tree_size, label_count, touched;
   uint32_t W = 1;            // widest batch (scratch offset of the statistics arrays)
   int64_t stats_batches = 0;
   DBuf<unsigned long long> ctr;
@@ -310,6 +310,7 @@ struct Builder {
     p.out_cap = std::min<uint64_t>(b.key.n, b.ent.n);
     p.spill = spill.p;
     p.tree_size = tree_size.p;
+    p.touched = o.verbose ? touched.p : nullptr;
     p.label_count = label_count.p;
     p.ctr = ctr.p;
     CHL_HIP(launch_fill_u64((uint64_t*)ctr.p, 0, CTR_COUNT, stream));
@@ -328,6 +329,36 @@ struct Builder {
       if (stats) stats->relax_ms += t;
       if (o.verbose)
         fprintf(stderr, "  side %d pass %d: %u roots, %.3f ms, written %llu, spilled %llu (%llu labels), processed %llu arcs %llu rounds %llu\n", side, pass, p.R, t, hc[CTR_LABELS], hc[CTR_OVERFLOW], hc[CTR_SPILL], hc[CTR_PROCESSED], hc[CTR_ARCS], hc[CTR_ROUNDS]);
+      if (o.verbose >= 2 && pass == 0 && p.touched) {
+        // per-root distributions of processed and reached vertices
+        std::vector<uint32_t> a(p.R), b(p.R);
+        CHL_HIP(hipMemcpy(a.data(), p.tree_size, sizeof(uint32_t) * p.R, hipMemcpyDeviceToHost));
+        CHL_HIP(hipMemcpy(b.data(), p.touched, sizeof(uint32_t) * p.R, hipMemcpyDeviceToHost));
+        std::sort(a.begin(), a.end());
+        std::sort(b.begin(), b.end());
+        auto q = [&](const std::vector<uint32_t>& x, double f) { return x[(size_t)(f * (x.size() - 1))]; };
+        size_t fit[4] = {0, 0, 0, 0};
+        double wsum = 0, wfit[4] = {0, 0, 0, 0};
+        for (uint32_t k = 0; k < p.R; ++k) wsum += a[k];
+        for (uint32_t k = 0; k < p.R; ++k)
+          for (int c = 0; c < 4; ++c)
+            if (b[k] <= (2048u << c)) ++fit[c];
+        std::vector<std::pair<uint32_t, uint32_t>> ab(p.R);
+        {
+          std::vector<uint32_t> a2(p.R), b2(p.R);
+          CHL_HIP(hipMemcpy(a2.data(), p.tree_size, sizeof(uint32_t) * p.R, hipMemcpyDeviceToHost));
+          CHL_HIP(hipMemcpy(b2.data(), p.touched, sizeof(uint32_t) * p.R, hipMemcpyDeviceToHost));
+          for (uint32_t k = 0; k < p.R; ++k)
+            for (int c = 0; c < 4; ++c)
+              if (b2[k] <= (2048u << c)) wfit[c] += a2[k];
+        }
+        fprintf(stderr,
+                "    processed p50 %u p90 %u p99 %u max %u | reached p50 %u p90 %u p99 %u max %u | roots reaching "
+                "<= 2k/4k/8k/16k: %.3f %.3f %.3f %.3f (work %.3f %.3f %.3f %.3f)\n",
+                q(a, .5), q(a, .9), q(a, .99), a.back(), q(b, .5), q(b, .9), q(b, .99), b.back(),
+                (double)fit[0] / p.R, (double)fit[1] / p.R, (double)fit[2] / p.R, (double)fit[3] / p.R,
+                wfit[0] / wsum, wfit[1] / wsum, wfit[2] / wsum, wfit[3] / wsum);
+      }
 
       if (!hc[CTR_OVERFLOW]) break;
       // second launch for the spilled roots (their statistics are already recorded)
@@ -554,6 +585,7 @@ struct Builder {
     CHL_HIP(roots2.reserve(W));
     CHL_HIP(spill.reserve(W));
     CHL_HIP(tree_size.reserve(2 * (size_t)W));
+    if (o.verbose) CHL_HIP(touched.reserve(2 * (size_t)W));
     CHL_HIP(label_count.reserve(2 * (size_t)W));
     CHL_HIP(ctr.reserve(CTR_COUNT));
     CHL_HIP(misc.reserve(4));

@@ -329,6 +329,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
       }
       s_base = b;
       p.label_count[ri] = emit;
+      if (p.touched) p.touched[ri] = nvis;
     }
     __syncthreads();
     const unsigned long long obase = s_base;

@@ -52,8 +52,11 @@ def test_gradient_matches_reference(chl):
 
 
 def test_graph_replays_match_eager(chl):
-    # the captured training step, replayed 12 times with eager steps and
-    # synchronisations in between; FP32 atomics reorder sums, so 1e-3 relative
+    # the captured training step, replayed 12 times with eager steps, state
+    # copies and synchronisations in between, each replay against one eager step
+    # from the same state; FP32 atomics reorder the gradient sums, and a
+    # near-zero gradient whose sign flips moves its parameter by 2 lr under Adam,
+    # so 1e-3 of the gradient's and the parameters' norms
     rng = np.random.default_rng(3)
     x = features(rng, 700)
     t = rng.normal(size=(700, M.O)).astype(np.float32)
