@@ -9,18 +9,19 @@
 // Every function cites the reference file:line it follows. Pinning (what checks
 // each function against something other than itself):
 //   * orc_graph_*           : hand-written DIMACS/METIS/SNAP/edgelist fixtures with known
-//                             arc sets (tests/test_oracle_formats.py) and the reference
-//                             `hhl` binary's labels on the same files.
+//                             arc sets (tests/test_host.py) and the reference `hhl`
+//                             binary's labels on the same files.
 //   * orc_dijkstra          : scipy.sparse.csgraph.dijkstra golden vectors and
 //                             Floyd-Warshall on tiny graphs.
 //   * orc_gll               : the reference shared_memory `hhl` built from source
 //                             (oracle/_ref/hhl) — byte-identical label files — and the
 //                             brute-force CHL definition (orc_brute_chl).
-//   * orc_plant_tree        : brute-force CHL restricted to one hub; closed forms
-//                             (path / star graphs).
+//   * orc_plant_range       : equality with orc_gll and with the brute-force CHL
+//                             (tests/test_oracle.py); closed forms (path / star graphs).
 //   * orc_hybrid            : equality with orc_gll (CHL is unique for a graph and order);
 //                             the distributed reference needs MPI and is not built here.
-//   * orc_clean_cover       : brute force over small label sets (tests).
+//   * orc_clean_cover       : runs inside orc_gll's cleaning, so the byte-identical
+//                             labels against the reference binary pin it.
 //   * orc_query             : Dijkstra on 1e5 pairs and all pairs on tiny graphs.
 //   * orc_brute_chl         : closed forms (path with midpoint order, star) and the
 //                             2-hop cover property against Floyd-Warshall.

@@ -80,8 +80,10 @@ static constexpr uint64_t kStateInf = ~0ull;
 // indices go to spill[]); CTR_SPILL: labels those roots still have to write.
 // CTR_SPILL_START: smallest output offset reserved by a root that did not fit
 // (all ones when none), so [0, CTR_SPILL_START) is the written prefix.
+// CTR_SKIPPED: roots not started because the output was already full (they are
+// in spill[] too, with unknown label counts).
 enum { CTR_PROCESSED = 0, CTR_ARCS = 1, CTR_PRUNE_ENT = 2, CTR_LABELS = 3, CTR_OVERFLOW = 4, CTR_NEXT = 5,
-       CTR_SPILL = 6, CTR_ROUNDS = 7, CTR_SPILL_START = 8, CTR_COUNT = 10 };
+       CTR_SPILL = 6, CTR_ROUNDS = 7, CTR_SPILL_START = 8, CTR_SKIPPED = 9, CTR_COUNT = 10 };
 
 struct TreeParams {
   // graph in device ids (a locality order): arcs of the range walked by the tree
@@ -132,6 +134,8 @@ hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t count, hipStream_t s)
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s);
 // arank[a] = rank[arc[a] >> 32] for a in [0, m)
 hipError_t launch_arc_ranks(const uint64_t* arc, uint64_t m, const uint32_t* rank, uint32_t* arank, hipStream_t s);
+// dst[idx[j]] = src[j] for j in [0, m)
+hipError_t launch_scatter_u32(const uint32_t* src, const uint32_t* idx, uint32_t m, uint32_t* dst, hipStream_t s);
 
 // Label-store maintenance.
 // seg[v] = first index k of the sorted keys with key >> 32 >= v, for v in [0, n].

@@ -183,7 +183,7 @@ struct Builder {
   uint32_t slots = 0, gcap = 0;
   uint32_t active = 0;  // slots used by launches (at most slots)
   int wave = 0;         // 1: wavefront per root
-  DBuf<uint32_t> roots, roots2, spill, tree_size, label_count, touched;
+  DBuf<uint32_t> roots, roots2, spill, tree_size, label_count, touched, origd;
   uint32_t W = 1;            // widest batch (scratch offset of the statistics arrays)
   int64_t stats_batches = 0;
   DBuf<unsigned long long> ctr;
@@ -275,8 +275,9 @@ struct Builder {
   }
 
   // Runs the trees of roots[0..R) for one side into the side's output buffer;
-  // b.m = labels written. Roots whose labels did not fit are run again, alone,
-  // after the buffer has grown by exactly what they reported.
+  // b.m = labels written. Roots whose labels did not fit, and the roots not
+  // started once the buffer was full, run in a further launch after the buffer
+  // has grown by what the former reported and an estimate for the latter.
   int run_trees(int side, uint32_t R, const uint32_t* hroots, uint32_t prefix, uint32_t flags, LabelStore& L,
                 double& ms) {
     SideBatch& b = sb[side];
@@ -315,12 +316,20 @@ struct Builder {
     p.ctr = ctr.p;
     CHL_HIP(launch_fill_u64((uint64_t*)ctr.p, 0, CTR_COUNT, stream));
     CHL_HIP(launch_fill_u64((uint64_t*)ctr.p + CTR_SPILL_START, ~0ull, 1, stream));
-    std::vector<uint32_t> cur(hroots, hroots + R), idx;
+    std::vector<uint32_t> cur(hroots, hroots + R), idx, next, orig(R), next_orig;
+    for (uint32_t j = 0; j < R; ++j) orig[j] = j;  // position of each current root in the batch
     unsigned long long hc[CTR_COUNT];
+    uint64_t lab0 = 0;  // label counter at the start of the pass
     for (int pass = 0;; ++pass) {
       CHL_HIP(hipEventRecord(ev[0], stream));
       CHL_HIP(launch_tree(p, flags, std::min<uint32_t>(active, p.R), wave, stream));
       CHL_HIP(hipEventRecord(ev[1], stream));
+      if (pass > 0) {
+        // this pass's statistics to the roots' positions in the batch
+        CHL_HIP(launch_scatter_u32(p.tree_size, origd.p, p.R, tree_size.p, stream));
+        CHL_HIP(launch_scatter_u32(p.label_count, origd.p, p.R, label_count.p, stream));
+        if (p.touched) CHL_HIP(launch_scatter_u32(p.touched, origd.p, p.R, touched.p, stream));
+      }
       CHL_HIP(hipMemcpyAsync(hc, ctr.p, sizeof(hc), hipMemcpyDeviceToHost, stream));
       CHL_HIP(hipStreamSynchronize(stream));
       float t = 0;
@@ -328,48 +337,32 @@ struct Builder {
       ms += t;
       if (stats) stats->relax_ms += t;
       if (o.verbose)
-        fprintf(stderr, "  side %d pass %d: %u roots, %.3f ms, written %llu, spilled %llu (%llu labels), processed %llu arcs %llu rounds %llu\n", side, pass, p.R, t, hc[CTR_LABELS], hc[CTR_OVERFLOW], hc[CTR_SPILL], hc[CTR_PROCESSED], hc[CTR_ARCS], hc[CTR_ROUNDS]);
-      if (o.verbose >= 2 && pass == 0 && p.touched) {
-        // per-root distributions of processed and reached vertices
-        std::vector<uint32_t> a(p.R), b(p.R);
-        CHL_HIP(hipMemcpy(a.data(), p.tree_size, sizeof(uint32_t) * p.R, hipMemcpyDeviceToHost));
-        CHL_HIP(hipMemcpy(b.data(), p.touched, sizeof(uint32_t) * p.R, hipMemcpyDeviceToHost));
-        std::sort(a.begin(), a.end());
-        std::sort(b.begin(), b.end());
-        auto q = [&](const std::vector<uint32_t>& x, double f) { return x[(size_t)(f * (x.size() - 1))]; };
-        size_t fit[4] = {0, 0, 0, 0};
-        double wsum = 0, wfit[4] = {0, 0, 0, 0};
-        for (uint32_t k = 0; k < p.R; ++k) wsum += a[k];
-        for (uint32_t k = 0; k < p.R; ++k)
-          for (int c = 0; c < 4; ++c)
-            if (b[k] <= (2048u << c)) ++fit[c];
-        std::vector<std::pair<uint32_t, uint32_t>> ab(p.R);
-        {
-          std::vector<uint32_t> a2(p.R), b2(p.R);
-          CHL_HIP(hipMemcpy(a2.data(), p.tree_size, sizeof(uint32_t) * p.R, hipMemcpyDeviceToHost));
-          CHL_HIP(hipMemcpy(b2.data(), p.touched, sizeof(uint32_t) * p.R, hipMemcpyDeviceToHost));
-          for (uint32_t k = 0; k < p.R; ++k)
-            for (int c = 0; c < 4; ++c)
-              if (b2[k] <= (2048u << c)) wfit[c] += a2[k];
-        }
-        fprintf(stderr,
-                "    processed p50 %u p90 %u p99 %u max %u | reached p50 %u p90 %u p99 %u max %u | roots reaching "
-                "<= 2k/4k/8k/16k: %.3f %.3f %.3f %.3f (work %.3f %.3f %.3f %.3f)\n",
-                q(a, .5), q(a, .9), q(a, .99), a.back(), q(b, .5), q(b, .9), q(b, .99), b.back(),
-                (double)fit[0] / p.R, (double)fit[1] / p.R, (double)fit[2] / p.R, (double)fit[3] / p.R,
-                wfit[0] / wsum, wfit[1] / wsum, wfit[2] / wsum, wfit[3] / wsum);
-      }
-
+        fprintf(stderr, "  side %d pass %d: %u roots, %.3f ms, written %llu, spilled %llu (%llu labels, %llu not started), processed %llu arcs %llu rounds %llu\n", side, pass, p.R, t, hc[CTR_LABELS], hc[CTR_OVERFLOW], hc[CTR_SPILL], hc[CTR_SKIPPED], hc[CTR_PROCESSED], hc[CTR_ARCS], hc[CTR_ROUNDS]);
       if (!hc[CTR_OVERFLOW]) break;
-      // second launch for the spilled roots (their statistics are already recorded)
+      // next launch for the roots that did not fit or were not started
       const uint32_t ns = (uint32_t)hc[CTR_OVERFLOW];
+      const uint64_t skipped = hc[CTR_SKIPPED];
       idx.resize(ns);
       CHL_HIP(hipMemcpy(idx.data(), spill.p, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost));
-      std::vector<uint32_t> next(ns);
-      for (uint32_t j = 0; j < ns; ++j) next[j] = cur[idx[j]];
+      next.resize(ns);
+      next_orig.resize(ns);
+      for (uint32_t j = 0; j < ns; ++j) {
+        next[j] = cur[idx[j]];
+        next_orig[j] = orig[idx[j]];
+      }
       cur.swap(next);
+      orig.swap(next_orig);
       const uint64_t written = hc[CTR_SPILL_START];  // contiguous prefix of fitting roots
-      const uint64_t need = written + hc[CTR_SPILL];
+      // labels still to write: exact for the roots that ran, and for the roots
+      // not started 1.5 times the mean of the roots that ran in this pass
+      // (bounded by a 48th of the free memory: key and entry now, their sorted
+      // copies and the store append later; a short estimate costs one more pass)
+      const uint64_t ran = p.R - skipped;
+      const double mean = ran ? (double)(hc[CTR_LABELS] - lab0) / (double)ran : 0.0;
+      size_t freeb = 0, totalb = 0;
+      CHL_HIP(hipMemGetInfo(&freeb, &totalb));
+      const double est = std::min((double)freeb / 48.0, 1.5 * mean * (double)skipped);
+      const uint64_t need = written + hc[CTR_SPILL] + (uint64_t)est + 4096;
       out_cap = std::max<uint64_t>(out_cap, need + need / 8);
       rc = grow_keep(b.key, out_cap, written);
       if (rc) return rc;
@@ -378,16 +371,45 @@ struct Builder {
       hids.resize(ns);
       for (uint32_t j = 0; j < ns; ++j) hids[j] = id_of_rank[cur[j]];
       CHL_HIP(hipMemcpy(roots2.p, hids.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
+      CHL_HIP(hipMemcpy(origd.p, orig.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
       p.roots = roots2.p;
       p.R = ns;
       p.out_key = b.key.p;
       p.out_ent = b.ent.p;
       p.out_cap = out_cap;
-      p.tree_size = tree_size.p + W;  // scratch: the first pass holds these roots' statistics
+      p.tree_size = tree_size.p + W;  // scratch, scattered back after the launch
       p.label_count = label_count.p + W;
+      if (p.touched) p.touched = touched.p + W;
+      lab0 = written;
       CHL_HIP(launch_fill_u64((uint64_t*)ctr.p + CTR_LABELS, written, 1, stream));
       CHL_HIP(launch_fill_u64((uint64_t*)ctr.p + CTR_OVERFLOW, 0, 3, stream));  // OVERFLOW, NEXT, SPILL
       CHL_HIP(launch_fill_u64((uint64_t*)ctr.p + CTR_SPILL_START, ~0ull, 1, stream));
+      CHL_HIP(launch_fill_u64((uint64_t*)ctr.p + CTR_SKIPPED, 0, 1, stream));
+    }
+    if (o.verbose >= 2 && touched.p) {
+      // per-root distributions of processed and reached vertices
+      std::vector<uint32_t> a(R), c(R);
+      CHL_HIP(hipMemcpy(a.data(), tree_size.p, sizeof(uint32_t) * R, hipMemcpyDeviceToHost));
+      CHL_HIP(hipMemcpy(c.data(), touched.p, sizeof(uint32_t) * R, hipMemcpyDeviceToHost));
+      double wsum = 0, wfit[4] = {0, 0, 0, 0};
+      size_t fit[4] = {0, 0, 0, 0};
+      for (uint32_t k = 0; k < R; ++k) {
+        wsum += a[k];
+        for (int q = 0; q < 4; ++q)
+          if (c[k] <= (2048u << q)) {
+            ++fit[q];
+            wfit[q] += a[k];
+          }
+      }
+      std::sort(a.begin(), a.end());
+      std::sort(c.begin(), c.end());
+      auto pct = [&](const std::vector<uint32_t>& x, double f) { return x[(size_t)(f * (x.size() - 1))]; };
+      fprintf(stderr,
+              "    processed p50 %u p90 %u p99 %u max %u | reached p50 %u p90 %u p99 %u max %u | roots reaching "
+              "<= 2k/4k/8k/16k: %.3f %.3f %.3f %.3f (work %.3f %.3f %.3f %.3f)\n",
+              pct(a, .5), pct(a, .9), pct(a, .99), a.back(), pct(c, .5), pct(c, .9), pct(c, .99), c.back(),
+              (double)fit[0] / R, (double)fit[1] / R, (double)fit[2] / R, (double)fit[3] / R, wfit[0] / wsum,
+              wfit[1] / wsum, wfit[2] / wsum, wfit[3] / wsum);
     }
     b.m = hc[CTR_LABELS];
     if (stats) {
@@ -535,9 +557,12 @@ struct Builder {
     CHL_HIP(hipEventRecord(ev[6], stream));
     const uint64_t newt = L.total[side] + m;
     CHL_HIP(noff.reserve(n + 1));
+    // the merged store goes to the spare buffer, which keeps a quarter of slack
+    // so that most batches reuse it (a fresh multi-GB allocation per batch
+    // costs far more than the merge)
     if (nent.n < newt) {
       nent.release();
-      CHL_HIP(nent.reserve(newt + newt / 2));
+      CHL_HIP(nent.reserve(newt + newt / 4));
     }
     CHL_HIP(launch_merge_offsets(L.off[side].p, b.seg.p, (uint32_t)n, noff.p, stream));
     CHL_HIP(launch_merge_rows(L.off[side].p, L.ent[side].p, b.seg.p, noff.p, b.sent.p, (uint32_t)n, nent.p, stream));
@@ -584,6 +609,7 @@ struct Builder {
     CHL_HIP(roots.reserve(W));
     CHL_HIP(roots2.reserve(W));
     CHL_HIP(spill.reserve(W));
+    CHL_HIP(origd.reserve(W));
     CHL_HIP(tree_size.reserve(2 * (size_t)W));
     if (o.verbose) CHL_HIP(touched.reserve(2 * (size_t)W));
     CHL_HIP(label_count.reserve(2 * (size_t)W));

@@ -51,7 +51,8 @@ struct Learned {
   bool trained = false;
   int64_t pred_r0 = -1;
   uint32_t pred_cnt = 0;
-  std::vector<float> pred;  // [pred_cnt][2]
+  std::vector<float> pred;  // [pred_cnt][2], log1p units
+  std::vector<float> lin;   // [pred_cnt][2], expm1 of pred
   double last_mean_log = -1.0;
   double mae_l = 0, mae_s = 0;
   int64_t mae_n = 0;
@@ -77,8 +78,16 @@ struct Learned {
   double theta = 16.0, ms_per_vertex = 0.0, trial_plant_ms = 0.0, trial_ratio = 0.0;
   int64_t decided_r0 = -1;
   bool decided_trial = false;
+  uint32_t last_width = 0;  // width of the last batch this policy sized
   int64_t trials = 0;
   double host_ms = 0;  // wall time inside the policy
+  // Sums of many per-root predictions made in log space come out low (the mean
+  // of exp is above exp of the mean); the measured over predicted sums of the
+  // pilot and of every finished batch scale the predicted sums back.
+  double calib_t = 1.0, calib_l = 1.0;
+  std::vector<float> hxs;    // host copy of the features, rows by rank
+  std::vector<float> lrank;  // log1p(rank), the trend's abscissa
+  DBuf<float> Xp, Yp;      // gathered rows and their predictions
 
   Learned(const Graph& gg, const std::vector<uint32_t>& ord, const chl_build_opts& opts, chl_build_stats* s)
       : g(gg), order(ord), o(opts), stats(s), n(gg.n), hybrid(opts.mode == CHL_MODE_HYBRID) {}
@@ -139,6 +148,9 @@ struct Learned {
     }
     CHL_HIP(X.reserve(hx.size()));
     CHL_HIP(hipMemcpy(X.p, hx.data(), sizeof(float) * hx.size(), hipMemcpyHostToDevice));
+    hxs.swap(hx);
+    lrank.resize(n);
+    for (int64_t r = 0; r < n; ++r) lrank[r] = (float)std::log1p((double)r);
     // parameters: uniform(+-sqrt(6/(fan_in+fan_out))) weights, zero biases
     std::vector<float> hp(kMlpParams, 0.0f);
     auto fill = [&](int off, int fin, int fout) {
@@ -190,8 +202,17 @@ struct Learned {
     pred.resize((size_t)cnt * kMlpO);
     CHL_HIP(hipMemcpyAsync(pred.data(), Y.p, sizeof(float) * cnt * kMlpO, hipMemcpyDeviceToHost, stream));
     timed_end();
-    for (uint32_t k = 0; k < cnt; ++k)
-      for (int o = 0; o < kMlpO; ++o) pred[(size_t)k * kMlpO + o] += (float)trend(o, (uint32_t)(r0 + k));
+    // add the trend; keep the linear values (tree size, labels) beside the logs
+    lin.resize((size_t)cnt * kMlpO);
+    const float a0 = (float)ta[0], a1 = (float)ta[1], b0 = (float)tb[0], b1 = (float)tb[1];
+    parallel_for(cnt, host_threads(), 16384, [&](int64_t k) {
+      const float x = lrank[r0 + k];
+      float* y = &pred[(size_t)k * kMlpO];
+      y[0] += a0 + b0 * x;
+      y[1] += a1 + b1 * x;
+      lin[(size_t)k * kMlpO] = std::expm1(std::max(0.0f, y[0]));
+      lin[(size_t)k * kMlpO + 1] = std::expm1(std::max(0.0f, y[1]));
+    });
     pred_r0 = r0;
     pred_cnt = cnt;
     return CHL_OK;
@@ -199,6 +220,8 @@ struct Learned {
 
   float pred_tree(uint32_t rank) const { return pred[(size_t)(rank - pred_r0) * kMlpO]; }
   float pred_labels(uint32_t rank) const { return pred[(size_t)(rank - pred_r0) * kMlpO + 1]; }
+  float lin_tree(uint32_t rank) const { return lin[(size_t)(rank - pred_r0) * kMlpO]; }
+  float lin_labels(uint32_t rank) const { return lin[(size_t)(rank - pred_r0) * kMlpO + 1]; }
 
   // Predicted sums of tree sizes and labels over ranks [a, b) (inside the
   // predicted range).
@@ -206,21 +229,26 @@ struct Learned {
     tree = 0;
     labels = 0;
     for (int64_t r = a; r < b; ++r) {
-      tree += std::expm1((double)std::max(0.0f, pred_tree((uint32_t)r)));
-      labels += std::expm1((double)std::max(0.0f, pred_labels((uint32_t)r)));
+      tree += lin_tree((uint32_t)r);
+      labels += lin_labels((uint32_t)r);
     }
   }
   double pred_ratio(int64_t a, int64_t b) const {
     double t, l;
     pred_sums(a, b, t, l);
-    return t / std::max(1.0, l);
+    return t * calib_t / std::max(1.0, l * calib_l);
   }
 
   uint32_t next_width(int64_t r0, uint32_t static_width, uint32_t max_width) {
     (void)static_width;
     decided_r0 = -1;
     if (!trained || switched) return 0;
-    const uint32_t cnt = (uint32_t)std::min<int64_t>(max_width, n - r0);
+    // at most a quarter of the vertices and eight times the previous width per
+    // batch, so that an error of the predicted label total (heavy-tailed sums
+    // from a small sample) costs a few more batches, not the device memory
+    int64_t cap = std::min<int64_t>(max_width, std::max<int64_t>(65536, n / 4));
+    if (last_width) cap = std::min<int64_t>(cap, std::max<int64_t>(8 * (int64_t)last_width, 4096));
+    const uint32_t cnt = (uint32_t)std::min<int64_t>(cap, n - r0);
     if (predict(r0, cnt)) return 0;
     // the widest batch whose predicted labels fit the output budget (every batch
     // boundary is a device-wide wait for the batch's last tree), and at least one
@@ -236,7 +264,7 @@ struct Learned {
     double labels = 0;
     uint32_t R = 0;
     while (R < cnt) {
-      labels += std::expm1((double)std::max(0.0f, pred_labels((uint32_t)(r0 + R))));
+      labels += calib_l * lin_labels((uint32_t)(r0 + R));
       if (2.0 * labels > label_budget && R >= slots) break;  // 2x margin on the prediction
       // hybrid: end the batch before the first block predicted to cross theta
       if (hybrid && R >= slots && R % kBlock == 0 && R + kBlock <= cnt &&
@@ -246,6 +274,7 @@ struct Learned {
     }
     decided_r0 = r0;
     decided_trial = false;
+    last_width = R;
     return R;
   }
 
@@ -260,6 +289,8 @@ struct Learned {
     if (predict(r0, R)) return 1;
     double t, l;
     pred_sums(r0, r0 + R, t, l);
+    t *= calib_t;
+    l *= calib_l;
     const bool trial = decided_r0 == r0 ? decided_trial : t / std::max(1.0, l) >= theta;
     if (!trial) return 1;
     trial_pending = true;
@@ -269,12 +300,28 @@ struct Learned {
     return 0;
   }
 
+  // Stable counting sort of roots by predicted output o (log1p units, 1/32
+  // steps), largest first: an approximate longest-processing-time order in O(R).
+  void lpt_order(uint32_t* roots, uint32_t R, int o) {
+    constexpr int kBuckets = 1024;
+    std::vector<uint32_t> cnt(kBuckets + 1, 0), key(R), out(R);
+    for (uint32_t k = 0; k < R; ++k) {
+      const float y = pred[(size_t)(roots[k] - pred_r0) * kMlpO + o];
+      const int b = (int)std::min<float>(kBuckets - 1, std::max(0.0f, y * 32.0f));
+      key[k] = (uint32_t)(kBuckets - 1 - b);  // descending
+      ++cnt[key[k] + 1];
+    }
+    for (int b = 0; b < kBuckets; ++b) cnt[b + 1] += cnt[b];
+    for (uint32_t k = 0; k < R; ++k) out[cnt[key[k]]++] = roots[k];
+    std::memcpy(roots, out.data(), sizeof(uint32_t) * R);
+  }
+
   void order_roots(uint32_t* roots, uint32_t R) {
     if (!trained || pred_r0 < 0 || roots[0] < pred_r0 || roots[0] + R > pred_r0 + pred_cnt) return;
     if (switched || trial_pending)  // pruned trees: their work follows their labels
-      std::stable_sort(roots, roots + R, [&](uint32_t a, uint32_t b) { return pred_labels(a) > pred_labels(b); });
+      lpt_order(roots, R, 1);
     else
-      std::stable_sort(roots, roots + R, [&](uint32_t a, uint32_t b) { return pred_tree(a) > pred_tree(b); });
+      lpt_order(roots, R, 0);
   }
 
   // The whole budget once the batch is predicted to need more than a few million
@@ -284,8 +331,9 @@ struct Learned {
     double s = 0;
     for (uint32_t k = 0; k < R; ++k) {
       if (roots[k] < pred_r0 || roots[k] >= pred_r0 + pred_cnt) return 0;
-      s += std::expm1((double)std::max(0.0f, pred_labels(roots[k])));
+      s += lin_labels(roots[k]);
     }
+    s *= calib_l;
     return (uint64_t)std::min(label_budget, std::max(2.0 * s, s > 4e6 ? label_budget : 0.0)) + 4096;
   }
 
@@ -308,6 +356,32 @@ struct Learned {
     return R;
   }
 
+  static double clamp_calib(double c) { return std::isfinite(c) ? std::min(100.0, std::max(0.5, c)) : 1.0; }
+
+  // Predicted tree and label sums (linear units, before calibration) of the
+  // roots at the given ranks.
+  int pred_sums_at(const uint32_t* ranks, uint32_t R, double& tsum, double& lsum) {
+    tsum = 0;
+    lsum = 0;
+    if (!R) return CHL_OK;
+    std::vector<float> rows((size_t)R * kMlpF);
+    for (uint32_t k = 0; k < R; ++k)
+      std::memcpy(&rows[(size_t)k * kMlpF], &hxs[(size_t)ranks[k] * kMlpF], sizeof(float) * kMlpF);
+    CHL_HIP(Xp.reserve(rows.size()));
+    CHL_HIP(Yp.reserve((size_t)R * kMlpO));
+    CHL_HIP(hipMemcpyAsync(Xp.p, rows.data(), sizeof(float) * rows.size(), hipMemcpyHostToDevice, stream));
+    timed_begin();
+    CHL_HIP(mlp_forward(Xp.p, (int)R, wb.p, P.p, Yp.p, stream));
+    timed_end();
+    std::vector<float> y((size_t)R * kMlpO);
+    CHL_HIP(hipMemcpy(y.data(), Yp.p, sizeof(float) * y.size(), hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < R; ++k) {
+      tsum += std::expm1(std::max(0.0, (double)y[(size_t)k * kMlpO] + trend(0, ranks[k])));
+      lsum += std::expm1(std::max(0.0, (double)y[(size_t)k * kMlpO + 1] + trend(1, ranks[k])));
+    }
+    return CHL_OK;
+  }
+
   int on_pilot(const uint32_t* roots, const uint32_t* ts, const uint32_t* lc, uint32_t R) {
     // the static estimate of the first batch: the sample mean
     double mean_log = 0;
@@ -325,6 +399,15 @@ struct Learned {
       int rc = train();
       if (rc) return rc;
     }
+    double pt, pl, mt = 0, ml = 0;
+    int rc = pred_sums_at(roots, R, pt, pl);
+    if (rc) return rc;
+    for (uint32_t k = 0; k < R; ++k) {
+      mt += ts[k];
+      ml += lc[k];
+    }
+    calib_t = clamp_calib(mt / pt);
+    calib_l = clamp_calib(ml / pl);
     return CHL_OK;
   }
 
@@ -355,12 +438,19 @@ struct Learned {
       for (uint32_t k = 0; k < R && covered; ++k)
         covered = roots[k] >= pred_r0 && roots[k] < pred_r0 + pred_cnt;
       if (covered) {
+        double pt = 0, pl = 0, mt = 0, ml = 0;
         for (uint32_t k = 0; k < R; ++k) {
           const double y = std::log1p((double)ts[k]);
           mae_l += std::fabs(pred_tree(roots[k]) - y);
           mae_s += std::fabs(last_mean_log - y);
+          pt += lin_tree(roots[k]);
+          pl += lin_labels(roots[k]);
+          mt += ts[k];
+          ml += lc[k];
         }
         mae_n += R;
+        calib_t = clamp_calib(mt / pt);
+        calib_l = clamp_calib(ml / pl);
       }
     }
     last_mean_log = mean_log;

@@ -143,6 +143,14 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
   for (;;) {
     if (tid == 0) {
       unsigned long long r = atomicAdd(&p.ctr[CTR_NEXT], 1ull);
+      // once a root's labels did not fit, later roots are not started: they go
+      // back to the host with the spilled ones (no tree is computed in vain)
+      while (r < p.R && __hip_atomic_load(&p.ctr[CTR_OVERFLOW], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        const unsigned long long k = atomicAdd(&p.ctr[CTR_OVERFLOW], 1ull);
+        p.spill[k] = (uint32_t)r;
+        atomicAdd(&p.ctr[CTR_SKIPPED], 1ull);
+        r = atomicAdd(&p.ctr[CTR_NEXT], 1ull);
+      }
       s_ri = r < p.R ? (uint32_t)r : kEmpty;
     }
     __syncthreads();
@@ -387,6 +395,11 @@ __global__ void arc_ranks_kernel(const uint64_t* arc, uint64_t m, const uint32_t
     arank[i] = rank[arc[i] >> 32];
 }
 
+__global__ void scatter_u32_kernel(const uint32_t* src, const uint32_t* idx, uint32_t m, uint32_t* dst) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x)
+    dst[idx[i]] = src[i];
+}
+
 static unsigned fill_blocks(size_t count) {
   size_t b = (count + 255) / 256;
   return (unsigned)(b < 8192 ? (b ? b : 1) : 8192);
@@ -425,6 +438,11 @@ hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t count, hipStream_t s)
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s) {
   if (!count) return hipSuccess;
   hipLaunchKernelGGL(fill_u32_kernel, dim3(fill_blocks(count)), dim3(256), 0, s, p, v, count);
+  return hipGetLastError();
+}
+hipError_t launch_scatter_u32(const uint32_t* src, const uint32_t* idx, uint32_t m, uint32_t* dst, hipStream_t s) {
+  if (!m) return hipSuccess;
+  hipLaunchKernelGGL(scatter_u32_kernel, dim3(fill_blocks(m)), dim3(256), 0, s, src, idx, m, dst);
   return hipGetLastError();
 }
 hipError_t launch_arc_ranks(const uint64_t* arc, uint64_t m, const uint32_t* rank, uint32_t* arank, hipStream_t s) {

@@ -119,16 +119,18 @@ def test_path_midpoint_order_closed_form(chl, orc):
         depth[v] = 0 if parent[v] is None else depth[parent[v]] + 1
     order = sorted(order, key=lambda v: (depth[v], v))
     rank = {v: i for i, v in enumerate(order)}
-    L, _ = orc.gll(g, np.array(order, np.uint32), threads=2)
-    off, hub, dist = L.copy(0)
-    for v in range(n):
-        anc, a = [], v
-        while a is not None:
-            anc.append(a)
-            a = parent[a]
-        want = sorted((rank[a], abs(a - v)) for a in anc)
-        got = list(zip(hub[off[v]:off[v + 1]].tolist(), dist[off[v]:off[v + 1]].tolist()))
-        assert got == want
+    gll, _ = orc.gll(g, np.array(order, np.uint32), threads=2)
+    plant, _, _ = orc.plant_range(g, np.array(order, np.uint32), 0, n, threads=2)
+    for L in (gll, plant):
+        off, hub, dist = L.copy(0)
+        for v in range(n):
+            anc, a = [], v
+            while a is not None:
+                anc.append(a)
+                a = parent[a]
+            want = sorted((rank[a], abs(a - v)) for a in anc)
+            got = list(zip(hub[off[v]:off[v + 1]].tolist(), dist[off[v]:off[v + 1]].tolist()))
+            assert got == want
 
 
 def test_star_closed_form(orc):
@@ -138,12 +140,14 @@ def test_star_closed_form(orc):
     g = orc.OGraph.from_arcs(n, np.r_[np.zeros(n - 1, np.uint32), leaves], np.r_[leaves, np.zeros(n - 1, np.uint32)],
                              np.r_[w, w])
     order = np.arange(n, dtype=np.uint32)
-    L, _ = orc.gll(g, order, threads=2)
-    for side in (0, 1):
-        off, hub, dist = L.copy(side)
-        assert list(hub[off[0]:off[1]]) == [0]
-        for v in range(1, n):
-            assert list(zip(hub[off[v]:off[v + 1]], dist[off[v]:off[v + 1]])) == [(0, v), (v, 0)]
+    gll, _ = orc.gll(g, order, threads=2)
+    plant, _, _ = orc.plant_range(g, order, 0, n, threads=2)
+    for L in (gll, plant):
+        for side in (0, 1):
+            off, hub, dist = L.copy(side)
+            assert list(hub[off[0]:off[1]]) == [0]
+            for v in range(1, n):
+                assert list(zip(hub[off[v]:off[v + 1]], dist[off[v]:off[v + 1]])) == [(0, v), (v, 0)]
 
 
 def test_dijkstra_matches_scipy_and_queries(chl, orc):
