@@ -39,69 +39,152 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def query_bench(chl, L, n, batches=200, batch=4096, seed=11):
-    """Queries/s of back-to-back 4096-pair batches through the server, with p50/p99."""
+class SOpts(C.Structure):
+    _fields_ = [("streams", C.c_int), ("max_inflight_pairs", C.c_int64), ("max_batch_pairs", C.c_int64),
+                ("reserved", C.c_int * 8)]
+
+
+class SStats(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("pairs", C.c_int64), ("rejected", C.c_int64),
+                ("p50_us", C.c_double), ("p99_us", C.c_double), ("mean_us", C.c_double),
+                ("max_us", C.c_double), ("qps", C.c_double), ("p90_us", C.c_double), ("p999_us", C.c_double),
+                ("reserved", C.c_double * 6)]
+
+
+def server_lib(chl):
     lib = chl.lib()
     vp = C.c_void_p
-
-    class SOpts(C.Structure):
-        _fields_ = [("streams", C.c_int), ("max_inflight_pairs", C.c_int64), ("max_batch_pairs", C.c_int64),
-                    ("reserved", C.c_int * 8)]
-
-    class SStats(C.Structure):
-        _fields_ = [("completed", C.c_int64), ("pairs", C.c_int64), ("rejected", C.c_int64),
-                    ("p50_us", C.c_double), ("p99_us", C.c_double), ("mean_us", C.c_double),
-                    ("max_us", C.c_double), ("qps", C.c_double), ("reserved", C.c_double * 8)]
-
     lib.chl_server_opts_default.argtypes = [C.POINTER(SOpts)]
     lib.chl_server_create.argtypes = [vp, C.POINTER(SOpts), C.POINTER(vp)]
-    lib.chl_server_submit.argtypes = [vp, vp, C.c_int64, vp, C.POINTER(C.c_uint64)]
-    lib.chl_server_poll.argtypes = [vp, C.c_uint64, C.c_int]
-    lib.chl_server_drain.argtypes = [vp]
+    lib.chl_server_replay.argtypes = [vp, vp, vp, vp, C.c_int64, vp]
     lib.chl_server_stats_get.argtypes = [vp, C.POINTER(SStats)]
     lib.chl_server_stats_reset.argtypes = [vp]
     lib.chl_server_free.argtypes = [vp]
+    return lib
+
+
+def replay(lib, srv, pairs, sizes, arrival_us):
+    """Open-loop replay of a request trace from the library's native submitter;
+    returns the served rate and latency percentiles."""
+    out = np.empty(pairs.shape[0], np.int32)
+    out.fill(-1)  # fault the pages in before the timed replay
+    sizes = np.ascontiguousarray(sizes, np.int64)
+    arrival_us = np.ascontiguousarray(arrival_us, np.float64)
+    lib.chl_server_stats_reset(srv)
+    rc = lib.chl_server_replay(srv, pairs.ctypes.data, sizes.ctypes.data, arrival_us.ctypes.data, sizes.size,
+                               out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(lib.chl_last_error().decode())
+    st = SStats()
+    lib.chl_server_stats_get(srv, C.byref(st))
+    return {"requests": int(st.completed), "pairs": int(st.pairs), "served_queries_per_s": st.qps,
+            "p50_us": st.p50_us, "p90_us": st.p90_us, "p99_us": st.p99_us, "p999_us": st.p999_us,
+            "max_us": st.max_us}
+
+
+def query_bench(chl, L, g, seconds=0.2, seed=11):
+    """Serving over the resident labels: open-loop Poisson arrivals replayed
+    natively, (a) 4096-pair requests at several offered rates, (b) the mixed
+    workload (sizes log-uniform in [1, 65536], 20% of the pairs within 50 hops),
+    plus the merge-join kernel alone on 1M device-resident pairs."""
+    n = g.n
+    lib = server_lib(chl)
     o = SOpts()
     lib.chl_server_opts_default(C.byref(o))
     o.streams = 8
-    srv = vp()
+    o.max_inflight_pairs = 1 << 21
+    o.max_batch_pairs = 1 << 16
+    srv = C.c_void_p()
     if lib.chl_server_create(L._h, C.byref(o), C.byref(srv)) != 0:
         raise RuntimeError(lib.chl_last_error().decode())
     rng = np.random.default_rng(seed)
-    pairs = [rng.integers(0, n, (batch, 2), dtype=np.uint32) for _ in range(16)]
-    outs = [np.zeros(batch, np.int32) for _ in range(batches)]
-    t = C.c_uint64()
-    # warm-up
-    for k in range(16):
-        lib.chl_server_submit(srv, pairs[k].ctypes.data, batch, outs[k].ctypes.data, C.byref(t))
-    lib.chl_server_drain(srv)
-    lib.chl_server_stats_reset(srv)
-    tickets = []
-    t0 = time.perf_counter()
-    for k in range(batches):
-        while True:
-            rc = lib.chl_server_submit(srv, pairs[k % 16].ctypes.data, batch, outs[k].ctypes.data, C.byref(t))
-            if rc == 0:
-                break
-            if rc != -9:
-                raise RuntimeError(lib.chl_last_error().decode())
-        tickets.append(t.value)
-    lib.chl_server_drain(srv)
-    wall = time.perf_counter() - t0
-    st = SStats()
-    lib.chl_server_stats_get(srv, C.byref(st))
-    for tk in tickets:
-        lib.chl_server_poll(srv, tk, 1)
+    res = {"fixed_4096": []}
+    # warm-up: a third of a second of 4096-pair requests at 2e8 pairs/s, so the
+    # measured replays see a device and host already at their serving clocks
+    nreq = int(2e8 * 0.33 / 4096)
+    replay(lib, srv, rng.integers(0, n, (nreq * 4096, 2), dtype=np.uint32), np.full(nreq, 4096),
+           np.cumsum(rng.exponential(4096.0 / 2e8 * 1e6, nreq)))
+    for rate in (1.0e8, 2.0e8, 2.5e8, 3.0e8):
+        nreq = int(rate * seconds / 4096)
+        arrivals = np.cumsum(rng.exponential(4096.0 / rate * 1e6, nreq))
+        pairs = rng.integers(0, n, (nreq * 4096, 2), dtype=np.uint32)
+        r = replay(lib, srv, pairs, np.full(nreq, 4096), arrivals)
+        r["offered_queries_per_s"] = rate
+        res["fixed_4096"].append(r)
+        log(f"serving 4096-pair requests offered {rate:.2e}/s: served {r['served_queries_per_s']:.3e}/s "
+            f"p50 {r['p50_us']:.0f} us p99 {r['p99_us']:.0f} us")
+    ok = [r for r in res["fixed_4096"] if r["p99_us"] < 1000.0 and
+          r["served_queries_per_s"] >= 0.95 * r["offered_queries_per_s"]]
+    best = max(ok, key=lambda r: r["served_queries_per_s"]) if ok else None
+    res["queries_per_s"] = best["served_queries_per_s"] if best else 0.0
+    res["p50_us"] = best["p50_us"] if best else None
+    res["p99_us"] = best["p99_us"] if best else None
+    # mixed workload
+    rate = 1.5e8
+    mean_size = (65536 - 1) / np.log(65536.0)
+    nreq = int(rate * seconds / mean_size)
+    sizes = np.floor(np.exp(rng.uniform(0, np.log(65536.0), nreq))).astype(np.int64)
+    total = int(sizes.sum())
+    pairs = rng.integers(0, n, (total, 2), dtype=np.uint32)
+    local = chl.local_pairs(g, 200000, 50, seed=seed)
+    pick = rng.random(total) < 0.2
+    pairs[pick] = local[rng.integers(0, local.shape[0], int(pick.sum()))]
+    arrivals = np.cumsum(rng.exponential(mean_size / rate * 1e6, nreq))
+    r = replay(lib, srv, pairs, sizes, arrivals)
+    r.update({"offered_queries_per_s": rate, "sizes": "log-uniform [1, 65536]", "local_fraction": 0.2,
+              "local_hops": 50})
+    res["mix"] = r
+    log(f"serving mixed requests offered {rate:.2e}/s: served {r['served_queries_per_s']:.3e}/s "
+        f"p50 {r['p50_us']:.0f} us p99 {r['p99_us']:.0f} us")
     lib.chl_server_free(srv)
-    # a direct batched call for the kernel-only rate
+    # one host-pointer call of 1M pairs (PCIe copies included) and the kernel alone
     big = rng.integers(0, n, (1 << 20, 2), dtype=np.uint32)
     L.query(big[:4096])
     t1 = time.perf_counter()
     L.query(big)
-    direct = time.perf_counter() - t1
-    return {"batch_pairs": batch, "batches": batches, "queries_per_s": batches * batch / wall,
-            "p50_us": st.p50_us, "p99_us": st.p99_us, "max_us": st.max_us,
-            "direct_1M_pairs_queries_per_s": big.shape[0] / direct}
+    res["host_call_1M_pairs_queries_per_s"] = big.shape[0] / (time.perf_counter() - t1)
+    res.update(query_kernel_rate(chl, L, big))
+    return res
+
+
+def query_kernel_rate(chl, L, pairs, reps=5):
+    """The merge-join kernel alone on device-resident pairs (HIP events around
+    chl_query_device on one stream), with its algorithmic bytes: per pair the
+    pair, the result, two offset pairs, two rank lookups and both label rows."""
+    hip = C.CDLL("libamdhip64.so")
+    vp = C.c_void_p
+    q = pairs.shape[0]
+    dp, do = vp(), vp()
+    hip.hipMalloc(C.byref(dp), C.c_size_t(pairs.nbytes))
+    hip.hipMalloc(C.byref(do), C.c_size_t(4 * q))
+    hip.hipMemcpy(dp, pairs.ctypes.data_as(vp), C.c_size_t(pairs.nbytes), 1)  # host to device
+    stream = vp()
+    hip.hipStreamCreate(C.byref(stream))
+    e0, e1 = vp(), vp()
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
+    lib = chl.lib()
+    lib.chl_query_device(L._h, dp, q, do, stream)  # warm-up
+    hip.hipStreamSynchronize(stream)
+    hip.hipEventRecord(e0, stream)
+    for _ in range(reps):
+        lib.chl_query_device(L._h, dp, q, do, stream)
+    hip.hipEventRecord(e1, stream)
+    hip.hipEventSynchronize(e1)
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+    for h in (dp, do):
+        hip.hipFree(h)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipStreamDestroy(stream)
+    sec = ms.value / 1e3 / reps
+    len1, len0 = L.row_lengths(1), L.row_lengths(0)
+    rows = len1[pairs[:, 0]].astype(np.float64) + len0[pairs[:, 1]].astype(np.float64)
+    algo = float(q) * (8 + 4 + 32 + 8) + 8.0 * rows.sum()
+    gbs = algo / sec / 1e9
+    return {"kernel_1M_pairs_queries_per_s": q / sec, "kernel_ms": sec * 1e3, "kernel_algorithmic_GBps": gbs,
+            "kernel_hbm_frac": gbs / HBM_PEAK_GBS}
 
 
 def copy_calibration(nbytes=1 << 30):
@@ -220,7 +303,9 @@ def main():
                    "switch_rank": s["switch_rank"], "switch_trials": s["switch_trials"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "tree_kernel",
-                     "bytes_per_build": s["relax_bytes"], "kernel_ms_per_build": s["relax_ms"]},
+                     "bytes_per_build": s["relax_bytes"], "kernel_ms_per_build": s["relax_ms"],
+                     "launches_per_build": s["tree_launches"],
+                     "avg_launch_ms": s["relax_ms"] / max(1, s["tree_launches"])},
     }
     if not args.no_compare:
         other = dict(opts, batching=0 if learned else 1)
@@ -235,7 +320,7 @@ def main():
                             "learned_speedup": t_static / t_learned, "labels_identical": same}
         log(f"other batching: {t_other:.3f} s, labels identical: {same}")
     if not args.no_query:
-        line["query"] = query_bench(chl, L, g.n)
+        line["query"] = query_bench(chl, L, g)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(g, order, args.cpu_seconds)
     print(json.dumps(line), flush=True)

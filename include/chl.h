@@ -76,6 +76,11 @@ int chl_gen_knn(int64_t n, int k, uint64_t seed, chl_graph** g);
 int chl_gen_rmat(int scale, int64_t samples, double a, double b, double c, int32_t wmax, uint64_t seed,
                  chl_graph** g);
 
+/* Locality-skewed query pairs (no reference counterpart; the serving workload
+ * mixes them with uniform pairs): for each pair a uniform source u and a target
+ * drawn uniformly from the vertices within `hops` out-arc hops of u. */
+int chl_gen_local_pairs(const chl_graph* g, int64_t q, int hops, uint64_t seed, uint32_t* pairs);
+
 /* ---- vertex order (order[rank] = vertex, rank 0 most important) --------------
  * Replaces hl::Order::read (shared_memory/ordering.hpp:52-67). Writes up to cap
  * ids and the file's count through *n. */
@@ -139,7 +144,8 @@ typedef struct chl_build_stats {
   double autotune_ms;           /* wall time spent by the autotuner (its trial trees included) */
   int64_t switch_trials;        /* hybrid under learned batching: pruned trial batches run */
   double policy_ms;             /* learned batching: wall time inside the policy (features, MLP, hooks) */
-  double reserved[6];
+  int64_t tree_launches;        /* tree kernel launches (relax_ms is their summed device time) */
+  double reserved[5];
 } chl_build_stats;
 
 void chl_build_opts_default(chl_build_opts* o);
@@ -164,6 +170,8 @@ int64_t chl_labels_total(const chl_labels* l, int side);
 double chl_labels_avg(const chl_labels* l);
 /* Host copy of one side: off[n+1], hub[total], dist[total]. */
 int chl_labels_copy(const chl_labels* l, int side, uint64_t* off, uint32_t* hub, int32_t* dist);
+/* len[v] = entries in vertex v's row of one side (no entry is copied). */
+int chl_labels_row_lengths(const chl_labels* l, int side, uint32_t* len);
 void chl_labels_free(chl_labels* l);
 
 /* ---- queries ----------------------------------------------------------------
@@ -173,6 +181,17 @@ void chl_labels_free(chl_labels* l);
 int chl_query(const chl_labels* l, const uint32_t* pairs, int64_t q, int32_t* out);
 /* Same on device pointers, enqueued on `stream` (a hipStream_t; NULL = default). */
 int chl_query_device(const chl_labels* l, const uint32_t* d_pairs, int64_t q, int32_t* d_out, void* stream);
+
+/* Query sources and results of distributed_memory/query.hpp:18-51:
+ * chl_query_generate = generate_query: 2q values rand() % range from the C
+ * library generator in its default (unseeded, seed 1) state, without touching
+ * the caller's rand() state; chl_query_load = load_query: one "u v" pair per
+ * line (writes up to cap pairs, the file's count through *q; a malformed or
+ * over-long line is CHL_E_FORMAT); chl_query_write = write_result: one distance
+ * per line. */
+int chl_query_generate(int64_t q, uint32_t range, uint32_t* pairs);
+int chl_query_load(const char* path, uint32_t* pairs, int64_t cap, int64_t* q);
+int chl_query_write(const char* path, const int32_t* out, int64_t q);
 
 /* ---- tree-size predictor of learned batching (no reference counterpart; it
  * predicts the (label_count, tree_size) pair of distributed_memory/akiba.hpp:99-103).
@@ -198,8 +217,9 @@ int chl_mfma_probe(const uint16_t* a, const uint16_t* b, float* d);
  * latency/throughput modes of distributed_memory/query.hpp:62-91) ------------- */
 typedef struct chl_server_opts {
   int streams;             /* concurrent device streams (default 4) */
-  int64_t max_inflight_pairs; /* admission budget in pairs resident on the device */
-  int64_t max_batch_pairs;    /* largest single request */
+  int64_t max_inflight_pairs; /* admission budget: pairs in flight at once (queued requests beyond twice
+                                 this are refused with CHL_E_BUSY) */
+  int64_t max_batch_pairs;    /* largest single request, and the most pairs one launch takes */
   int reserved[8];
 } chl_server_opts;
 
@@ -207,7 +227,8 @@ typedef struct chl_server_stats {
   int64_t completed, pairs, rejected;
   double p50_us, p99_us, mean_us, max_us;
   double qps;              /* pairs per second since creation / last reset */
-  double reserved[8];
+  double p90_us, p999_us;  /* more latency percentiles */
+  double reserved[6];
 } chl_server_stats;
 
 void chl_server_opts_default(chl_server_opts* o);
@@ -220,6 +241,13 @@ int chl_server_poll(chl_server* s, uint64_t ticket, int wait);
 int chl_server_drain(chl_server* s);
 int chl_server_stats_get(chl_server* s, chl_server_stats* st);
 void chl_server_stats_reset(chl_server* s);
+/* Replays a request trace from a native thread, open loop: request i (sizes[i]
+ * pairs, taken in order from `pairs`) is submitted arrival_us[i] microseconds
+ * after the call starts, whatever is still in flight (a request refused by
+ * admission is retried, and its wait counts in its latency); then drains. The
+ * results land in `out` in the same order; latencies in the server statistics. */
+int chl_server_replay(chl_server* s, const uint32_t* pairs, const int64_t* sizes, const double* arrival_us,
+                      int64_t count, int32_t* out);
 void chl_server_free(chl_server* s);
 
 #ifdef __cplusplus

@@ -50,7 +50,7 @@ class BuildStats(C.Structure):
                 ("clean_ms", C.c_double), ("relax_bytes", C.c_double), ("mlp_ms", C.c_double),
                 ("pred_mae_learned", C.c_double), ("pred_mae_static", C.c_double), ("pred_roots", C.c_int64),
                 ("granularity_used", C.c_int64), ("autotune_ms", C.c_double), ("switch_trials", C.c_int64),
-                ("policy_ms", C.c_double), ("reserved", C.c_double * 6)]
+                ("policy_ms", C.c_double), ("tree_launches", C.c_int64), ("reserved", C.c_double * 5)]
 
     def as_dict(self):
         return {k: (getattr(self, k) if k != "reserved" else None) for k, _ in self._fields_ if k != "reserved"}
@@ -98,9 +98,14 @@ def lib():
             "chl_labels_total": ([vp, C.c_int], i64),
             "chl_labels_avg": ([vp], C.c_double),
             "chl_labels_copy": ([vp, C.c_int, vp, vp, vp], C.c_int),
+            "chl_labels_row_lengths": ([vp, C.c_int, vp], C.c_int),
             "chl_labels_free": ([vp], None),
             "chl_query": ([vp, vp, i64, vp], C.c_int),
             "chl_query_device": ([vp, vp, i64, vp, vp], C.c_int),
+            "chl_query_generate": ([i64, C.c_uint32, vp], C.c_int),
+            "chl_gen_local_pairs": ([vp, i64, C.c_int, u64, vp], C.c_int),
+            "chl_query_load": ([C.c_char_p, vp, i64, C.POINTER(i64)], C.c_int),
+            "chl_query_write": ([C.c_char_p, vp, i64], C.c_int),
             "chl_mlp_param_count": ([], C.c_int),
             "chl_mlp_forward": ([vp, vp, i64, vp], C.c_int),
             "chl_mlp_grad": ([vp, vp, vp, i64, vp, vp], C.c_int),
@@ -301,12 +306,47 @@ class Labels:
         _check(lib().chl_labels_from_csr(n, *[_ptr(a) for a in arrs], device, C.byref(h)))
         return cls(h)
 
+    def row_lengths(self, side):
+        """Entries in each vertex's row of one side."""
+        out = np.zeros(self.n, np.uint32)
+        _check(lib().chl_labels_row_lengths(self._h, side, _ptr(out)))
+        return out
+
     def query(self, pairs):
         """d(u -> v) for each row (u, v) of pairs; INFTY when no hub is shared."""
         pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
         out = np.zeros(pairs.shape[0], np.int32)
         _check(lib().chl_query(self._h, _ptr(pairs), pairs.shape[0], _ptr(out)))
         return out
+
+
+def query_generate(q, vertex_range):
+    """q pairs drawn as generate_query does (distributed_memory/query.hpp:44-51)."""
+    out = np.zeros((q, 2), np.uint32)
+    _check(lib().chl_query_generate(q, vertex_range, _ptr(out)))
+    return out
+
+
+def local_pairs(graph, q, hops, seed=0):
+    """q pairs (u, v) with v uniform among the vertices within `hops` out-arc hops of a uniform u."""
+    out = np.zeros((q, 2), np.uint32)
+    _check(lib().chl_gen_local_pairs(graph._h, q, hops, seed, _ptr(out)))
+    return out
+
+
+def query_load(path):
+    """Pairs of a "u v" per line file (load_query, distributed_memory/query.hpp:18-32)."""
+    q = C.c_int64()
+    _check(lib().chl_query_load(str(path).encode(), None, 0, C.byref(q)))
+    out = np.zeros((q.value, 2), np.uint32)
+    _check(lib().chl_query_load(str(path).encode(), _ptr(out), q.value, C.byref(q)))
+    return out
+
+
+def query_write(path, dist):
+    """One distance per line (write_result, distributed_memory/query.hpp:34-41)."""
+    dist = np.ascontiguousarray(dist, np.int32)
+    _check(lib().chl_query_write(str(path).encode(), _ptr(dist), dist.size))
 
 
 def build(graph, order, **opts):

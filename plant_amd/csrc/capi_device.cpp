@@ -231,6 +231,19 @@ int chl_labels_copy(const chl_labels* l, int side, uint64_t* off, uint32_t* hub,
   return CHL_OK;
 }
 
+int chl_labels_row_lengths(const chl_labels* l, int side, uint32_t* len) {
+  if (!l || side < 0 || side > 1 || !len) return set_error(CHL_E_INVAL, "bad arguments");
+  const int64_t n = l->store.n;
+  std::vector<uint64_t> roff(n + 1);
+  CHL_HIP(hipSetDevice(l->store.device));
+  CHL_HIP(hipMemcpy(roff.data(), l->store.d_off(side), sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost));
+  for (int64_t v = 0; v < n; ++v) {
+    const uint32_t r = l->row_of[v];
+    len[v] = (uint32_t)(roff[r + 1] - roff[r]);
+  }
+  return CHL_OK;
+}
+
 void chl_labels_free(chl_labels* l) { delete l; }
 
 int chl_query_device(const chl_labels* l, const uint32_t* d_pairs, int64_t q, int32_t* d_out, void* stream) {

@@ -1,8 +1,12 @@
 // C ABI: graph, generator, order and error entry points (host only).
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/chl.h"
 #include "host.hpp"
@@ -167,6 +171,103 @@ int chl_order_degree(const chl_graph* g, uint64_t seed, int reference_ties, uint
   std::vector<uint32_t> o;
   order_degree(g->g, seed, reference_ties != 0, o);
   memcpy(order, o.data(), sizeof(uint32_t) * o.size());
+  return CHL_OK;
+}
+
+int chl_gen_local_pairs(const chl_graph* g, int64_t q, int hops, uint64_t seed, uint32_t* pairs) {
+  if (!g || q < 0 || hops < 0 || (q > 0 && !pairs) || g->g.n == 0) return set_error(CHL_E_INVAL, "bad arguments");
+  const Graph& G = g->g;
+  const int64_t n = G.n;
+  const int threads = (int)std::max<int64_t>(1, std::min<int64_t>(host_threads(), (q + 255) / 256));
+  // one breadth-first ball per pair; every thread has its own visit stamps and
+  // takes the pairs k = t, t + threads, ...
+  auto work = [&](int t) {
+    std::vector<uint32_t> st(n, 0), b;
+    for (int64_t k = t; k < q; k += threads) {
+      uint64_t s = seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(k + 1));
+      const uint32_t u = (uint32_t)(splitmix64(s) % (uint64_t)n);
+      const uint32_t mark = (uint32_t)(k / threads + 1);
+      b.clear();
+      b.push_back(u);
+      st[u] = mark;
+      size_t lo = 0;
+      for (int h = 0; h < hops && lo < b.size(); ++h) {
+        const size_t hi = b.size();
+        for (size_t i = lo; i < hi; ++i)
+          for (uint64_t a = G.off[1][b[i]]; a < G.off[1][b[i] + 1]; ++a) {
+            const uint32_t w = G.head[1][a];
+            if (st[w] != mark) {
+              st[w] = mark;
+              b.push_back(w);
+            }
+          }
+        lo = hi;
+      }
+      pairs[2 * k] = u;
+      pairs[2 * k + 1] = b[splitmix64(s) % b.size()];
+    }
+  };
+  std::vector<std::thread> ts;
+  for (int t = 1; t < threads; ++t) ts.emplace_back(work, t);
+  work(0);
+  for (auto& t : ts) t.join();
+  return CHL_OK;
+}
+
+// ---- query sources and results (distributed_memory/query.hpp:18-51) ---------
+
+int chl_query_generate(int64_t q, uint32_t range, uint32_t* pairs) {
+  if (q < 0 || (q > 0 && (!pairs || range == 0))) return set_error(CHL_E_INVAL, "bad arguments");
+  // the C library's additive generator in its own state, seeded 1: the stream
+  // that an unseeded rand() gives
+  char statebuf[128];
+  struct random_data rd;
+  memset(&rd, 0, sizeof(rd));
+  if (initstate_r(1u, statebuf, sizeof(statebuf), &rd) != 0) return set_error(CHL_E_INVAL, "generator state");
+  for (int64_t i = 0; i < 2 * q; ++i) {
+    int32_t r = 0;
+    random_r(&rd, &r);
+    pairs[i] = (uint32_t)r % range;
+  }
+  return CHL_OK;
+}
+
+int chl_query_load(const char* path, uint32_t* pairs, int64_t cap, int64_t* q) {
+  if (!path || !q) return set_error(CHL_E_INVAL, "bad arguments");
+  FILE* f = fopen(path, "r");
+  if (!f) return set_error(CHL_E_IO, "cannot open %s", path);
+  char buf[512];
+  int64_t k = 0;
+  int rc = CHL_OK;
+  while (fgets(buf, sizeof(buf), f)) {
+    const size_t len = strlen(buf);
+    long long u = 0, v = 0;
+    if ((len == 0 || buf[len - 1] != '\n') && !feof(f)) {
+      rc = set_error(CHL_E_FORMAT, "%s: line %lld too long", path, (long long)k + 1);
+      break;
+    }
+    if (sscanf(buf, "%lld %lld", &u, &v) != 2 || u < 0 || v < 0 || u > 0xFFFFFFFFll || v > 0xFFFFFFFFll) {
+      rc = set_error(CHL_E_FORMAT, "%s: line %lld is not a pair of vertex ids", path, (long long)k + 1);
+      break;
+    }
+    if (pairs && k < cap) {
+      pairs[2 * k] = (uint32_t)u;
+      pairs[2 * k + 1] = (uint32_t)v;
+    }
+    ++k;
+  }
+  fclose(f);
+  if (rc == CHL_OK) *q = k;
+  return rc;
+}
+
+int chl_query_write(const char* path, const int32_t* out, int64_t q) {
+  if (!path || q < 0 || (q > 0 && !out)) return set_error(CHL_E_INVAL, "bad arguments");
+  FILE* f = fopen(path, "w");
+  if (!f) return set_error(CHL_E_IO, "cannot open %s", path);
+  for (int64_t i = 0; i < q; ++i) fprintf(f, "%d\n", out[i]);
+  const bool ok = !ferror(f);
+  if (fclose(f) != 0 || !ok) return set_error(CHL_E_IO, "cannot write %s", path);
   return CHL_OK;
 }
 

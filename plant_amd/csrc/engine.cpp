@@ -335,7 +335,10 @@ struct Builder {
       float t = 0;
       CHL_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
       ms += t;
-      if (stats) stats->relax_ms += t;
+      if (stats) {
+        stats->relax_ms += t;
+        stats->tree_launches++;
+      }
       if (o.verbose)
         fprintf(stderr, "  side %d pass %d: %u roots, %.3f ms, written %llu, spilled %llu (%llu labels, %llu not started), processed %llu arcs %llu rounds %llu\n", side, pass, p.R, t, hc[CTR_LABELS], hc[CTR_OVERFLOW], hc[CTR_SPILL], hc[CTR_SKIPPED], hc[CTR_PROCESSED], hc[CTR_ARCS], hc[CTR_ROUNDS]);
       if (!hc[CTR_OVERFLOW]) break;

@@ -109,17 +109,31 @@ __global__ void clean_kernel(const uint64_t* key, const uint64_t* ent, uint64_t 
 }
 
 // Query: one wavefront per pair. The side-1 row of u is staged into LDS (in
-// tiles), each lane takes entries of the side-0 row of v and binary-searches
-// their hub among the staged hubs; the answer is the minimum sum over common
-// hubs, INT_MAX when there is none (labeling.hpp:56-65).
+// tiles of kQT entries), each lane takes entries of the side-0 row of v and
+// binary-searches their hub among the staged hubs; the answer is the minimum
+// sum over common hubs, INT_MAX when there is none (labeling.hpp:56-65).
+// Every lane issues all its loads of a tile (kQPer entries) and of v's chunk
+// before it uses any of them, so a pair costs a few memory latencies, not one
+// per 64 entries, and its kQPer searches advance in lock step (their LDS reads
+// overlap).
 constexpr int kQWaves = 4;
-constexpr int kQTile = 512;
+constexpr int kQT = 1024;
+constexpr int kQPer = kQT / 64;
+
+__device__ __forceinline__ void load_chunk(const uint64_t* ent, uint64_t base, uint64_t end, uint32_t lane,
+                                           uint64_t (&e)[kQPer]) {
+#pragma unroll
+  for (int i = 0; i < kQPer; ++i) {
+    const uint64_t j = base + lane + 64u * i;
+    e[i] = j < end ? ent[j] : ~0ull;  // hub all ones: past every staged hub
+  }
+}
 
 __global__ __launch_bounds__(256) void query_kernel(const uint64_t* off1, const uint64_t* ent1, const uint64_t* off0,
                                                     const uint64_t* ent0, const uint32_t* pairs,
                                                     const uint32_t* rank_of, uint32_t n, uint64_t q, int32_t* out) {
-  __shared__ uint32_t s_h[kQWaves][kQTile];
-  __shared__ uint32_t s_d[kQWaves][kQTile];
+  __shared__ uint32_t s_h[kQWaves][kQT];
+  __shared__ uint32_t s_d[kQWaves][kQT];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t k = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; k < q; k += nw) {
@@ -131,41 +145,60 @@ __global__ __launch_bounds__(256) void query_kernel(const uint64_t* off1, const 
     const uint32_t u = rank_of[pu], v = rank_of[pv];
     const uint64_t ab = off1[u], ae = off1[u + 1], bb = off0[v], be = off0[v + 1];
     uint32_t best = kInfDist;
-    for (uint64_t t0 = ab; t0 < ae; t0 += kQTile) {
-      const uint32_t tl = (uint32_t)((ae - t0) < kQTile ? (ae - t0) : kQTile);
-      for (uint32_t j = lane; j < tl; j += 64) {
-        const uint64_t e = ent1[t0 + j];
-        s_h[wave][j] = (uint32_t)(e >> 32);
-        s_d[wave][j] = (uint32_t)e;
+    for (uint64_t t0 = ab; t0 < ae; t0 += kQT) {
+      const uint32_t tl = (uint32_t)((ae - t0) < kQT ? (ae - t0) : kQT);
+      uint64_t eu[kQPer], ev[kQPer];
+      load_chunk(ent1, t0, ae, lane, eu);
+      load_chunk(ent0, bb, be, lane, ev);
+#pragma unroll
+      for (int i = 0; i < kQPer; ++i) {
+        const uint32_t j = lane + 64u * i;
+        if (j < tl) {
+          s_h[wave][j] = (uint32_t)(eu[i] >> 32);
+          s_d[wave][j] = (uint32_t)eu[i];
+        }
       }
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       const uint32_t lo_h = s_h[wave][0], hi_h = s_h[wave][tl - 1];
-      for (uint64_t j0 = bb; j0 < be; j0 += 64) {
-        const uint64_t j = j0 + lane;
-        uint32_t h = 0xFFFFFFFFu, dv = 0;
-        if (j < be) {
-          const uint64_t e = ent0[j];
-          h = (uint32_t)(e >> 32);
-          dv = (uint32_t)e;
+      for (uint64_t c0 = bb;;) {
+        // kQPer branch-free lower-bound searches in lock step: every step is one
+        // LDS read per search, all independent, so their latencies overlap
+        uint32_t base[kQPer];
+        bool past = true;
+#pragma unroll
+        for (int i = 0; i < kQPer; ++i) {
+          base[i] = 0;
+          past = past && (uint32_t)(ev[i] >> 32) > hi_h;
         }
-        if (h >= lo_h && h <= hi_h) {
-          uint32_t lo = 0, hi = tl;  // first index with s_h >= h
-          while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (s_h[wave][mid] < h) lo = mid + 1;
-            else hi = mid;
+        for (uint32_t len = tl; len > 1;) {
+          const uint32_t half = len >> 1;
+#pragma unroll
+          for (int i = 0; i < kQPer; ++i) {
+            const uint32_t h = (uint32_t)(ev[i] >> 32);
+            base[i] = s_h[wave][base[i] + half] < h ? base[i] + half : base[i];
           }
-          if (lo < tl && s_h[wave][lo] == h) {
-            const uint32_t s = s_d[wave][lo] + dv;
-            best = s < best ? s : best;
+          len -= half;
+        }
+#pragma unroll
+        for (int i = 0; i < kQPer; ++i) {
+          const uint32_t h = (uint32_t)(ev[i] >> 32);
+          // base = last index with hub < h (or 0); the match, if any, is there or next
+          const uint32_t at = s_h[wave][base[i]] < h ? base[i] + 1 : base[i];
+          if (h >= lo_h && h <= hi_h && at < tl && s_h[wave][at] == h) {
+            const uint32_t sum = s_d[wave][at] + (uint32_t)ev[i];
+            best = sum < best ? sum : best;
           }
         }
         // the side-0 row is hub-ascending: stop once every lane is past this tile
-        if (__all(h > hi_h)) break;
+        c0 += kQT;
+        if (__all(past) || c0 >= be) break;
+        load_chunk(ent0, c0, be, lane, ev);
       }
       __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     // wave minimum
     for (int o = 32; o > 0; o >>= 1) {

@@ -4,6 +4,7 @@
 //   hhl -g <graph> -o <order> [-p <host threads>] [-s <sf>] [-paraPLL] [-l <labels out>]
 //       [-mode plant|gll|hybrid] [-b <roots per batch>] [-delta <d>] [-ps <ratio>]
 //       [-learned] [-prune] [-device <id>]
+//       [-rq <query file> | -nq <count>] [-qo <result file>]
 //
 // The reference flags keep their meaning where it carries over: -g and -o name
 // the inputs, -l the label file (labeling.hpp:167-183 text). -p sets host
@@ -11,6 +12,9 @@
 // -s and -paraPLL are accepted for compatibility (the GPU batches by rank range,
 // not by sf*N labels). stdout repeats the argv line and then the two result lines
 // of shared_memory/akiba.hpp:214-215; stderr carries "Order size: N |V|: n".
+// Queries follow the distributed driver (distributed_memory/hhl.cpp:47-48,82-91):
+// -rq reads "u v" lines (load_query), -nq generates pairs as generate_query
+// does; the distances go one per line to -qo (write_result) or to stdout.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -25,6 +29,9 @@ int main(int argc, char** argv) {
   const char* gpath = nullptr;
   const char* opath = nullptr;
   const char* lpath = nullptr;
+  const char* qpath = nullptr;
+  const char* qopath = nullptr;
+  long long nq = 0;
   chl_build_opts o;
   chl_build_opts_default(&o);
   int threads = 16;
@@ -45,6 +52,9 @@ int main(int argc, char** argv) {
     else if (arg("-device")) o.device = atoi(argv[++i]);
     else if (strcmp(argv[i], "-learned") == 0) o.batching = CHL_BATCH_LEARNED;
     else if (strcmp(argv[i], "-prune") == 0) o.plant_prune = 1;
+    else if (arg("-rq")) qpath = argv[++i];
+    else if (arg("-nq")) nq = atoll(argv[++i]);
+    else if (arg("-qo")) qopath = argv[++i];
   }
   if (threads > 0) {
     char buf[32];
@@ -83,6 +93,37 @@ int main(int argc, char** argv) {
   if (lpath && chl_labels_write(L, lpath) != CHL_OK) {
     fprintf(stderr, "hhl: cannot write labels: %s\n", chl_last_error());
     rc = 1;
+  }
+  if (rc == CHL_OK && (qpath || nq > 0)) {
+    std::vector<uint32_t> pairs;
+    int64_t q = 0;
+    if (qpath) {
+      if (chl_query_load(qpath, nullptr, 0, &q) != CHL_OK) {
+        fprintf(stderr, "hhl: cannot read queries: %s\n", chl_last_error());
+        rc = 1;
+      } else {
+        pairs.resize(2 * (size_t)q);
+        chl_query_load(qpath, pairs.data(), q, &q);
+      }
+    } else {
+      q = nq;
+      pairs.resize(2 * (size_t)q);
+      chl_query_generate(q, (uint32_t)N, pairs.data());
+    }
+    if (rc == CHL_OK) {
+      std::vector<int32_t> dist((size_t)q);
+      if (chl_query(L, pairs.data(), q, dist.data()) != CHL_OK) {
+        fprintf(stderr, "hhl: query failed: %s\n", chl_last_error());
+        rc = 1;
+      } else if (qopath) {
+        if (chl_query_write(qopath, dist.data(), q) != CHL_OK) {
+          fprintf(stderr, "hhl: cannot write results: %s\n", chl_last_error());
+          rc = 1;
+        }
+      } else {
+        for (int64_t k = 0; k < q; ++k) printf("%d\n", dist[k]);
+      }
+    }
   }
   chl_labels_free(L);
   chl_graph_free(g);

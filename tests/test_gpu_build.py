@@ -188,3 +188,22 @@ def test_label_file_roundtrip(chl, grid, tmp_path):
     assert_same_labels(L2, ref)
     pairs = np.random.default_rng(1).integers(0, g.n, (3000, 2)).astype(np.uint32)
     assert np.array_equal(L2.query(pairs), ref.query(pairs))
+
+
+def test_query_long_rows(chl, orc):
+    # a path ranked from one end: vertex v's row holds every hub h <= v, so rows
+    # reach 2500 entries and the query merges over several staged tiles
+    n = 2500
+    rng = np.random.default_rng(21)
+    w = rng.integers(1, 10, n - 1).astype(np.int32)
+    src = np.arange(n - 1, dtype=np.uint32)
+    g = chl.Graph.from_arcs(n, src, src + 1, w, undirected=True)
+    order = np.arange(n, dtype=np.uint32)
+    L = chl.build(g, order, batch_roots=512)
+    off, hub, dist = L.copy(0)
+    assert int(np.diff(off).max()) == n
+    pos = np.concatenate([[0], np.cumsum(w, dtype=np.int64)])
+    pairs = rng.integers(0, n, (20000, 2)).astype(np.uint32)
+    pairs[:50] = [[n - 1, n - 1 - k] for k in range(50)]  # both rows long
+    want = np.abs(pos[pairs[:, 0]] - pos[pairs[:, 1]]).astype(np.int32)
+    assert np.array_equal(L.query(pairs), want)

@@ -151,3 +151,44 @@ def test_build_rejects_bad_order(chl):
     assert e.value.code == -5
     with pytest.raises(chl.ChlError):
         chl.build(g, np.arange(15, dtype=np.uint32))
+
+
+def test_query_generate_is_the_unseeded_c_rand_stream(chl):
+    # generate_query (distributed_memory/query.hpp:44-51) draws rand() % range
+    # from an unseeded generator: the C library's stream for seed 1
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(1)
+    want = np.array([libc.rand() % 977 for _ in range(2 * 500)], np.uint32).reshape(-1, 2)
+    got = chl.query_generate(500, 977)
+    assert np.array_equal(got, want)
+    # the library keeps its own state: a second call gives the same stream
+    assert np.array_equal(chl.query_generate(500, 977), want)
+
+
+def test_query_files_roundtrip_and_reject_malformed(chl, tmp_path):
+    p = tmp_path / "q.txt"
+    p.write_text("0 5\n17 3\n4294967295 0\n")
+    pairs = chl.query_load(p)
+    assert pairs.tolist() == [[0, 5], [17, 3], [4294967295, 0]]
+    bad = tmp_path / "bad.txt"
+    bad.write_text("0 5\n17\n")
+    with pytest.raises(chl.ChlError):
+        chl.query_load(bad)
+    out = tmp_path / "r.txt"
+    chl.query_write(out, np.array([3, chl.INFTY, 0], np.int32))
+    assert out.read_text() == "3\n2147483647\n0\n"
+
+
+def test_local_pairs_stay_within_the_hop_ball(chl):
+    # a path: the targets of a 3-hop ball are within 3 positions of the source,
+    # and every offset in [-3, 3] occurs
+    n = 200
+    src = np.arange(n - 1, dtype=np.uint32)
+    g = chl.Graph.from_arcs(n, src, src + 1, np.ones(n - 1, np.int32), undirected=True)
+    p = chl.local_pairs(g, 5000, 3, seed=4).astype(np.int64)
+    d = p[:, 1] - p[:, 0]
+    assert np.abs(d).max() <= 3
+    assert set(np.unique(d).tolist()) == set(range(-3, 4))
+    assert np.array_equal(p, chl.local_pairs(g, 5000, 3, seed=4).astype(np.int64))  # seeded
+    assert np.array_equal(chl.local_pairs(g, 10, 0, seed=1)[:, 0], chl.local_pairs(g, 10, 0, seed=1)[:, 1])
