@@ -82,11 +82,27 @@ def replay(lib, srv, pairs, sizes, arrival_us):
             "max_us": st.max_us}
 
 
-def query_bench(chl, L, g, seconds=0.2, seed=11):
+def query_bench(chl, L, g, seconds=0.2, seed=11, serve=True):
     """Serving over the resident labels: open-loop Poisson arrivals replayed
     natively, (a) 4096-pair requests at several offered rates, (b) the mixed
     workload (sizes log-uniform in [1, 65536], 20% of the pairs within 50 hops),
     plus the merge-join kernel alone on 1M device-resident pairs."""
+    n = g.n
+    rng = np.random.default_rng(seed)
+    res = {}
+    if serve:
+        res.update(serving(chl, L, g, rng, seconds))
+    # one host-pointer call of 1M pairs (PCIe copies included) and the kernel alone
+    big = rng.integers(0, n, (1 << 20, 2), dtype=np.uint32)
+    L.query(big[:4096])
+    t1 = time.perf_counter()
+    L.query(big)
+    res["host_call_1M_pairs_queries_per_s"] = big.shape[0] / (time.perf_counter() - t1)
+    res.update(query_kernel_rate(chl, L, big))
+    return res
+
+
+def serving(chl, L, g, rng, seconds):
     n = g.n
     lib = server_lib(chl)
     o = SOpts()
@@ -97,7 +113,6 @@ def query_bench(chl, L, g, seconds=0.2, seed=11):
     srv = C.c_void_p()
     if lib.chl_server_create(L._h, C.byref(o), C.byref(srv)) != 0:
         raise RuntimeError(lib.chl_last_error().decode())
-    rng = np.random.default_rng(seed)
     res = {"fixed_4096": []}
     # warm-up: a third of a second of 4096-pair requests at 2e8 pairs/s, so the
     # measured replays see a device and host already at their serving clocks
@@ -137,13 +152,6 @@ def query_bench(chl, L, g, seconds=0.2, seed=11):
     log(f"serving mixed requests offered {rate:.2e}/s: served {r['served_queries_per_s']:.3e}/s "
         f"p50 {r['p50_us']:.0f} us p99 {r['p99_us']:.0f} us")
     lib.chl_server_free(srv)
-    # one host-pointer call of 1M pairs (PCIe copies included) and the kernel alone
-    big = rng.integers(0, n, (1 << 20, 2), dtype=np.uint32)
-    L.query(big[:4096])
-    t1 = time.perf_counter()
-    L.query(big)
-    res["host_call_1M_pairs_queries_per_s"] = big.shape[0] / (time.perf_counter() - t1)
-    res.update(query_kernel_rate(chl, L, big))
     return res
 
 
@@ -238,6 +246,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-query", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="query kernel only, no server replays")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -306,6 +315,12 @@ def main():
                      "bytes_per_build": s["relax_bytes"], "kernel_ms_per_build": s["relax_ms"],
                      "launches_per_build": s["tree_launches"],
                      "avg_launch_ms": s["relax_ms"] / max(1, s["tree_launches"])},
+        "clean_roofline": ({"bound": "hbm", "kernel": "clean_kernel", "bytes_per_build": s["clean_bytes"],
+                            "kernel_ms_per_build": s["clean_kernel_ms"],
+                            "achieved": s["clean_bytes"] / (s["clean_kernel_ms"] / 1e3) / 1e9,
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": s["clean_bytes"] / (s["clean_kernel_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS}
+                           if s["clean_kernel_ms"] > 0 else None),
     }
     if not args.no_compare:
         other = dict(opts, batching=0 if learned else 1)
@@ -320,7 +335,7 @@ def main():
                             "learned_speedup": t_static / t_learned, "labels_identical": same}
         log(f"other batching: {t_other:.3f} s, labels identical: {same}")
     if not args.no_query:
-        line["query"] = query_bench(chl, L, g)
+        line["query"] = query_bench(chl, L, g, serve=not args.no_serve)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(g, order, args.cpu_seconds)
     print(json.dumps(line), flush=True)

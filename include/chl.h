@@ -145,7 +145,9 @@ typedef struct chl_build_stats {
   int64_t switch_trials;        /* hybrid under learned batching: pruned trial batches run */
   double policy_ms;             /* learned batching: wall time inside the policy (features, MLP, hooks) */
   int64_t tree_launches;        /* tree kernel launches (relax_ms is their summed device time) */
-  double reserved[5];
+  double clean_bytes;           /* algorithmic bytes of the cleaning merge-join */
+  double clean_kernel_ms;       /* its device time */
+  double reserved[3];
 } chl_build_stats;
 
 void chl_build_opts_default(chl_build_opts* o);

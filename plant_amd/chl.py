@@ -50,7 +50,8 @@ class BuildStats(C.Structure):
                 ("clean_ms", C.c_double), ("relax_bytes", C.c_double), ("mlp_ms", C.c_double),
                 ("pred_mae_learned", C.c_double), ("pred_mae_static", C.c_double), ("pred_roots", C.c_int64),
                 ("granularity_used", C.c_int64), ("autotune_ms", C.c_double), ("switch_trials", C.c_int64),
-                ("policy_ms", C.c_double), ("tree_launches", C.c_int64), ("reserved", C.c_double * 5)]
+                ("policy_ms", C.c_double), ("tree_launches", C.c_int64),
+                ("clean_bytes", C.c_double), ("clean_kernel_ms", C.c_double), ("reserved", C.c_double * 3)]
 
     def as_dict(self):
         return {k: (getattr(self, k) if k != "reserved" else None) for k, _ in self._fields_ if k != "reserved"}

@@ -235,7 +235,9 @@ struct Builder {
     size_t freeb = 0, totalb = 0;
     CHL_HIP(hipMemGetInfo(&freeb, &totalb));
     const size_t per = (size_t)n * 28;  // 16-byte cell + three u32 lists per vertex
-    size_t budget = freeb / 3;
+    // a third of the free memory, at most a quarter of the device: on large
+    // graphs the label store (twice, while a batch is merged) needs the rest
+    size_t budget = std::min(freeb / 3, totalb / 4);
     uint32_t fit = (uint32_t)std::max<size_t>(1, budget / std::max<size_t>(per, 1));
     slots = std::max<uint32_t>(1, std::min(want, fit));
     const size_t cells = (size_t)slots * n;
@@ -510,18 +512,27 @@ struct Builder {
     SideBatch& b = sb[side];
     SideBatch& ob = sb[sym ? side : 1 - side];
     const uint64_t m = b.m;
-    CHL_HIP(hipEventRecord(ev[4], stream));
     if (m) {
       CHL_HIP(b.keep.reserve(m));
       CHL_HIP(b.ckey.reserve(m));
       CHL_HIP(b.cent.reserve(m));
-      CHL_HIP(launch_clean(b.skey.p, b.sent.p, m, b.seg.p, ob.sent.p, ob.seg.p, b.keep.p, ctr.p, stream));
+      CHL_HIP(launch_fill_u64((uint64_t*)ctr.p + CTR_PRUNE_ENT, 0, 1, stream));
     }
+    CHL_HIP(hipEventRecord(ev[4], stream));
+    if (m) CHL_HIP(launch_clean(b.skey.p, b.sent.p, m, b.seg.p, ob.sent.p, ob.seg.p, b.keep.p, ctr.p, stream));
     CHL_HIP(hipEventRecord(ev[5], stream));
     CHL_HIP(hipEventSynchronize(ev[5]));
     float t = 0;
     CHL_HIP(hipEventElapsedTime(&t, ev[4], ev[5]));
     ms += t;
+    if (m && stats) {
+      // merge steps (two entries each), plus the candidate's key, entry and flag
+      unsigned long long steps = 0;
+      CHL_HIP(hipMemcpy(&steps, ctr.p + CTR_PRUNE_ENT, sizeof(steps), hipMemcpyDeviceToHost));
+      stats->clean_entries += (int64_t)(2 * steps);
+      stats->clean_bytes += 16.0 * (double)steps + 17.0 * (double)m;
+      stats->clean_kernel_ms += t;
+    }
     return CHL_OK;
   }
 
