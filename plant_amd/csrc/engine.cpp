@@ -51,14 +51,19 @@ int ensure_device(int device) {
   return CHL_OK;
 }
 
-// Locality order of the vertices: breadth-first over the arcs of both ranges,
-// starting from a vertex of maximum degree (then from the next unvisited vertex
-// by id), so that vertices close in the graph get close device ids and a search
-// front touches few cache lines. inv[id] = vertex.
+// Locality order of the vertices, so that a search front touches few cache
+// lines of the per-vertex state. First a breadth-first order over the arcs of
+// both ranges from a vertex of maximum degree (then from the next unvisited
+// vertex by id); then clusters of up to kLocalityCluster vertices, each grown
+// breadth-first from the first unassigned vertex of that order, take
+// consecutive ids: a tree that enters a cluster finds its neighbours' states on
+// the same lines. inv[id] = vertex.
+constexpr size_t kLocalityCluster = 64;
+
 static void locality_order(const Graph& g, std::vector<uint32_t>& inv) {
   const int64_t n = g.n;
-  inv.clear();
-  inv.reserve(n);
+  std::vector<uint32_t> bfs;
+  bfs.reserve(n);
   std::vector<uint8_t> seen(n, 0);
   int64_t best = 0;
   uint64_t bdeg = 0;
@@ -73,19 +78,41 @@ static void locality_order(const Graph& g, std::vector<uint32_t>& inv) {
     const uint32_t s = (uint32_t)(k < 0 ? best : k);
     if (seen[s]) continue;
     seen[s] = 1;
-    size_t head = inv.size();
-    inv.push_back(s);
-    while (head < inv.size()) {
-      const uint32_t u = inv[head++];
+    size_t head = bfs.size();
+    bfs.push_back(s);
+    while (head < bfs.size()) {
+      const uint32_t u = bfs[head++];
       for (int rg = 1; rg >= 0; --rg)
         for (uint64_t a = g.off[rg][u]; a < g.off[rg][u + 1]; ++a) {
           const uint32_t v = g.head[rg][a];
           if (!seen[v]) {
             seen[v] = 1;
-            inv.push_back(v);
+            bfs.push_back(v);
           }
         }
     }
+  }
+  inv.clear();
+  inv.reserve(n);
+  std::fill(seen.begin(), seen.end(), 0);
+  std::vector<uint32_t> q;
+  for (const uint32_t s : bfs) {
+    if (seen[s]) continue;
+    q.clear();
+    q.push_back(s);
+    seen[s] = 1;
+    for (size_t head = 0; head < q.size() && q.size() < kLocalityCluster; ++head) {
+      const uint32_t u = q[head];
+      for (int rg = 1; rg >= 0 && q.size() < kLocalityCluster; --rg)
+        for (uint64_t a = g.off[rg][u]; a < g.off[rg][u + 1] && q.size() < kLocalityCluster; ++a) {
+          const uint32_t v = g.head[rg][a];
+          if (!seen[v]) {
+            seen[v] = 1;
+            q.push_back(v);
+          }
+        }
+    }
+    inv.insert(inv.end(), q.begin(), q.end());
   }
 }
 
