@@ -2,6 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <utility>
+#include <vector>
+
 #include <cstdint>
 
 #include "../../include/chl.h"
@@ -26,6 +29,49 @@ namespace chl {
 hipError_t dev_alloc(void** p, size_t bytes, size_t* got);
 void dev_free(void* p, size_t bytes);
 void dev_cache_flush();
+
+// Device array that grows in place: a virtual range as large as the device
+// memory is reserved once and physical memory is mapped behind it as the array
+// grows (HIP virtual memory management), so growing never copies and never
+// needs the old and the new array at the same time. Where the runtime refuses
+// virtual memory the array falls back to allocate-copy-free.
+struct GrowArray {
+  char* base = nullptr;
+  size_t reserved = 0;  // bytes of the virtual range (virtual memory only)
+  size_t mapped = 0;    // usable bytes
+  int device = 0;
+  bool vmm = false;
+  std::vector<std::pair<size_t, hipMemGenericAllocationHandle_t>> maps;  // (size, handle) in order
+  GrowArray() = default;
+  GrowArray(const GrowArray&) = delete;
+  GrowArray& operator=(const GrowArray&) = delete;
+  ~GrowArray() { release(); }
+  // At least `bytes` usable; the first keep_bytes are preserved.
+  hipError_t grow(size_t bytes, size_t keep_bytes, hipStream_t s);
+  void release();
+
+ private:
+  hipError_t vmm_map(size_t bytes);
+};
+
+template <class T>
+struct GrowBuf {
+  GrowArray a;
+  T* p = nullptr;
+  size_t n = 0;  // elements usable
+  hipError_t reserve(size_t count, size_t keep = 0, hipStream_t s = nullptr) {
+    if (count <= n && p) return hipSuccess;
+    hipError_t e = a.grow((count ? count : 1) * sizeof(T), keep * sizeof(T), s);
+    p = reinterpret_cast<T*>(a.base);
+    n = a.mapped / sizeof(T);
+    return e;
+  }
+  void release() {
+    a.release();
+    p = nullptr;
+    n = 0;
+  }
+};
 
 // Device buffer that returns its memory to the cache.
 template <class T>
@@ -149,10 +195,13 @@ hipError_t launch_max_row(const uint64_t* off, uint32_t n, unsigned long long* o
 // *flag when some root's tree_size / label_count exceeds ratio.
 hipError_t launch_switch_test(const uint32_t* tree_size, const uint32_t* label_count, uint32_t R, double ratio,
                               unsigned long long* flag, hipStream_t s);
-// Rows of the new store: old row v, then the batch segment of v (sorted).
-hipError_t launch_merge_rows(const uint64_t* old_off, const uint64_t* old_ent, const uint64_t* excl,
-                             const uint64_t* new_off, const uint64_t* bat_ent, uint32_t n, uint64_t* new_ent,
+// In-place merge of one vertex block [a, b): its old rows, copied to tmp from
+// old_off[a] on, go to new_off[v], each followed by the batch's rows; blocks are
+// merged from the last one down. And a plain device copy.
+hipError_t launch_place_rows(const uint64_t* old_off, const uint64_t* tmp, uint64_t tmp_base, const uint64_t* new_off,
+                             const uint64_t* excl, const uint64_t* bat, uint32_t a, uint32_t b, uint64_t* ent,
                              hipStream_t s);
+hipError_t launch_copy_u64(const uint64_t* src, uint64_t count, uint64_t* dst, hipStream_t s);
 // In-batch cleaning (shared_memory/labeling.hpp:78-88 over the batch labels, which
 // are sorted by (vertex, hub)): keep[k] = 0 when candidate k is covered.
 hipError_t launch_clean(const uint64_t* key, const uint64_t* ent, uint64_t m, const uint64_t* seg_off,

@@ -215,7 +215,10 @@ struct Builder {
   int64_t stats_batches = 0;
   DBuf<unsigned long long> ctr;
   DBuf<unsigned long long> misc;  // [0] switch flag, [1] max row
-  DBuf<uint64_t> noff, nent;
+  DBuf<uint64_t> noff;
+  DBuf<uint64_t> mtmp;          // staging of the in-place merge
+  std::vector<uint64_t> ho;     // host copy of the store offsets
+  static constexpr uint64_t kMergeTmp = (uint64_t)1 << 26;  // entries staged per block
   DBuf<uint8_t> tmp;
   SideBatch sb[2];
   uint64_t out_cap = 0;
@@ -590,25 +593,41 @@ struct Builder {
     return CHL_OK;
   }
 
+  // Appends the batch's sorted labels (segments b.seg, entries b.sent) to the
+  // store rows in place: the store grows without moving, then its rows are
+  // shifted up block by block from the last vertex down (each block's old rows
+  // staged in a temporary first), so the store is never held twice.
   int merge_side(int side, LabelStore& L, double& ms) {
     SideBatch& b = sb[side];
     const uint64_t m = b.m;
     b.kept = m;
     if (!m) return CHL_OK;
     CHL_HIP(hipEventRecord(ev[6], stream));
-    const uint64_t newt = L.total[side] + m;
+    const uint64_t oldt = L.total[side], newt = oldt + m;
+    CHL_HIP(L.ent[side].reserve(newt, oldt, stream));
     CHL_HIP(noff.reserve(n + 1));
-    // the merged store goes to the spare buffer, which keeps a quarter of slack
-    // so that most batches reuse it (a fresh multi-GB allocation per batch
-    // costs far more than the merge)
-    if (nent.n < newt) {
-      nent.release();
-      CHL_HIP(nent.reserve(newt + newt / 4));
-    }
     CHL_HIP(launch_merge_offsets(L.off[side].p, b.seg.p, (uint32_t)n, noff.p, stream));
-    CHL_HIP(launch_merge_rows(L.off[side].p, L.ent[side].p, b.seg.p, noff.p, b.sent.p, (uint32_t)n, nent.p, stream));
+    if (oldt == 0) {
+      CHL_HIP(launch_copy_u64(b.sent.p, m, L.ent[side].p, stream));
+    } else {
+      ho.resize(n + 1);
+      CHL_HIP(hipMemcpyAsync(ho.data(), L.off[side].p, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, stream));
+      CHL_HIP(hipStreamSynchronize(stream));
+      const uint64_t cap = std::min<uint64_t>(kMergeTmp, oldt);
+      CHL_HIP(mtmp.reserve(cap));
+      for (uint64_t hi = (uint64_t)n; hi > 0;) {
+        // the widest block [lo, hi) whose old rows fit the temporary
+        uint64_t lo = (uint64_t)(std::lower_bound(ho.begin(), ho.begin() + hi, ho[hi] - std::min(ho[hi], cap)) -
+                                 ho.begin());
+        if (lo == hi) lo = hi - 1;  // one row longer than the temporary: cannot happen (rows < n entries)
+        if (ho[hi] - ho[lo] > cap) return set_error(CHL_E_RANGE, "label row of %llu entries", (unsigned long long)(ho[hi] - ho[lo]));
+        CHL_HIP(launch_copy_u64(L.ent[side].p + ho[lo], ho[hi] - ho[lo], mtmp.p, stream));
+        CHL_HIP(launch_place_rows(L.off[side].p, mtmp.p, ho[lo], noff.p, b.seg.p, b.sent.p, (uint32_t)lo,
+                                  (uint32_t)hi, L.ent[side].p, stream));
+        hi = lo;
+      }
+    }
     L.off[side].swap(noff);
-    L.ent[side].swap(nent);
     L.total[side] = newt;
     CHL_HIP(hipEventRecord(ev[7], stream));
     CHL_HIP(hipEventSynchronize(ev[7]));

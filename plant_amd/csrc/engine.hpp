@@ -15,7 +15,7 @@ struct LabelStore {
   bool shared_sides = false;
   int device = 0;
   DBuf<uint64_t> off[2];
-  DBuf<uint64_t> ent[2];
+  GrowBuf<uint64_t> ent[2];  // grows in place as batches are merged
   uint64_t total[2] = {0, 0};
   const uint64_t* d_off(int s) const { return off[shared_sides ? 0 : s].p; }
   const uint64_t* d_ent(int s) const { return ent[shared_sides ? 0 : s].p; }

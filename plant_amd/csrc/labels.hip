@@ -56,19 +56,28 @@ __global__ void merge_offsets_kernel(const uint64_t* old_off, const uint64_t* ex
     new_off[v] = old_off[v] + excl[v];
 }
 
-// One wavefront per vertex: old row, then the vertex's batch segment.
-__global__ __launch_bounds__(256) void merge_rows_kernel(const uint64_t* old_off, const uint64_t* old_ent,
-                                                         const uint64_t* excl, const uint64_t* new_off,
-                                                         const uint64_t* bat, uint32_t n, uint64_t* new_ent) {
+// In-place merge of a batch into the store, one block of vertices [a, b) at a
+// time from the last block down: the block's old rows were first copied to tmp
+// (they start at old_off[a]); row v goes to new_off[v] >= old_off[v] followed by
+// the batch's entries of v. Later blocks are already in place and earlier rows
+// lie below new_off[a], so no unread entry is overwritten.
+__global__ __launch_bounds__(256) void place_rows_kernel(const uint64_t* old_off, const uint64_t* tmp, uint64_t tmp_base,
+                                                         const uint64_t* new_off, const uint64_t* excl,
+                                                         const uint64_t* bat, uint32_t a, uint32_t b, uint64_t* ent) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t w0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t v = w0; v < n; v += nw) {
-    const uint64_t ob = old_off[v], oe = old_off[v + 1], nb = new_off[v];
-    for (uint64_t j = lane; j < oe - ob; j += 64) new_ent[nb + j] = old_ent[ob + j];
-    const uint64_t bb = excl[v], be = excl[v + 1], b2 = nb + (oe - ob);
-    for (uint64_t j = lane; j < be - bb; j += 64) new_ent[b2 + j] = bat[bb + j];
+  for (uint64_t v = a + w0; v < b; v += nw) {
+    const uint64_t ob = old_off[v] - tmp_base, len = old_off[v + 1] - old_off[v], nb = new_off[v];
+    for (uint64_t j = lane; j < len; j += 64) ent[nb + j] = tmp[ob + j];
+    const uint64_t bb = excl[v], be = excl[v + 1];
+    for (uint64_t j = lane; j < be - bb; j += 64) ent[nb + len + j] = bat[bb + j];
   }
+}
+
+__global__ void copy_u64_kernel(const uint64_t* src, uint64_t count, uint64_t* dst) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 // clean_cover (shared_memory/labeling.hpp:78-88) of candidate k = (v, h, d):
@@ -243,11 +252,19 @@ hipError_t launch_merge_offsets(const uint64_t* old_off, const uint64_t* excl, u
   return hipGetLastError();
 }
 
-hipError_t launch_merge_rows(const uint64_t* old_off, const uint64_t* old_ent, const uint64_t* excl,
-                             const uint64_t* new_off, const uint64_t* bat_ent, uint32_t n, uint64_t* new_ent,
+
+hipError_t launch_place_rows(const uint64_t* old_off, const uint64_t* tmp, uint64_t tmp_base, const uint64_t* new_off,
+                             const uint64_t* excl, const uint64_t* bat, uint32_t a, uint32_t b, uint64_t* ent,
                              hipStream_t s) {
-  hipLaunchKernelGGL(merge_rows_kernel, dim3(grid_for((uint64_t)n * 64, 256, 65536)), dim3(256), 0, s, old_off,
-                     old_ent, excl, new_off, bat_ent, n, new_ent);
+  if (b <= a) return hipSuccess;
+  hipLaunchKernelGGL(place_rows_kernel, dim3(grid_for((uint64_t)(b - a) * 64, 256, 65536)), dim3(256), 0, s, old_off,
+                     tmp, tmp_base, new_off, excl, bat, a, b, ent);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_u64(const uint64_t* src, uint64_t count, uint64_t* dst, hipStream_t s) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(copy_u64_kernel, dim3(grid_for(count, 256, 16384)), dim3(256), 0, s, src, count, dst);
   return hipGetLastError();
 }
 
