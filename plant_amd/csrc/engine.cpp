@@ -222,6 +222,7 @@ struct Builder {
   DBuf<uint8_t> tmp;
   SideBatch sb[2];
   uint64_t out_cap = 0;
+  double tail_labels = -1;  // labels per root at the top ranks of the last batch
   hipEvent_t ev[8];
 
   Builder(const Graph& gg, DeviceGraph& d, const std::vector<uint32_t>& ord, const chl_build_opts& opts,
@@ -726,6 +727,18 @@ struct Builder {
         const uint32_t lw = hooks->next_width(hooks->ctx, r0, width, W);
         if (lw) bw = std::min<uint32_t>(lw, W);
       }
+      {
+        // memory cap under either policy: a batch's labels (output, sorted
+        // copy, sort scratch: about 96 bytes each) within the free memory, at
+        // 1.5 times the labels per root of the last batch's highest eighth of
+        // ranks (labels per root fall with rank); before any batch, a root
+        // may label every vertex
+        size_t freeb = 0, totalb = 0;
+        CHL_HIP(hipMemGetInfo(&freeb, &totalb));
+        const double per_root = tail_labels >= 0 ? std::max(1.0, 1.5 * tail_labels) : (double)n;
+        const double cap = (double)freeb / 96.0 / per_root / nsides;
+        bw = (uint32_t)std::max<double>(1.0, std::min<double>(bw, cap));
+      }
       const uint32_t R = (uint32_t)std::min<int64_t>(bw, n - r0);
       const uint32_t prefix = (uint32_t)r0;
       const double t_batch = lt + stt + ct;
@@ -828,6 +841,18 @@ struct Builder {
                 (unsigned long long)L.total[0]);
       }
       ++stats_batches;
+      {
+        // labels per root (candidates for pruned trees) of the batch's highest
+        // eighth of ranks, forward side
+        const uint32_t lo = (uint32_t)(r0 + R - std::max<uint32_t>(1, R / 8));
+        double sum = 0, cnt = 0;
+        for (uint32_t k = 0; k < R; ++k)
+          if (hroots[k] >= lo) {
+            sum += hlc[k];
+            cnt += 1;
+          }
+        if (cnt > 0) tail_labels = sum / cnt;
+      }
       // output estimate for the next batch from this one's labels per root
       {
         const uint32_t nextR = fixed ? static_cap : std::min<uint32_t>(static_cap, width * 2);

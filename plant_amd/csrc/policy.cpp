@@ -111,7 +111,7 @@ struct Learned {
     {
       size_t freeb = 0, totalb = 0;
       CHL_HIP(hipMemGetInfo(&freeb, &totalb));
-      label_budget = std::min<double>(2147483647.0, std::max<double>(1 << 22, (double)freeb / 8.0 / 32.0));
+      update_budget(freeb);
     }
     // features by rank: log rank, log degree, log 2-hop degree sum, log score
     std::vector<double> f1(n), f2(n), f3(n, 0.0);
@@ -239,9 +239,18 @@ struct Learned {
     return t * calib_t / std::max(1.0, l * calib_l);
   }
 
+  void update_budget(size_t freeb) {
+    label_budget = std::min<double>(2147483647.0, std::max<double>(1 << 22, (double)freeb / 8.0 / 32.0));
+  }
+
   uint32_t next_width(int64_t r0, uint32_t static_width, uint32_t max_width) {
     (void)static_width;
     decided_r0 = -1;
+    {
+      // the budget follows the free memory as the label store grows
+      size_t freeb = 0, totalb = 0;
+      if (hipMemGetInfo(&freeb, &totalb) == hipSuccess) update_budget(freeb);
+    }
     if (!trained || switched) return 0;
     // at most a quarter of the vertices and eight times the previous width per
     // batch, so that an error of the predicted label total (heavy-tailed sums
