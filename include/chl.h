@@ -172,6 +172,11 @@ int64_t chl_labels_total(const chl_labels* l, int side);
 double chl_labels_avg(const chl_labels* l);
 /* Host copy of one side: off[n+1], hub[total], dist[total]. */
 int chl_labels_copy(const chl_labels* l, int side, uint64_t* off, uint32_t* hub, int32_t* dist);
+/* Order-independent digest of one side, computed on the device: the sum (mod 2^64)
+ * over every vertex v and entry (hub h, dist d) of v's row of
+ * splitmix64_finalizer(((uint64_t)h << 32 | (uint32_t)d) ^ v * 0x9E3779B97F4A7C15).
+ * Equal label sets give equal digests whatever the batch schedule. */
+int chl_labels_checksum(const chl_labels* l, int side, uint64_t* sum);
 /* len[v] = entries in vertex v's row of one side (no entry is copied). */
 int chl_labels_row_lengths(const chl_labels* l, int side, uint32_t* len);
 void chl_labels_free(chl_labels* l);

@@ -100,6 +100,7 @@ def lib():
             "chl_labels_avg": ([vp], C.c_double),
             "chl_labels_copy": ([vp, C.c_int, vp, vp, vp], C.c_int),
             "chl_labels_row_lengths": ([vp, C.c_int, vp], C.c_int),
+            "chl_labels_checksum": ([vp, C.c_int, C.POINTER(u64)], C.c_int),
             "chl_labels_free": ([vp], None),
             "chl_query": ([vp, vp, i64, vp], C.c_int),
             "chl_query_device": ([vp, vp, i64, vp, vp], C.c_int),
@@ -306,6 +307,12 @@ class Labels:
         h = C.c_void_p()
         _check(lib().chl_labels_from_csr(n, *[_ptr(a) for a in arrs], device, C.byref(h)))
         return cls(h)
+
+    def checksum(self, side):
+        """The device-side digest of one side (chl_labels_checksum)."""
+        out = C.c_uint64()
+        _check(lib().chl_labels_checksum(self._h, side, C.byref(out)))
+        return out.value
 
     def row_lengths(self, side):
         """Entries in each vertex's row of one side."""

@@ -244,6 +244,21 @@ int chl_labels_row_lengths(const chl_labels* l, int side, uint32_t* len) {
   return CHL_OK;
 }
 
+int chl_labels_checksum(const chl_labels* l, int side, uint64_t* sum) {
+  if (!l || side < 0 || side > 1 || !sum) return set_error(CHL_E_INVAL, "bad arguments");
+  CHL_HIP(hipSetDevice(l->store.device));
+  DBuf<unsigned long long> acc;
+  CHL_HIP(acc.reserve(1));
+  CHL_HIP(hipMemset(acc.p, 0, sizeof(unsigned long long)));
+  if (l->store.n)
+    CHL_HIP(launch_checksum(l->store.d_off(side), l->store.d_ent(side), l->rank_of.p, (uint32_t)l->store.n, acc.p,
+                            nullptr));
+  unsigned long long h = 0;
+  CHL_HIP(hipMemcpy(&h, acc.p, sizeof(h), hipMemcpyDeviceToHost));
+  *sum = h;
+  return CHL_OK;
+}
+
 void chl_labels_free(chl_labels* l) { delete l; }
 
 int chl_query_device(const chl_labels* l, const uint32_t* d_pairs, int64_t q, int32_t* d_out, void* stream) {

@@ -202,6 +202,10 @@ hipError_t launch_place_rows(const uint64_t* old_off, const uint64_t* tmp, uint6
                              const uint64_t* excl, const uint64_t* bat, uint32_t a, uint32_t b, uint64_t* ent,
                              hipStream_t s);
 hipError_t launch_copy_u64(const uint64_t* src, uint64_t count, uint64_t* dst, hipStream_t s);
+// *out += sum over vertices v and entries e of row row_of[v] of mix64(e ^ v * golden)
+// (out must be zeroed): an order-independent digest of one label side.
+hipError_t launch_checksum(const uint64_t* off, const uint64_t* ent, const uint32_t* row_of, uint32_t n,
+                           unsigned long long* out, hipStream_t s);
 // In-batch cleaning (shared_memory/labeling.hpp:78-88 over the batch labels, which
 // are sorted by (vertex, hub)): keep[k] = 0 when candidate k is covered.
 hipError_t launch_clean(const uint64_t* key, const uint64_t* ent, uint64_t m, const uint64_t* seg_off,

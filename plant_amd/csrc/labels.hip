@@ -75,6 +75,28 @@ __global__ __launch_bounds__(256) void place_rows_kernel(const uint64_t* old_off
   }
 }
 
+// Order-independent digest of a label side: the sum over all entries (hub, d) of
+// vertex v's row of a 64-bit mix of (v, hub, d) (the splitmix64 finalizer).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__global__ __launch_bounds__(256) void checksum_kernel(const uint64_t* off, const uint64_t* ent, const uint32_t* row_of,
+                                                       uint32_t n, unsigned long long* out) {
+  unsigned long long acc = 0;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t w0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t v = w0; v < n; v += nw) {
+    const uint32_t r = row_of[v];
+    const uint64_t key = (uint64_t)v * 0x9E3779B97F4A7C15ull;
+    for (uint64_t k = off[r] + lane; k < off[r + 1]; k += 64) acc += mix64(ent[k] ^ key);
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0 && acc) atomicAdd(out, acc);
+}
+
 __global__ void copy_u64_kernel(const uint64_t* src, uint64_t count, uint64_t* dst) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
@@ -259,6 +281,13 @@ hipError_t launch_place_rows(const uint64_t* old_off, const uint64_t* tmp, uint6
   if (b <= a) return hipSuccess;
   hipLaunchKernelGGL(place_rows_kernel, dim3(grid_for((uint64_t)(b - a) * 64, 256, 65536)), dim3(256), 0, s, old_off,
                      tmp, tmp_base, new_off, excl, bat, a, b, ent);
+  return hipGetLastError();
+}
+
+hipError_t launch_checksum(const uint64_t* off, const uint64_t* ent, const uint32_t* row_of, uint32_t n,
+                           unsigned long long* out, hipStream_t s) {
+  hipLaunchKernelGGL(checksum_kernel, dim3(grid_for((uint64_t)n * 64, 256, 16384)), dim3(256), 0, s, off, ent, row_of,
+                     n, out);
   return hipGetLastError();
 }
 

@@ -1,0 +1,98 @@
+"""Parity at the configurations' own sizes, through properties that do not need the
+oracle's labels: the reference's label file for the 100x100 grid (byte-identical,
+every mode), the device digest of the label set under different batch schedules,
+and query distances against the oracle's plain Dijkstra on sampled pairs. The
+digest itself is checked against a host computation over the oracle's labels."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MANIFEST = json.load(open(os.path.join(GOLD, "manifest.json")))
+
+
+def host_digest(off, hub, dist):
+    """chl_labels_checksum restated in numpy (uint64 arithmetic wraps mod 2^64)."""
+    off = np.asarray(off, np.int64)
+    v = np.repeat(np.arange(off.size - 1, dtype=np.uint64), np.diff(off))
+    z = (np.asarray(hub, np.uint64) << np.uint64(32)) | np.asarray(dist, np.int32).astype(np.uint32).astype(np.uint64)
+    z ^= v * np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z ^= z >> np.uint64(31)
+    return int(z.sum(dtype=np.uint64))
+
+
+def oracle_dijkstra_check(orc, g, L, sources=4, targets=5000, seed=3):
+    src, dst, w = g.arc_list(1)
+    og = orc.OGraph.from_arcs(g.n, src, dst, w)
+    rng = np.random.default_rng(seed)
+    S = rng.integers(0, g.n, sources).astype(np.uint32)
+    D = og.dijkstra_multi(S)
+    T = rng.integers(0, g.n, (sources, targets)).astype(np.uint32)
+    pairs = np.stack([np.repeat(S, targets), T.reshape(-1)], 1).astype(np.uint32)
+    want = np.concatenate([D[i][T[i]] for i in range(sources)])
+    assert np.array_equal(L.query(pairs), want)
+    v = rng.integers(0, g.n, 1000).astype(np.uint32)
+    assert np.all(L.query(np.stack([v, v], 1)) == 0)
+
+
+def test_device_digest_matches_host(chl, orc):
+    g = chl.Graph.knn(4000, 3, seed=2)
+    order = g.order_betweenness(32, seed=1)
+    L = chl.build(g, order)
+    src, dst, w = g.arc_list(1)
+    ref, _ = orc.gll(orc.OGraph.from_arcs(g.n, src, dst, w), order, threads=8)
+    for side in (0, 1):
+        assert L.checksum(side) == host_digest(*ref.copy(side))
+    # a planted change moves the digest
+    off, hub, dist = ref.copy(0)
+    dist = dist.copy()
+    dist[len(dist) // 2] += 1
+    assert L.checksum(0) != host_digest(off, hub, dist)
+
+
+def test_c1_grid100_label_file_is_the_reference_file(chl, tmp_path):
+    # C1: the label file the reference's own hhl wrote for this graph and order
+    # (sha256 in the manifest), from the graph file read as hhl reads it
+    m = MANIFEST["grid100"]
+    g = chl.Graph.grid(100, 100, 1, 100, seed=1)
+    order = g.order_betweenness(64, seed=1)
+    gp = tmp_path / "g.gr"
+    g.write(str(gp))
+    assert hashlib.sha256(gp.read_bytes()).hexdigest() == m["graph_sha256"]
+    g2 = chl.Graph.load(str(gp))
+    for mode in (chl.MODE_PLANT, chl.MODE_GLL, chl.MODE_HYBRID):
+        for batching in (chl.BATCH_STATIC, chl.BATCH_LEARNED):
+            L = chl.build(g2, order, mode=mode, batching=batching)
+            lp = tmp_path / f"l{mode}{batching}.txt"
+            L.write(str(lp))
+            assert hashlib.sha256(lp.read_bytes()).hexdigest() == m["labels_sha256"], (mode, batching)
+
+
+def test_c2_road_1M_schedules_agree_and_match_dijkstra(chl, orc):
+    # C2 at its own size: 1M points, 3-NN, 256-root sampled-betweenness order
+    g = chl.Graph.knn(1_000_000, 3, seed=1)
+    order = g.order_betweenness(256, seed=7)
+    learned = chl.build(g, order, batching=chl.BATCH_LEARNED)
+    static = chl.build(g, order, batching=chl.BATCH_STATIC)
+    assert learned.stats.labels == static.stats.labels
+    assert learned.checksum(0) == static.checksum(0)
+    oracle_dijkstra_check(orc, g, learned)
+
+
+@pytest.mark.parametrize("wmax", [1, 16])
+def test_c3_rmat20_hybrid_schedules_agree_and_match_dijkstra(chl, orc, wmax):
+    # C3 at its own size: R-MAT scale 20, edge factor 3, degree order, hybrid
+    g = chl.Graph.rmat(20, 3 << 20, wmax=wmax, seed=1)
+    order = g.order_degree(seed=2)
+    learned = chl.build(g, order, mode=chl.MODE_HYBRID, batching=chl.BATCH_LEARNED)
+    static = chl.build(g, order, mode=chl.MODE_HYBRID, batching=chl.BATCH_STATIC, switch_ratio=16.0)
+    assert learned.stats.labels == static.stats.labels
+    assert learned.checksum(0) == static.checksum(0)
+    oracle_dijkstra_check(orc, g, learned)
