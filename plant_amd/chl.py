@@ -89,6 +89,7 @@ def lib():
             "chl_order_write": ([C.c_char_p, vp, i64], C.c_int),
             "chl_order_betweenness": ([vp, i64, u64, vp], C.c_int),
             "chl_order_betweenness_scored": ([vp, i64, u64, vp, vp], C.c_int),
+            "chl_order_betweenness_device": ([vp, i64, u64, C.c_int, vp, vp, vp], C.c_int),
             "chl_order_degree": ([vp, u64, C.c_int, vp], C.c_int),
             "chl_build_opts_default": ([C.POINTER(BuildOpts)], None),
             "chl_build": ([vp, vp, i64, C.POINTER(BuildOpts), pp, C.POINTER(BuildStats)], C.c_int),
@@ -226,6 +227,16 @@ class Graph:
         sc = np.zeros(self.n, np.float32)
         _check(lib().chl_order_betweenness_scored(self._h, roots, seed, _ptr(o), _ptr(sc)))
         return o, sc
+
+    def order_betweenness_device(self, roots, seed=7, with_score=False, device=0):
+        """The same order (and scores) as order_betweenness, computed on the GPU.
+        Returns (order, score or None, device ms)."""
+        o = np.zeros(self.n, np.uint32)
+        sc = np.zeros(self.n, np.float32) if with_score else None
+        ms = C.c_double(0.0)
+        _check(lib().chl_order_betweenness_device(self._h, roots, seed, device, _ptr(o),
+                                                   _ptr(sc) if with_score else None, C.byref(ms)))
+        return o, sc, ms.value
 
     def order_degree(self, seed=7, reference_ties=False):
         o = np.zeros(self.n, np.uint32)

@@ -183,6 +183,32 @@ hipError_t launch_arc_ranks(const uint64_t* arc, uint64_t m, const uint32_t* ran
 // dst[idx[j]] = src[j] for j in [0, m)
 hipError_t launch_scatter_u32(const uint32_t* src, const uint32_t* idx, uint32_t m, uint32_t* dst, hipStream_t s);
 
+// Sampled-betweenness order (order.hip). Graph in device ids; per-slot scratch
+// sized n: cell 2 n words (distance, stamp/parent; unreached = (INF, 0)), acc n
+// words (children << 32 | subtree; rest state 1), three u32 lists.
+enum { BC_NEXT = 0, BC_REACHED = 1, BC_ARCS = 2, BC_ROUNDS = 3, BC_COUNT = 4 };
+struct BcParams {
+  const uint32_t* off;   // out-arcs, arc = (head << 32) | length
+  const uint64_t* arc;
+  const uint32_t* ioff;  // in-arcs, arc = (tail << 32) | length
+  const uint64_t* iarc;
+  const uint32_t* inv;   // device id -> vertex id (the parent tie rule)
+  const uint32_t* roots;  // device ids
+  uint32_t R;
+  uint32_t n;
+  uint32_t delta;
+  uint32_t* cell;
+  unsigned long long* acc;
+  uint32_t* qa;
+  uint32_t* qb;
+  uint32_t* vis;
+  unsigned long long* score;  // per device id, summed over roots
+  unsigned long long* ctr;    // BC_*
+};
+hipError_t launch_betweenness(const BcParams& p, uint32_t slots, hipStream_t s);
+hipError_t launch_score_keys(const unsigned long long* score, const uint32_t* inv, uint32_t n, uint64_t* key,
+                             uint64_t* val, hipStream_t s);
+
 // Label-store maintenance.
 // seg[v] = first index k of the sorted keys with key >> 32 >= v, for v in [0, n].
 hipError_t launch_seg_offsets(const uint64_t* keys, uint64_t m, uint32_t n, uint64_t* seg, hipStream_t s);

@@ -71,6 +71,11 @@ int ensure_device(int device);
 // dg: the graph on the device (built into *dg first when dg->device differs).
 int build_labels(const Graph& g, DeviceGraph& dg, const std::vector<uint32_t>& order, const chl_build_opts& opts,
                  chl_build_stats* stats, BuildHooks* hooks, LabelStore& L);
+// Sampled-betweenness order computed on the device; the same order (and
+// scores, per vertex id) as the host order_betweenness. device_ms: kernel and
+// sort time.
+int order_betweenness_device(const Graph& g, DeviceGraph& dg, int device, int64_t roots, uint64_t seed,
+                             std::vector<uint32_t>& order, std::vector<uint64_t>* score, double* device_ms);
 // Build under the batching policy of opts.batching (static rank order or learned packing).
 int run_build(const Graph& g, DeviceGraph& dg, const std::vector<uint32_t>& order, const chl_build_opts& opts,
               chl_build_stats* stats, LabelStore& L);

@@ -154,19 +154,23 @@ void gen_rmat(int scale, int64_t samples, double a, double b, double c, int32_t 
 // shortest-path tree over out-arcs whose parent of v is the smallest-id u with
 // dist(u) + len(u,v) == dist(v); a vertex scores the number of its proper
 // descendants, summed over the trees. Order: score descending, then id ascending.
+void sample_roots(int64_t n, int64_t roots, uint64_t seed, std::vector<uint32_t>& rs) {
+  roots = std::min<int64_t>(roots, n);
+  rs.clear();
+  std::vector<uint8_t> used(n, 0);
+  uint64_t s = seed;
+  while ((int64_t)rs.size() < roots) {
+    uint32_t r = (uint32_t)(splitmix64(s) % (uint64_t)n);
+    if (!used[r]) { used[r] = 1; rs.push_back(r); }
+  }
+}
+
 void order_betweenness(const Graph& g, int64_t roots, uint64_t seed, int threads, std::vector<uint32_t>& order,
                        std::vector<uint64_t>* score_out) {
   const int64_t n = g.n;
   roots = std::min<int64_t>(roots, n);
   std::vector<uint32_t> rs;
-  {
-    std::vector<uint8_t> used(n, 0);
-    uint64_t s = seed;
-    while ((int64_t)rs.size() < roots) {
-      uint32_t r = (uint32_t)(splitmix64(s) % (uint64_t)n);
-      if (!used[r]) { used[r] = 1; rs.push_back(r); }
-    }
-  }
+  sample_roots(n, roots, seed, rs);
   if (threads <= 0) threads = host_threads();
   int T = (int)std::min<int64_t>(threads, std::max<int64_t>(1, roots));
   std::vector<std::vector<uint64_t>> part(T, std::vector<uint64_t>(n, 0));

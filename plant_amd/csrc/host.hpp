@@ -48,6 +48,8 @@ void gen_rmat(int scale, int64_t samples, double a, double b, double c, int32_t 
               std::vector<Arc3>& arcs);
 
 // Vertex orders: order[rank] = vertex, rank 0 most important.
+// The distinct seeded roots of the sampled-betweenness order, in draw order.
+void sample_roots(int64_t n, int64_t roots, uint64_t seed, std::vector<uint32_t>& rs);
 void order_betweenness(const Graph& g, int64_t roots, uint64_t seed, int threads, std::vector<uint32_t>& order,
                        std::vector<uint64_t>* score);
 void order_degree(const Graph& g, uint64_t seed, bool reference_ties, std::vector<uint32_t>& order);

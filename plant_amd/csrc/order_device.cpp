@@ -1,0 +1,145 @@
+// Host side of the device sampled-betweenness order: roots, scratch slots, the
+// persistent tree launch, then a stable radix sort of (~score, id).
+#include <algorithm>
+#include <cstring>
+
+#include "engine.hpp"
+#include "internal.hpp"
+
+namespace chl {
+
+namespace {
+
+int order_impl(const Graph& g, DeviceGraph& dg, int device, int64_t roots, uint64_t seed,
+               std::vector<uint32_t>& order, std::vector<uint64_t>* score, double* device_ms, size_t* scratch) {
+  const int64_t n = g.n;
+  order.resize(n);
+  if (score) score->assign(n, 0);
+  if (device_ms) *device_ms = 0;
+  if (n == 0) return CHL_OK;
+  if (n >= 0xFFFFFFFFll) return set_error(CHL_E_RANGE, "more than 2^32 - 1 vertices");
+  int rc = ensure_device(device);
+  if (rc) return rc;
+  if (dg.device != device || dg.n != n) {
+    rc = dg.build(g, device);
+    if (rc) return rc;
+  }
+  std::vector<uint32_t> rs;
+  sample_roots(n, roots, seed, rs);
+  const uint32_t R = (uint32_t)rs.size();
+  for (auto& r : rs) r = dg.pos[r];
+
+  // bucket width: twice the mean arc length (the builds' default)
+  double sum = 0;
+  for (int32_t w : g.len[1]) sum += w;
+  const double mean = g.len[1].empty() ? 1.0 : sum / (double)g.len[1].size();
+  const uint32_t delta = (uint32_t)std::max(1.0, std::min(1e9, std::floor(2.0 * mean)));
+
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  CHL_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  struct Guard {
+    hipStream_t& s;
+    hipEvent_t &a, &b;
+    ~Guard() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+      if (s) (void)hipStreamDestroy(s);
+    }
+  } guard{stream, e0, e1};
+  CHL_HIP(hipEventCreate(&e0));
+  CHL_HIP(hipEventCreate(&e1));
+
+  // slots: 28 bytes of scratch per vertex each, within half the free memory
+  size_t freeb = 0, totalb = 0;
+  CHL_HIP(hipMemGetInfo(&freeb, &totalb));
+  const size_t per = (size_t)n * 28;
+  const size_t fit = std::max<size_t>(1, (freeb / 2) / per);
+  const uint32_t slots = (uint32_t)std::max<size_t>(1, std::min<size_t>({fit, (size_t)R, (size_t)1024}));
+  const size_t cells = (size_t)slots * n;
+  *scratch = cells * 28;
+  DBuf<uint64_t> cell, acc, score_d, key, val, skey, sval;
+  DBuf<uint32_t> qa, qb, vis, inv, droots;
+  DBuf<unsigned long long> ctr;
+  DBuf<uint8_t> tmp;
+  CHL_HIP(cell.reserve(cells));  // u32 pairs (distance, stamp)
+  CHL_HIP(acc.reserve(cells));
+  CHL_HIP(qa.reserve(cells));
+  CHL_HIP(qb.reserve(cells));
+  CHL_HIP(vis.reserve(cells));
+  CHL_HIP(score_d.reserve(n));
+  CHL_HIP(inv.reserve(n));
+  CHL_HIP(droots.reserve(std::max<uint32_t>(R, 1)));
+  CHL_HIP(ctr.reserve(BC_COUNT));
+  CHL_HIP(hipMemcpyAsync(inv.p, dg.inv.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream));
+  if (R) CHL_HIP(hipMemcpyAsync(droots.p, rs.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, stream));
+  CHL_HIP(launch_fill_u64(cell.p, 0x00000000FFFFFFFFull, cells, stream));  // distance INF, stamp 0
+  CHL_HIP(launch_fill_u64(acc.p, 1, cells, stream));
+  CHL_HIP(launch_fill_u64(score_d.p, 0, n, stream));
+  CHL_HIP(launch_fill_u64((uint64_t*)ctr.p, 0, BC_COUNT, stream));
+
+  BcParams p;
+  memset(&p, 0, sizeof(p));
+  const int ir = dg.sym ? 1 : 0;
+  p.off = dg.off[1].p;
+  p.arc = dg.arc[1].p;
+  p.ioff = dg.off[ir].p;
+  p.iarc = dg.arc[ir].p;
+  p.inv = inv.p;
+  p.roots = droots.p;
+  p.R = R;
+  p.n = (uint32_t)n;
+  p.delta = delta;
+  p.cell = reinterpret_cast<uint32_t*>(cell.p);
+  p.acc = reinterpret_cast<unsigned long long*>(acc.p);
+  p.qa = qa.p;
+  p.qb = qb.p;
+  p.vis = vis.p;
+  p.score = reinterpret_cast<unsigned long long*>(score_d.p);
+  p.ctr = ctr.p;
+  CHL_HIP(hipEventRecord(e0, stream));
+  CHL_HIP(launch_betweenness(p, slots, stream));
+
+  // order: stable ascending sort of ~score over ids in ascending order
+  CHL_HIP(key.reserve(n));
+  CHL_HIP(val.reserve(n));
+  CHL_HIP(skey.reserve(n));
+  CHL_HIP(sval.reserve(n));
+  CHL_HIP(launch_score_keys(p.score, inv.p, (uint32_t)n, key.p, val.p, stream));
+  size_t tb = 0;
+  CHL_HIP(sort_pairs_u64(nullptr, tb, key.p, skey.p, val.p, sval.p, (uint64_t)n, 64, stream));
+  CHL_HIP(tmp.reserve(tb));
+  CHL_HIP(sort_pairs_u64(tmp.p, tb, key.p, skey.p, val.p, sval.p, (uint64_t)n, 64, stream));
+  CHL_HIP(hipEventRecord(e1, stream));
+
+  std::vector<uint64_t> hv(n);
+  CHL_HIP(hipMemcpyAsync(hv.data(), sval.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, stream));
+  std::vector<uint64_t> hs;
+  if (score) {
+    hs.resize(n);
+    CHL_HIP(hipMemcpyAsync(hs.data(), score_d.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, stream));
+  }
+  CHL_HIP(hipStreamSynchronize(stream));
+  for (int64_t i = 0; i < n; ++i) order[i] = (uint32_t)hv[i];
+  if (score)
+    for (int64_t v = 0; v < n; ++v) (*score)[v] = hs[dg.pos[v]];
+  float ms = 0;
+  CHL_HIP(hipEventElapsedTime(&ms, e0, e1));
+  if (device_ms) *device_ms = ms;
+  return CHL_OK;
+}
+
+}  // namespace
+
+int order_betweenness_device(const Graph& g, DeviceGraph& dg, int device, int64_t roots, uint64_t seed,
+                             std::vector<uint32_t>& order, std::vector<uint64_t>* score, double* device_ms) {
+  size_t scratch = 0;
+  const int rc = order_impl(g, dg, device, roots, seed, order, score, device_ms, &scratch);
+  // large scratch goes back to the device rather than to the allocation cache,
+  // so that a following build on a large graph sees it as free
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) == hipSuccess && scratch > t / 16) dev_cache_flush();
+  return rc;
+}
+
+}  // namespace chl

@@ -127,6 +127,63 @@ def test_orders(chl, tmp_path):
     assert open(p).read().split("\n")[0] == "100"
 
 
+def _betweenness_all_roots(n, src, dst, w):
+    """Sampled betweenness with every vertex a root, restated with scipy's Dijkstra:
+    parent of v = smallest-id u with d(u) + w(u, v) == d(v); score = proper
+    descendants summed over the trees."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import dijkstra
+    A = csr_matrix((w.astype(np.float64), (src, dst)), shape=(n, n))
+    D = dijkstra(A, directed=True)
+    score = np.zeros(n, np.int64)
+    for r in range(n):
+        d = D[r]
+        par = np.full(n, n, np.int64)
+        for u, v, ww in zip(src, dst, w):
+            if np.isfinite(d[u]) and d[u] + ww == d[v] and v != r and u < par[v]:
+                par[v] = u
+        sub = np.ones(n, np.int64)
+        reach = np.flatnonzero(np.isfinite(d))
+        for v in reach[np.argsort(-d[reach], kind="stable")]:
+            if v != r:
+                sub[par[v]] += sub[v]
+        score[reach] += sub[reach] - 1
+    order = np.lexsort((np.arange(n), -score))
+    return order, score
+
+
+def test_betweenness_order_pinned(chl):
+    # closed form: on a unit path with every vertex a root, vertex k has k (n-1-k)
+    # proper descendants in each tree rooted on either side, and n-1 in its own
+    n = 9
+    src = np.arange(n - 1, dtype=np.uint32)
+    g = chl.Graph.from_arcs(n, src, src + 1, np.ones(n - 1, np.int32), undirected=True)
+    order, score = g.order_betweenness(n, seed=3, with_score=True)
+    k = np.arange(n)
+    assert np.array_equal(score, 2 * k * (n - 1 - k) + n - 1)
+    assert order.tolist() == [4, 3, 5, 2, 6, 1, 7, 0, 8]
+    # against scipy's Dijkstra and the tie rule, on weighted graphs with ties,
+    # a directed graph and a disconnected one
+    rng = np.random.default_rng(5)
+    cases = []
+    gr = chl.Graph.grid(6, 7, 1, 3, seed=2)  # small weights: many equal-length paths
+    cases.append((gr, *gr.arc_list(1)))
+    m = 160
+    s2, d2 = rng.integers(0, 40, m).astype(np.uint32), rng.integers(0, 40, m).astype(np.uint32)
+    w2 = rng.integers(1, 4, m).astype(np.int32)
+    gd = chl.Graph.from_arcs(40, s2, d2, w2)
+    cases.append((gd, *gd.arc_list(1)))
+    s3 = np.array([0, 1, 2, 4, 5], np.uint32)
+    d3 = np.array([1, 2, 0, 5, 6], np.uint32)
+    gu = chl.Graph.from_arcs(8, s3, d3, np.array([3, 4, 5, 1, 2], np.int32), undirected=True)
+    cases.append((gu, *gu.arc_list(1)))
+    for g, src, dst, w in cases:
+        want_order, want_score = _betweenness_all_roots(g.n, src, dst, w)
+        order, score = g.order_betweenness(g.n, seed=11, with_score=True)
+        assert np.array_equal(score.astype(np.int64), want_score)
+        assert np.array_equal(order, want_order)
+
+
 def test_library_exports_header(chl):
     lib = chl.lib()
     names = chl.header_symbols()
