@@ -222,6 +222,7 @@ struct Builder {
   DBuf<uint8_t> tmp;
   SideBatch sb[2];
   uint64_t out_cap = 0;
+  uint64_t next_need = 0;  // output estimate of the next batch (labels)
   double tail_labels = -1;  // labels per root at the top ranks of the last batch
   hipEvent_t ev[8];
 
@@ -304,6 +305,45 @@ struct Builder {
     if (used) CHL_HIP(hipMemcpyAsync(nb.p, buf.p, sizeof(uint64_t) * used, hipMemcpyDeviceToDevice, stream));
     buf.swap(nb);
     CHL_HIP(hipStreamSynchronize(stream));
+    return CHL_OK;
+  }
+
+  // When the device runs short (the store grows in place behind the batches),
+  // batch buffers far larger than the next batch needs go back to the device
+  // and so does the allocation cache: the first batches of a large graph grow
+  // them to many times what later batches use.
+  int trim_batch_buffers(uint64_t need) {
+    if (!need) return CHL_OK;
+    size_t freeb = 0, totalb = 0;
+    CHL_HIP(hipMemGetInfo(&freeb, &totalb));
+    if (freeb > totalb / 4) return CHL_OK;
+    const uint64_t keep = 2 * need + ((uint64_t)1 << 20);
+    size_t freed = 0;
+    auto trim = [&](auto& buf) {
+      if (buf.n > keep) {
+        freed += buf.bytes;
+        buf.release();
+      }
+    };
+    CHL_HIP(hipStreamSynchronize(stream));
+    for (auto& b : sb) {
+      trim(b.key);
+      trim(b.ent);
+      trim(b.skey);
+      trim(b.sent);
+      trim(b.ckey);
+      trim(b.cent);
+      trim(b.keep);
+    }
+    if (tmp.n > 16 * keep) {
+      freed += tmp.bytes;
+      tmp.release();
+    }
+    if (freed) {
+      out_cap = std::min<uint64_t>(out_cap, keep);
+      dev_cache_flush();
+      if (o.verbose) fprintf(stderr, "trim: %.2f GB of batch buffers back to the device\n", freed / 1e9);
+    }
     return CHL_OK;
   }
 
@@ -727,6 +767,8 @@ struct Builder {
         const uint32_t lw = hooks->next_width(hooks->ctx, r0, width, W);
         if (lw) bw = std::min<uint32_t>(lw, W);
       }
+      rc = trim_batch_buffers(next_need);
+      if (rc) return rc;
       {
         // memory cap under either policy: a batch's labels (output, sorted
         // copy, sort scratch: about 96 bytes each) within the free memory, at
@@ -858,7 +900,8 @@ struct Builder {
         const uint32_t nextR = fixed ? static_cap : std::min<uint32_t>(static_cap, width * 2);
         uint64_t emitted = std::max(sb[0].kept, nsides == 2 ? sb[1].kept : 0);
         uint64_t est = emitted / std::max<uint32_t>(R, 1) * nextR;
-        out_cap = std::max<uint64_t>(out_cap, est + est / 4);
+        next_need = est + est / 4;
+        out_cap = std::max<uint64_t>(out_cap, next_need);
         width = nextR;
       }
       r0 += R;
