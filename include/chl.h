@@ -115,7 +115,8 @@ typedef struct chl_build_opts {
   int64_t batch_roots;  /* roots per batch (0 = default) */
   int32_t delta;        /* bucket width of the delta-stepping relaxation (0 = twice the mean arc length) */
   int32_t slots;        /* concurrent root slots = workgroups of the tree kernel (0 = default) */
-  int granularity;      /* per-root parallelism: 0 = default (workgroup), 1 = wavefront, 2 = workgroup */
+  int granularity;      /* per-root parallelism: 0 = default (workgroup), 1 = wavefront, 2 = workgroup of
+                           256 threads, 3 = workgroup of 1024 threads (a quarter of the roots in flight) */
   double switch_ratio;  /* ps of distributed_memory/hhl.cpp:23 (default 1024) */
   int plant_prune;      /* 1: PLaNT trees also prune by the completed rank-prefix labels */
   int device;           /* HIP device ordinal */
@@ -147,7 +148,7 @@ typedef struct chl_build_stats {
      batches the MLP predicted */
   double pred_mae_learned, pred_mae_static;
   int64_t pred_roots;           /* roots those errors cover */
-  int64_t granularity_used;     /* 1 wavefront, 2 workgroup per root */
+  int64_t granularity_used;     /* 1 wavefront, 2 workgroup, 3 wide workgroup per root */
   double autotune_ms;           /* wall time spent by the autotuner (its trial trees included) */
   int64_t switch_trials;        /* hybrid under learned batching: pruned trial batches run */
   double policy_ms;             /* learned batching: wall time inside the policy (features, MLP, hooks) */

@@ -171,9 +171,9 @@ struct TreeParams {
   unsigned long long* ctr;  // CTR_* (CTR_NEXT is the work-queue cursor)
 };
 
-// One workgroup per slot: 256 threads (workgroup per root) or, with
-// wave_per_root, 64 threads (wavefront per root).
-hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, int wave_per_root, hipStream_t s);
+// One workgroup per slot of threads_per_root threads: 64 (a wavefront per root),
+// 256 or 1024.
+hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, int threads_per_root, hipStream_t s);
 // count cells set to the unreached state (state all ones, stamp 0)
 hipError_t launch_init_cells(uint64_t* cell, size_t count, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t count, hipStream_t s);

@@ -190,6 +190,13 @@ struct SideBatch {
   uint64_t kept = 0;    // labels appended
 };
 
+// Search slots for a granularity: base_slots workgroups of 256 threads, as
+// many threads in flight at 64 or 1024 threads per root.
+static uint32_t slots_for(uint32_t base_slots, uint32_t tpr) {
+  return std::max<uint32_t>(1, (uint32_t)((uint64_t)base_slots * 256 / tpr));
+}
+static int64_t granularity_of(uint32_t tpr) { return tpr == 64 ? 1 : tpr == 1024 ? 3 : 2; }
+
 struct Builder {
   const Graph& g;
   DeviceGraph& dg;
@@ -209,7 +216,7 @@ struct Builder {
   std::vector<uint32_t> hids;        // staging of root ids
   uint32_t slots = 0, gcap = 0;
   uint32_t active = 0;  // slots used by launches (at most slots)
-  int wave = 0;         // 1: wavefront per root
+  uint32_t tpr = 256;   // threads per root: 64 (wavefront), 256 or 1024 (workgroups)
   DBuf<uint32_t> roots, roots2, spill, tree_size, label_count, touched, origd;
   uint32_t W = 1;            // widest batch (scratch offset of the statistics arrays)
   int64_t stats_batches = 0;
@@ -395,7 +402,7 @@ struct Builder {
     uint64_t lab0 = 0;  // label counter at the start of the pass
     for (int pass = 0;; ++pass) {
       CHL_HIP(hipEventRecord(ev[0], stream));
-      CHL_HIP(launch_tree(p, flags, std::min<uint32_t>(active, p.R), wave, stream));
+      CHL_HIP(launch_tree(p, flags, std::min<uint32_t>(active, p.R), (int)tpr, stream));
       CHL_HIP(hipEventRecord(ev[1], stream));
       if (pass > 0) {
         // this pass's statistics to the roots' positions in the batch
@@ -502,13 +509,16 @@ struct Builder {
 
   // Autotuner: times the PLaNT trees of a fixed sample of roots (ranks
   // log-uniform over [max(1024, n/64), n); labels discarded) under each candidate
-  // per-root granularity (workgroup, wavefront) and bucket width (1x, 2x, 4x the
-  // default) and keeps the fastest. Labels depend on neither.
+  // per-root granularity (wavefront, workgroup, wide workgroup) at twice the
+  // default bucket width, then the best granularity at 1x and 4x, and keeps the
+  // fastest. The sample fills the slots of every granularity (as many roots as
+  // wavefront slots), so that no candidate wins by idle slots of another.
+  // Labels depend on neither.
   int autotune(LabelStore& L, double& lt, uint32_t base_slots) {
     const double t0 = now_s();
     std::mt19937_64 rng(0x5EEDu);
     const int64_t lo = std::min<int64_t>(std::max<int64_t>(1024, n / 64), n / 2);
-    const uint32_t R = (uint32_t)std::min<int64_t>(256, n - lo);
+    const uint32_t R = (uint32_t)std::min<int64_t>(std::min<uint32_t>(W, slots_for(base_slots, 64)), n - lo);
     std::vector<uint32_t> sample(R);
     std::uniform_real_distribution<double> u(std::log((double)lo + 1.0), std::log((double)n));
     for (auto& r : sample) r = (uint32_t)std::min<int64_t>(n - 1, std::max<int64_t>(lo, (int64_t)std::exp(u(rng)) - 1));
@@ -517,39 +527,44 @@ struct Builder {
     CHL_HIP(hipMemcpyAsync(roots.p, hids.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, stream));
     const int32_t base_delta = o.delta;
     double best = 1e300;
-    int best_wave = wave;
+    uint32_t best_tpr = tpr;
     int32_t best_delta = o.delta;
     uint32_t best_active = active;
     double warm = 0;
     int rc = run_trees(0, R, sample.data(), 0, TREE_FLAGS, L, warm);  // first launch of the kernel
     if (rc) return rc;
     lt += warm;
-    for (int gran = 0; gran < 2; ++gran)
-      for (int32_t mult : {1, 2, 4}) {
-        wave = gran;
-        active = std::min<uint32_t>(slots, gran ? 4 * base_slots : base_slots);
-        o.delta = std::max<int32_t>(1, base_delta * mult);
-        double ms = 0;
-        rc = run_trees(0, R, sample.data(), 0, TREE_FLAGS, L, ms);
-        if (rc) return rc;
-        lt += ms;
-        if (o.verbose) fprintf(stderr, "autotune: %s delta %d slots %u: %.3f ms\n", gran ? "wave" : "workgroup",
-                               o.delta, active, ms);
-        if (ms < best) {
-          best = ms;
-          best_wave = wave;
-          best_delta = o.delta;
-          best_active = active;
-        }
+    std::vector<std::pair<uint32_t, int32_t>> cand = {{256u, 2}, {64u, 2}, {1024u, 2}};
+    for (size_t c = 0; c < cand.size(); ++c) {
+      tpr = cand[c].first;
+      active = std::min<uint32_t>(slots, slots_for(base_slots, tpr));
+      o.delta = std::max<int32_t>(1, base_delta * cand[c].second);
+      double ms = 0;
+      rc = run_trees(0, R, sample.data(), 0, TREE_FLAGS, L, ms);
+      if (rc) return rc;
+      lt += ms;
+      if (o.verbose) fprintf(stderr, "autotune: %u threads per root, delta %d slots %u: %.3f ms\n", tpr, o.delta,
+                             active, ms);
+      if (ms < best) {
+        best = ms;
+        best_tpr = tpr;
+        best_delta = o.delta;
+        best_active = active;
       }
-    wave = best_wave;
+      if (c == 2) {
+        // second stage: the bucket width at the best granularity
+        cand.push_back({best_tpr, 1});
+        cand.push_back({best_tpr, 4});
+      }
+    }
+    tpr = best_tpr;
     o.delta = best_delta;
     active = best_active;
     if (stats) {
       stats->autotune_ms = (now_s() - t0) * 1e3;
       stats->delta_used = o.delta;
       stats->slots_used = active;
-      stats->granularity_used = wave ? 1 : 2;
+      stats->granularity_used = granularity_of(tpr);
     }
     return CHL_OK;
   }
@@ -693,11 +708,12 @@ struct Builder {
       double mean = g.len[1].empty() ? 1.0 : sum / (double)g.len[1].size();
       o.delta = (int32_t)std::max(1.0, std::min(1e9, std::floor(2.0 * mean)));
     }
-    // wavefront granularity runs four roots where a workgroup runs one
-    wave = o.granularity == 1 ? 1 : 0;
+    // the same threads in flight at every granularity: a wavefront per root runs
+    // four roots where a workgroup runs one, a wide workgroup a quarter
+    tpr = o.granularity == 1 ? 64u : o.granularity == 3 ? 1024u : 256u;
     const uint32_t base_slots = o.slots > 0 ? (uint32_t)o.slots : 1024u;
-    rc = alloc_slots(o.autotune && o.slots <= 0 ? 4 * base_slots : (wave ? 4 * base_slots : base_slots));
-    active = std::min<uint32_t>(slots, wave ? 4 * base_slots : base_slots);
+    rc = alloc_slots(o.autotune && o.slots <= 0 ? 4 * base_slots : slots_for(base_slots, tpr));
+    active = std::min<uint32_t>(slots, slots_for(base_slots, tpr));
     if (rc) return rc;
     // Static schedule: a fixed width when batch_roots is given; otherwise widths
     // doubling from the slot count up to 65536 roots (later roots have small
@@ -735,7 +751,7 @@ struct Builder {
       stats->switch_rank = -1;
       stats->delta_used = o.delta;
       stats->slots_used = active;
-      stats->granularity_used = wave ? 1 : 2;
+      stats->granularity_used = granularity_of(tpr);
       stats->batch_used = W;
     }
 

@@ -118,7 +118,9 @@ __device__ __forceinline__ void relax(uint64_t* cell, uint32_t* N, uint32_t* vis
 }
 
 // THREADS = 256: a workgroup of four wavefronts per root; 64: one wavefront per
-// root (more roots in flight, no cross-wave barrier cost for small trees).
+// root (more roots in flight, no cross-wave barrier cost for small trees); 1024:
+// sixteen wavefronts per root (a quarter of the trees in flight, so a quarter of
+// the per-tree state competes for each XCD's L2, for wide search fronts).
 template <bool FLAGS, bool PRUNE, int THREADS>
 __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
   constexpr int kThreads = THREADS;
@@ -418,8 +420,9 @@ static void launch_tree_t(const TreeParams& p, uint32_t flags, uint32_t slots, h
   }
 }
 
-hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, int wave_per_root, hipStream_t s) {
-  if (wave_per_root) launch_tree_t<64>(p, flags, slots, s);
+hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, int threads_per_root, hipStream_t s) {
+  if (threads_per_root == 64) launch_tree_t<64>(p, flags, slots, s);
+  else if (threads_per_root == 1024) launch_tree_t<1024>(p, flags, slots, s);
   else launch_tree_t<kThreads>(p, flags, slots, s);
   return hipGetLastError();
 }

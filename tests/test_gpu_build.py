@@ -52,7 +52,7 @@ def test_grid_schedule_independent(chl, grid, batch, delta, slots):
         assert_same_labels(L, ref)
 
 
-@pytest.mark.parametrize("granularity", [1, 2])
+@pytest.mark.parametrize("granularity", [1, 2, 3])
 def test_granularity_and_autotune(chl, orc, granularity):
     g = chl.Graph.knn(6000, 3, seed=12)
     order = g.order_betweenness(32, seed=4)
@@ -62,7 +62,7 @@ def test_granularity_and_autotune(chl, orc, granularity):
         assert_same_labels(L, ref)
     L = chl.build(g, order, autotune=1, batching=chl.BATCH_LEARNED)
     assert_same_labels(L, ref)
-    assert L.stats.autotune_ms > 0 and L.stats.granularity_used in (1, 2)
+    assert L.stats.autotune_ms > 0 and L.stats.granularity_used in (1, 2, 3)
 
 
 def test_output_spill_relaunch(chl, grid):
