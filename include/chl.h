@@ -111,7 +111,8 @@ typedef struct chl_build_opts {
                            device tests it); under learned batching where the trees predicted for the
                            next roots would cost more than pruned trees measured on a trial batch */
   int batching;         /* CHL_BATCH_STATIC (rank order) or CHL_BATCH_LEARNED (MLP-predicted packing) */
-  int autotune;         /* 1: pick batch width / delta / slots by measurement on the first batches */
+  int autotune;         /* 1 (PLaNT mode): pick per-root granularity and bucket width by timing trial trees
+                           of sampled ranks before the first batch; ignored in the other modes */
   int64_t batch_roots;  /* roots per batch (0 = default) */
   int32_t delta;        /* bucket width of the delta-stepping relaxation (0 = twice the mean arc length) */
   int32_t slots;        /* concurrent root slots = workgroups of the tree kernel (0 = default) */

@@ -509,16 +509,17 @@ struct Builder {
 
   // Autotuner: times the PLaNT trees of a fixed sample of roots (ranks
   // log-uniform over [max(1024, n/64), n); labels discarded) under each candidate
-  // per-root granularity (wavefront, workgroup, wide workgroup) at twice the
-  // default bucket width, then the best granularity at 1x and 4x, and keeps the
-  // fastest. The sample fills the slots of every granularity (as many roots as
-  // wavefront slots), so that no candidate wins by idle slots of another.
-  // Labels depend on neither.
+  // per-root granularity (workgroup, wavefront) at twice the default bucket
+  // width, then the better one at 1x and 4x, and keeps the fastest. Labels depend
+  // on neither. The wide workgroup (1024 threads) is not a candidate: on a
+  // sample it wins by finishing the largest sampled tree sooner, while over the
+  // batches of a whole build (many roots per slot) it was slower in every batch
+  // but the first on the 1M-vertex road-like graph.
   int autotune(LabelStore& L, double& lt, uint32_t base_slots) {
     const double t0 = now_s();
     std::mt19937_64 rng(0x5EEDu);
     const int64_t lo = std::min<int64_t>(std::max<int64_t>(1024, n / 64), n / 2);
-    const uint32_t R = (uint32_t)std::min<int64_t>(std::min<uint32_t>(W, slots_for(base_slots, 64)), n - lo);
+    const uint32_t R = (uint32_t)std::min<int64_t>(std::min<uint32_t>(W, 2 * base_slots), n - lo);
     std::vector<uint32_t> sample(R);
     std::uniform_real_distribution<double> u(std::log((double)lo + 1.0), std::log((double)n));
     for (auto& r : sample) r = (uint32_t)std::min<int64_t>(n - 1, std::max<int64_t>(lo, (int64_t)std::exp(u(rng)) - 1));
@@ -534,7 +535,7 @@ struct Builder {
     int rc = run_trees(0, R, sample.data(), 0, TREE_FLAGS, L, warm);  // first launch of the kernel
     if (rc) return rc;
     lt += warm;
-    std::vector<std::pair<uint32_t, int32_t>> cand = {{256u, 2}, {64u, 2}, {1024u, 2}};
+    std::vector<std::pair<uint32_t, int32_t>> cand = {{256u, 2}, {64u, 2}};
     for (size_t c = 0; c < cand.size(); ++c) {
       tpr = cand[c].first;
       active = std::min<uint32_t>(slots, slots_for(base_slots, tpr));
@@ -551,7 +552,7 @@ struct Builder {
         best_delta = o.delta;
         best_active = active;
       }
-      if (c == 2) {
+      if (c == 1) {
         // second stage: the bucket width at the best granularity
         cand.push_back({best_tpr, 1});
         cand.push_back({best_tpr, 4});
@@ -759,7 +760,10 @@ struct Builder {
     const bool policy_switch = o.mode == CHL_MODE_HYBRID && hooks && hooks->plant_next;
     std::vector<uint32_t> hroots(W), hts(W), hlc(W);
     double lt = 0, stt = 0, ct = 0;
-    if (o.autotune && plant_phase && n >= 4096) {
+    // PLaNT mode only: in hybrid mode the PLaNT trees of the sampled ranks are
+    // the slow trees that the switch exists to avoid (on R-MAT scale 20 the
+    // trials alone took several times the whole build)
+    if (o.autotune && o.mode == CHL_MODE_PLANT && n >= 4096) {
       rc = autotune(L, lt, base_slots);
       if (rc) return rc;
     }
