@@ -94,18 +94,19 @@ struct Book {
   }
 };
 
-// Relaxes arc (u -> v, w), v of rank rv, from state (d, lo); queues v in N once
-// per round.
+// Relaxes arc (u -> v, w), v of rank rv, from state (d, lo), given a recent read
+// `seen` of v's state (a stale read only costs an atomic that changes nothing);
+// queues v in N once per round.
 template <bool FLAGS>
-__device__ __forceinline__ void relax(uint64_t* cell, uint32_t* N, uint32_t* vis, uint32_t* s_nsize,
-                                      uint32_t* s_nvis, uint32_t nstamp, uint64_t arc, uint32_t rv, uint32_t d,
-                                      uint32_t lo, uint32_t B, Book& bk) {
+__device__ __forceinline__ void relax_seen(uint64_t* cell, uint32_t* N, uint32_t* vis, uint32_t* s_nsize,
+                                           uint32_t* s_nvis, uint32_t nstamp, uint64_t arc, uint32_t rv,
+                                           uint64_t seen, uint32_t d, uint32_t lo, uint32_t B, Book& bk) {
   const uint32_t v = (uint32_t)(arc >> 32);
   const uint32_t nd = d + (uint32_t)arc;
   const uint32_t nlo = FLAGS ? min(lo, rv) : lo;
   const uint64_t nv = ((uint64_t)nd << 32) | nlo;
   uint64_t* sv = cell + 2 * (size_t)v;
-  if (nv < *sv) {
+  if (nv < seen) {
     const uint64_t old = atomicMin((unsigned long long*)sv, (unsigned long long)nv);
     if (nv < old) {
       bk.note(nd, B);
@@ -116,6 +117,19 @@ __device__ __forceinline__ void relax(uint64_t* cell, uint32_t* N, uint32_t* vis
     }
   }
 }
+
+template <bool FLAGS>
+__device__ __forceinline__ void relax(uint64_t* cell, uint32_t* N, uint32_t* vis, uint32_t* s_nsize,
+                                      uint32_t* s_nvis, uint32_t nstamp, uint64_t arc, uint32_t rv, uint32_t d,
+                                      uint32_t lo, uint32_t B, Book& bk) {
+  relax_seen<FLAGS>(cell, N, vis, s_nsize, s_nvis, nstamp, arc, rv, cell[2 * (size_t)(uint32_t)(arc >> 32)], d, lo,
+                    B, bk);
+}
+
+// Arcs of one vertex relaxed per step of the lane-per-vertex loop: their head
+// states are read together, so the reads overlap instead of following each
+// other's atomics.
+constexpr uint32_t kArcStep = 4;
 
 // THREADS = 256: a workgroup of four wavefronts per root; 64: one wavefront per
 // root (more roots in flight, no cross-wave barrier cost for small trees); 1024:
@@ -222,6 +236,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
         for (uint32_t k = tid; k < fsize; k += kThreads) {
           const uint32_t u = F[k];
           const uint64_t x = cell[2 * (size_t)u];
+          const uint32_t ab = p.off[u], ae = p.off[u + 1];  // read beside the state
           const uint32_t d = (uint32_t)(x >> 32);
           const uint32_t lo = (uint32_t)x;
           if (d >= B) {
@@ -231,10 +246,24 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
           }
           if (!FLAGS && rank_of(cell, u) < skip) continue;
           ++c_proc;
-          const uint32_t ab = p.off[u], ae = p.off[u + 1];
           c_arcs += ae - ab;
-          for (uint32_t a = ab; a < ae; ++a)
-            relax<FLAGS>(cell, N, vis, &s_nsize, &s_nvis, nstamp, p.arc[a], p.arank[a], d, lo, B, bk);
+          for (uint32_t a0 = ab; a0 < ae; a0 += kArcStep) {
+            uint64_t av[kArcStep], seen[kArcStep];
+            uint32_t rv[kArcStep];
+#pragma unroll
+            for (uint32_t j = 0; j < kArcStep; ++j)
+              if (a0 + j < ae) {
+                av[j] = p.arc[a0 + j];
+                rv[j] = p.arank[a0 + j];
+              }
+#pragma unroll
+            for (uint32_t j = 0; j < kArcStep; ++j)
+              if (a0 + j < ae) seen[j] = cell[2 * (size_t)(uint32_t)(av[j] >> 32)];
+#pragma unroll
+            for (uint32_t j = 0; j < kArcStep; ++j)
+              if (a0 + j < ae)
+                relax_seen<FLAGS>(cell, N, vis, &s_nsize, &s_nvis, nstamp, av[j], rv[j], seen[j], d, lo, B, bk);
+          }
         }
       } else {
         // one wavefront per frontier entry: prune check, then relax
