@@ -141,7 +141,7 @@ def serving(chl, L, g, rng, seconds):
     sizes = np.floor(np.exp(rng.uniform(0, np.log(65536.0), nreq))).astype(np.int64)
     total = int(sizes.sum())
     pairs = rng.integers(0, n, (total, 2), dtype=np.uint32)
-    local = chl.local_pairs(g, 200000, 50, seed=seed)
+    local = chl.local_pairs(g, 200000, 50, seed=int(rng.integers(1 << 62)))
     pick = rng.random(total) < 0.2
     pairs[pick] = local[rng.integers(0, local.shape[0], int(pick.sum()))]
     arrivals = np.cumsum(rng.exponential(mean_size / rate * 1e6, nreq))
