@@ -137,7 +137,6 @@ struct TreeParams {
   // vertex's position in the vertex order
   const uint32_t* off;
   const uint64_t* arc;
-  const uint32_t* arank;  // rank of each arc's head, parallel to arc
   const uint32_t* rank;
   uint32_t n;
   // label store (rows in rank ids, hub-sorted, all hubs < prefix)
@@ -174,12 +173,11 @@ struct TreeParams {
 // One workgroup per slot of threads_per_root threads: 64 (a wavefront per root),
 // 256 or 1024.
 hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, int threads_per_root, hipStream_t s);
-// count cells set to the unreached state (state all ones, stamp 0)
-hipError_t launch_init_cells(uint64_t* cell, size_t count, hipStream_t s);
+// count cells set to the unreached state (state all ones, stamp 0) with the
+// rank of their vertex (cell i: device id i mod n)
+hipError_t launch_init_cells(uint64_t* cell, size_t count, const uint32_t* rank, uint32_t n, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t count, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s);
-// arank[a] = rank[arc[a] >> 32] for a in [0, m)
-hipError_t launch_arc_ranks(const uint64_t* arc, uint64_t m, const uint32_t* rank, uint32_t* arank, hipStream_t s);
 // dst[idx[j]] = src[j] for j in [0, m)
 hipError_t launch_scatter_u32(const uint32_t* src, const uint32_t* idx, uint32_t m, uint32_t* dst, hipStream_t s);
 

@@ -211,7 +211,6 @@ struct Builder {
 
   DBuf<uint64_t> cell;
   DBuf<uint32_t> qa, qb, vis, gkey, gval, grank;
-  DBuf<uint32_t> arank[2];  // this build's rank of every arc head, parallel to dg.arc
   std::vector<uint32_t> id_of_rank;  // rank -> device id
   std::vector<uint32_t> hids;        // staging of root ids
   uint32_t slots = 0, gcap = 0;
@@ -260,12 +259,6 @@ struct Builder {
     }
     CHL_HIP(grank.reserve(n));
     CHL_HIP(hipMemcpyAsync(grank.p, rank_of_id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream));
-    for (int r = 0; r < 2; ++r) {
-      if (sym && r == 0) continue;
-      const uint64_t m = (uint64_t)g.off[r][n];
-      CHL_HIP(arank[r].reserve(std::max<uint64_t>(m, 1)));
-      CHL_HIP(launch_arc_ranks(dg.arc[r].p, m, grank.p, arank[r].p, stream));
-    }
     CHL_HIP(hipStreamSynchronize(stream));
     return CHL_OK;
   }
@@ -284,7 +277,7 @@ struct Builder {
     CHL_HIP(qa.reserve(cells));
     CHL_HIP(qb.reserve(cells));
     CHL_HIP(vis.reserve(cells));
-    CHL_HIP(launch_init_cells(cell.p, cells, stream));
+    CHL_HIP(launch_init_cells(cell.p, cells, grank.p, (uint32_t)n, stream));
     return CHL_OK;
   }
 
@@ -368,7 +361,6 @@ struct Builder {
     memset(&p, 0, sizeof(p));
     p.off = dg.off[sym ? 1 : range].p;
     p.arc = dg.arc[sym ? 1 : range].p;
-    p.arank = arank[sym ? 1 : range].p;
     p.n = (uint32_t)n;
     p.scan_off = L.d_off(side);
     p.scan_ent = L.d_ent(side);

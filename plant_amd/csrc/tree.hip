@@ -69,10 +69,10 @@ __device__ __forceinline__ void table_insert(uint32_t* key, uint32_t* val, uint3
 }
 
 // cell[2 v] = state word; cell[2 v + 1] = round stamp (low half) and the
-// vertex's rank (high half), both written by the one 64-bit exchange that
-// queues the vertex, so that processing and emission read one 16-byte cell and
-// no rank array. (A plain store of the rank beside the atomics on the same
-// word measured twice as slow on gfx950.)
+// vertex's rank (high half). The rank is written once per build when the slots
+// are initialised and kept by every reset, so one 16-byte load gives a head's
+// state and rank during a relaxation, and processing and emission read no rank
+// array; queueing exchanges the stamp half only.
 __device__ __forceinline__ uint32_t* stamp_of(uint64_t* cell, uint32_t v) {
   return reinterpret_cast<uint32_t*>(cell + 2 * (size_t)v + 1);
 }
@@ -94,36 +94,40 @@ struct Book {
   }
 };
 
-// Relaxes arc (u -> v, w), v of rank rv, from state (d, lo), given a recent read
-// `seen` of v's state (a stale read only costs an atomic that changes nothing);
-// queues v in N once per round.
+// The 16-byte cell of v (state word, stamp and rank).
+__device__ __forceinline__ ulonglong2 load_cell(const uint64_t* cell, uint32_t v) {
+  return *reinterpret_cast<const ulonglong2*>(cell + 2 * (size_t)v);
+}
+
+// Relaxes arc (u -> v, w) from state (d, lo), given a recent read `seen` of v's
+// cell (v's rank is fixed; a stale state only costs an atomic that changes
+// nothing); queues v in N once per round.
 template <bool FLAGS>
 __device__ __forceinline__ void relax_seen(uint64_t* cell, uint32_t* N, uint32_t* vis, uint32_t* s_nsize,
-                                           uint32_t* s_nvis, uint32_t nstamp, uint64_t arc, uint32_t rv,
-                                           uint64_t seen, uint32_t d, uint32_t lo, uint32_t B, Book& bk) {
+                                           uint32_t* s_nvis, uint32_t nstamp, uint64_t arc, ulonglong2 seen,
+                                           uint32_t d, uint32_t lo, uint32_t B, Book& bk) {
   const uint32_t v = (uint32_t)(arc >> 32);
+  const uint32_t rv = (uint32_t)(seen.y >> 32);
   const uint32_t nd = d + (uint32_t)arc;
   const uint32_t nlo = FLAGS ? min(lo, rv) : lo;
   const uint64_t nv = ((uint64_t)nd << 32) | nlo;
   uint64_t* sv = cell + 2 * (size_t)v;
-  if (nv < seen) {
+  if (nv < seen.x) {
     const uint64_t old = atomicMin((unsigned long long*)sv, (unsigned long long)nv);
     if (nv < old) {
       bk.note(nd, B);
       if (old == kStateInf) vis[atomicAdd(s_nvis, 1u)] = v;
-      const uint64_t os = atomicExch((unsigned long long*)(cell + 2 * (size_t)v + 1),
-                                     (unsigned long long)(((uint64_t)rv << 32) | nstamp));
-      if ((uint32_t)os != nstamp) N[atomicAdd(s_nsize, 1u)] = v;
+      if (atomicExch(stamp_of(cell, v), nstamp) != nstamp) N[atomicAdd(s_nsize, 1u)] = v;
     }
   }
 }
 
 template <bool FLAGS>
 __device__ __forceinline__ void relax(uint64_t* cell, uint32_t* N, uint32_t* vis, uint32_t* s_nsize,
-                                      uint32_t* s_nvis, uint32_t nstamp, uint64_t arc, uint32_t rv, uint32_t d,
-                                      uint32_t lo, uint32_t B, Book& bk) {
-  relax_seen<FLAGS>(cell, N, vis, s_nsize, s_nvis, nstamp, arc, rv, cell[2 * (size_t)(uint32_t)(arc >> 32)], d, lo,
-                    B, bk);
+                                      uint32_t* s_nvis, uint32_t nstamp, uint64_t arc, uint32_t d, uint32_t lo,
+                                      uint32_t B, Book& bk) {
+  relax_seen<FLAGS>(cell, N, vis, s_nsize, s_nvis, nstamp, arc, load_cell(cell, (uint32_t)(arc >> 32)), d, lo, B,
+                    bk);
 }
 
 // Arcs of one vertex relaxed per step of the lane-per-vertex loop: their head
@@ -248,21 +252,18 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
           ++c_proc;
           c_arcs += ae - ab;
           for (uint32_t a0 = ab; a0 < ae; a0 += kArcStep) {
-            uint64_t av[kArcStep], seen[kArcStep];
-            uint32_t rv[kArcStep];
+            uint64_t av[kArcStep];
+            ulonglong2 seen[kArcStep];
 #pragma unroll
             for (uint32_t j = 0; j < kArcStep; ++j)
-              if (a0 + j < ae) {
-                av[j] = p.arc[a0 + j];
-                rv[j] = p.arank[a0 + j];
-              }
+              if (a0 + j < ae) av[j] = p.arc[a0 + j];
 #pragma unroll
             for (uint32_t j = 0; j < kArcStep; ++j)
-              if (a0 + j < ae) seen[j] = cell[2 * (size_t)(uint32_t)(av[j] >> 32)];
+              if (a0 + j < ae) seen[j] = load_cell(cell, (uint32_t)(av[j] >> 32));
 #pragma unroll
             for (uint32_t j = 0; j < kArcStep; ++j)
               if (a0 + j < ae)
-                relax_seen<FLAGS>(cell, N, vis, &s_nsize, &s_nvis, nstamp, av[j], rv[j], seen[j], d, lo, B, bk);
+                relax_seen<FLAGS>(cell, N, vis, &s_nsize, &s_nvis, nstamp, av[j], seen[j], d, lo, B, bk);
           }
         }
       } else {
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
           const uint32_t ab = p.off[u], ae = p.off[u + 1];
           if (lane == 0) c_arcs += ae - ab;
           for (uint32_t a = ab + lane; a < ae; a += 64)
-            relax<FLAGS>(cell, N, vis, &s_nsize, &s_nvis, nstamp, p.arc[a], p.arank[a], d, lo, B, bk);
+            relax<FLAGS>(cell, N, vis, &s_nsize, &s_nvis, nstamp, p.arc[a], d, lo, B, bk);
         }
       }
       // bookkeeping of the next frontier (wavefront reductions, then LDS)
@@ -382,7 +383,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
         p.out_ent[o] = ((uint64_t)rr << 32) | (x >> 32);
       }
       cell[2 * (size_t)v] = kStateInf;
-      cell[2 * (size_t)v + 1] = 0;
+      cell[2 * (size_t)v + 1] = (uint64_t)rv << 32;  // the rank stays
     }
     // tree size = processed entries of this root
     if (c_proc) atomicAdd(&s_ctr[0], c_proc);
@@ -405,11 +406,12 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
   }
 }
 
-// Cells start unreached: state all ones, stamp 0.
-__global__ void init_cells_kernel(uint64_t* cell, size_t count) {
+// Cells start unreached: state all ones, stamp 0, and the vertex's rank (cell i
+// of a slot belongs to device id i mod n).
+__global__ void init_cells_kernel(uint64_t* cell, size_t count, const uint32_t* rank, uint32_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
     cell[2 * i] = kStateInf;
-    cell[2 * i + 1] = 0;
+    cell[2 * i + 1] = (uint64_t)rank[i % n] << 32;
   }
 }
 __global__ void fill_u64_kernel(uint64_t* p, uint64_t v, size_t count) {
@@ -419,11 +421,6 @@ __global__ void fill_u64_kernel(uint64_t* p, uint64_t v, size_t count) {
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t count) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
     p[i] = v;
-}
-
-__global__ void arc_ranks_kernel(const uint64_t* arc, uint64_t m, const uint32_t* rank, uint32_t* arank) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x)
-    arank[i] = rank[arc[i] >> 32];
 }
 
 __global__ void scatter_u32_kernel(const uint32_t* src, const uint32_t* idx, uint32_t m, uint32_t* dst) {
@@ -456,9 +453,9 @@ hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, int 
   return hipGetLastError();
 }
 
-hipError_t launch_init_cells(uint64_t* cell, size_t count, hipStream_t s) {
+hipError_t launch_init_cells(uint64_t* cell, size_t count, const uint32_t* rank, uint32_t n, hipStream_t s) {
   if (!count) return hipSuccess;
-  hipLaunchKernelGGL(init_cells_kernel, dim3(fill_blocks(count)), dim3(256), 0, s, cell, count);
+  hipLaunchKernelGGL(init_cells_kernel, dim3(fill_blocks(count)), dim3(256), 0, s, cell, count, rank, n);
   return hipGetLastError();
 }
 
@@ -475,11 +472,6 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s)
 hipError_t launch_scatter_u32(const uint32_t* src, const uint32_t* idx, uint32_t m, uint32_t* dst, hipStream_t s) {
   if (!m) return hipSuccess;
   hipLaunchKernelGGL(scatter_u32_kernel, dim3(fill_blocks(m)), dim3(256), 0, s, src, idx, m, dst);
-  return hipGetLastError();
-}
-hipError_t launch_arc_ranks(const uint64_t* arc, uint64_t m, const uint32_t* rank, uint32_t* arank, hipStream_t s) {
-  if (!m) return hipSuccess;
-  hipLaunchKernelGGL(arc_ranks_kernel, dim3(fill_blocks(m)), dim3(256), 0, s, arc, m, rank, arank);
   return hipGetLastError();
 }
 
