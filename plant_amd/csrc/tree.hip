@@ -145,7 +145,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
   constexpr int kWaves = THREADS / 64;
   __shared__ uint32_t s_key[PRUNE ? kLdsHash : 1];
   __shared__ uint32_t s_val[PRUNE ? kLdsHash : 1];
-  __shared__ uint32_t s_ri, s_fsize, s_nsize, s_nvis, s_below, s_clean, s_minnext, s_emit, s_cursor;
+  __shared__ uint32_t s_ri, s_fsize, s_nsize, s_nvis, s_below, s_clean, s_minnext, s_emit;
   __shared__ unsigned long long s_base;
   __shared__ unsigned long long s_ctr[4];
 
@@ -340,19 +340,24 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
     }
 
     // Emit (root rank, d) at every touched vertex whose final low is the root's
-    // rank, then restore the slot's scratch to its unreached state.
+    // rank. One pass reads each touched cell once, keeps the label's (rank, d) in
+    // the two frontier lists (free once the search has ended) and restores the
+    // cell to its unreached state with its rank; then the root's output range is
+    // reserved and the kept labels are written out, coalesced.
     const uint32_t nvis = s_nvis;
-    if (tid == 0) {
-      s_emit = 0;
-      s_cursor = 0;
-    }
+    if (tid == 0) s_emit = 0;
     __syncthreads();
-    uint32_t mine = 0;
     for (uint32_t k = tid; k < nvis; k += kThreads) {
       const uint32_t v = vis[k];
-      mine += ((uint32_t)cell[2 * (size_t)v] == rr && rank_of(cell, v) >= rr) ? 1u : 0u;
+      const ulonglong2 c = load_cell(cell, v);
+      const uint32_t rv = (uint32_t)(c.y >> 32);
+      if ((uint32_t)c.x == rr && rv >= rr) {
+        const uint32_t j = atomicAdd(&s_emit, 1u);
+        F[j] = rv;
+        N[j] = (uint32_t)(c.x >> 32);
+      }
+      *reinterpret_cast<ulonglong2*>(cell + 2 * (size_t)v) = make_ulonglong2(kStateInf, (uint64_t)rv << 32);
     }
-    if (mine) atomicAdd(&s_emit, mine);
     __syncthreads();
     const uint32_t emit = s_emit;
     if (tid == 0) {
@@ -373,18 +378,11 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
     }
     __syncthreads();
     const unsigned long long obase = s_base;
-    for (uint32_t k = tid; k < nvis; k += kThreads) {
-      const uint32_t v = vis[k];
-      const uint64_t x = cell[2 * (size_t)v];
-      const uint32_t rv = rank_of(cell, v);
-      if ((uint32_t)x == rr && rv >= rr && obase != ~0ull) {
-        const unsigned long long o = obase + atomicAdd(&s_cursor, 1u);
-        p.out_key[o] = ((uint64_t)rv << 32) | rr;
-        p.out_ent[o] = ((uint64_t)rr << 32) | (x >> 32);
+    if (obase != ~0ull)
+      for (uint32_t k = tid; k < emit; k += kThreads) {
+        p.out_key[obase + k] = ((uint64_t)F[k] << 32) | rr;
+        p.out_ent[obase + k] = ((uint64_t)rr << 32) | N[k];
       }
-      cell[2 * (size_t)v] = kStateInf;
-      cell[2 * (size_t)v + 1] = (uint64_t)rv << 32;  // the rank stays
-    }
     // tree size = processed entries of this root
     if (c_proc) atomicAdd(&s_ctr[0], c_proc);
     __syncthreads();
