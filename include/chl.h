@@ -167,10 +167,20 @@ int chl_build(const chl_graph* g, const uint32_t* order, int64_t n, const chl_bu
               chl_labels** labels, chl_build_stats* stats);
 
 /* ---- labels ----------------------------------------------------------------- */
-/* Replaces Labeling::write (shared_memory/labeling.hpp:167-183), same text. */
+/* Replaces Labeling::write (shared_memory/labeling.hpp:167-183), same text;
+ * rows are formatted on all host threads. */
 int chl_labels_write(const chl_labels* l, const char* path);
-/* Replaces Labeling::read (labeling.hpp:186-208); uploads to the device. */
+/* Replaces Labeling::read (labeling.hpp:186-208), the same token rules (only
+ * whitespace after the last label); a one-row-per-line file is parsed on all
+ * host threads. The file is parsed before the device is touched, so a malformed
+ * file is CHL_E_FORMAT on any host; uploads to the device. */
 int chl_labels_read(const char* path, int device, chl_labels** l);
+/* Binary sidecar of a label handle: the device store as it is (rows, row map,
+ * shared sides), for fast save and load. No counterpart in the reference.
+ * chl_labels_load checks the header, the file size, offsets, hub order and the
+ * row map (CHL_E_FORMAT) before it touches the device. */
+int chl_labels_save(const chl_labels* l, const char* path);
+int chl_labels_load(const char* path, int device, chl_labels** l);
 /* Labels from host CSR arrays (rows must be sorted by hub). */
 int chl_labels_from_csr(int64_t n, const uint64_t* off0, const uint32_t* hub0, const int32_t* d0,
                         const uint64_t* off1, const uint32_t* hub1, const int32_t* d1, int device,

@@ -95,6 +95,8 @@ def lib():
             "chl_build": ([vp, vp, i64, C.POINTER(BuildOpts), pp, C.POINTER(BuildStats)], C.c_int),
             "chl_labels_write": ([vp, C.c_char_p], C.c_int),
             "chl_labels_read": ([C.c_char_p, C.c_int, pp], C.c_int),
+            "chl_labels_save": ([vp, C.c_char_p], C.c_int),
+            "chl_labels_load": ([C.c_char_p, C.c_int, pp], C.c_int),
             "chl_labels_from_csr": ([i64, vp, vp, vp, vp, vp, vp, C.c_int, pp], C.c_int),
             "chl_labels_n": ([vp], i64),
             "chl_labels_total": ([vp, C.c_int], i64),
@@ -307,6 +309,16 @@ class Labels:
     def read(cls, path, device=0):
         h = C.c_void_p()
         _check(lib().chl_labels_read(os.fsencode(path), device, C.byref(h)))
+        return cls(h)
+
+    def save(self, path):
+        """Binary sidecar: the device store as it is."""
+        _check(lib().chl_labels_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path, device=0):
+        h = C.c_void_p()
+        _check(lib().chl_labels_load(os.fsencode(path), device, C.byref(h)))
         return cls(h)
 
     @classmethod

@@ -1,5 +1,8 @@
 // C ABI: label construction, label handles and distance queries.
+#include <sys/stat.h>
+
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -184,6 +187,110 @@ int chl_labels_read(const char* path, int device, chl_labels** out) {
   int rc = labels_read(H, path);
   if (rc) return set_error(rc, "cannot read labels from %s", path);
   return from_host_labels(H, device, out);
+}
+
+// ---- binary sidecar ----------------------------------------------------------
+// Little-endian: "CHLLABv1", u64 n, u32 shared sides, u32 0, u64 total[2], then
+// per stored side off u64[n+1] and ent u64[total], then row_of u32[n] (vertex
+// -> row). The store is saved as it sits on the device, rows in row-id order.
+namespace {
+constexpr char kBinMagic[8] = {'C', 'H', 'L', 'L', 'A', 'B', 'v', '1'};
+struct BinHeader {
+  char magic[8];
+  uint64_t n;
+  uint32_t shared;
+  uint32_t zero;
+  uint64_t total[2];
+};
+static_assert(sizeof(BinHeader) == 40, "binary label header layout");
+
+bool write_all(FILE* f, const void* p, size_t bytes) { return bytes == 0 || fwrite(p, 1, bytes, f) == bytes; }
+bool read_all(FILE* f, void* p, size_t bytes) { return bytes == 0 || fread(p, 1, bytes, f) == bytes; }
+}  // namespace
+
+int chl_labels_save(const chl_labels* l, const char* path) {
+  if (!l || !path) return set_error(CHL_E_INVAL, "null argument");
+  const LabelStore& S = l->store;
+  BinHeader h;
+  memcpy(h.magic, kBinMagic, 8);
+  h.n = (uint64_t)S.n;
+  h.shared = S.shared_sides ? 1u : 0u;
+  h.zero = 0;
+  h.total[0] = S.side_total(0);
+  h.total[1] = S.shared_sides ? 0 : S.side_total(1);
+  FILE* f = fopen(path, "wb");
+  if (!f) return set_error(CHL_E_IO, "cannot open %s", path);
+  bool ok = write_all(f, &h, sizeof(h));
+  for (int s = 0; s < (S.shared_sides ? 1 : 2) && ok; ++s) {
+    std::vector<uint64_t> off, ent;
+    int rc = S.download(s, off, ent);
+    if (rc) {
+      fclose(f);
+      return rc;
+    }
+    ok = write_all(f, off.data(), sizeof(uint64_t) * off.size()) && write_all(f, ent.data(), sizeof(uint64_t) * ent.size());
+  }
+  ok = ok && write_all(f, l->row_of.data(), sizeof(uint32_t) * l->row_of.size());
+  if (fclose(f) != 0) ok = false;
+  return ok ? CHL_OK : set_error(CHL_E_IO, "cannot write labels to %s", path);
+}
+
+int chl_labels_load(const char* path, int device, chl_labels** out) {
+  if (!path || !out) return set_error(CHL_E_INVAL, "null argument");
+  *out = nullptr;
+  FILE* f = fopen(path, "rb");
+  if (!f) return set_error(CHL_E_IO, "cannot open %s", path);
+  struct Closer {
+    FILE* f;
+    ~Closer() { fclose(f); }
+  } closer{f};
+  BinHeader h;
+  if (!read_all(f, &h, sizeof(h)) || memcmp(h.magic, kBinMagic, 8) != 0 || h.shared > 1 || h.zero != 0 ||
+      h.n >= 0xFFFFFFFFull || (h.shared && h.total[1] != 0))
+    return set_error(CHL_E_FORMAT, "%s is not a binary label file", path);
+  const int sides = h.shared ? 1 : 2;
+  // the file size must be exactly what the header announces
+  unsigned long long want = sizeof(h) + 4ull * h.n;
+  for (int s = 0; s < sides; ++s) want += 8ull * (h.n + 1) + 8ull * h.total[s];
+  struct stat st;
+  if (fstat(fileno(f), &st) != 0 || (unsigned long long)st.st_size != want)
+    return set_error(CHL_E_FORMAT, "%s: size does not match its header", path);
+  const int64_t n = (int64_t)h.n;
+  std::vector<uint64_t> off[2], ent[2];
+  for (int s = 0; s < sides; ++s) {
+    off[s].resize(n + 1);
+    ent[s].resize(h.total[s]);
+    if (!read_all(f, off[s].data(), 8 * off[s].size()) || !read_all(f, ent[s].data(), 8 * ent[s].size()))
+      return set_error(CHL_E_IO, "cannot read %s", path);
+    if (off[s][0] != 0 || off[s][n] != h.total[s])
+      return set_error(CHL_E_FORMAT, "%s: row offsets do not span the entries", path);
+    for (int64_t v = 0; v < n; ++v)
+      if (off[s][v + 1] < off[s][v]) return set_error(CHL_E_FORMAT, "%s: row offsets decrease", path);
+    if (!rows_sorted(off[s], ent[s])) return set_error(CHL_E_FORMAT, "%s: label rows must be hub-ascending", path);
+  }
+  std::vector<uint32_t> row_of(n);
+  if (!read_all(f, row_of.data(), 4 * row_of.size())) return set_error(CHL_E_IO, "cannot read %s", path);
+  {
+    std::vector<uint8_t> seen(n, 0);
+    for (uint32_t r : row_of) {
+      if ((int64_t)r >= n || seen[r]) return set_error(CHL_E_FORMAT, "%s: the row map is not a permutation", path);
+      seen[r] = 1;
+    }
+  }
+  int rc = ensure_device(device);
+  if (rc) return rc;
+  chl_labels* l = new chl_labels();
+  l->store.n = n;
+  l->store.device = device;
+  l->store.shared_sides = h.shared != 0;
+  for (int s = 0; s < sides && rc == CHL_OK; ++s) rc = upload_side(l->store, s, off[s], ent[s]);
+  if (rc == CHL_OK) rc = make_handle(l, row_of);
+  if (rc) {
+    delete l;
+    return rc;
+  }
+  *out = l;
+  return CHL_OK;
 }
 
 int chl_labels_from_csr(int64_t n, const uint64_t* off0, const uint32_t* hub0, const int32_t* d0, const uint64_t* off1,

@@ -281,48 +281,6 @@ int order_write(const char* path, const std::vector<uint32_t>& order) {
   return fclose(f) == 0 ? CHL_OK : CHL_E_IO;
 }
 
-// ---- label files (labeling.hpp:167-208) ------------------------------------
-
-int labels_write(const Labels& L, const char* path) {
-  FILE* f = fopen(path, "w");
-  if (!f) return CHL_E_IO;
-  std::vector<char> big(1 << 22);
-  setvbuf(f, big.data(), _IOFBF, big.size());
-  fprintf(f, "%lld\n", (long long)L.n);
-  char line[64];
-  for (int64_t v = 0; v < L.n; ++v)
-    for (int s = 0; s < 2; ++s) {
-      fprintf(f, "%llu", (unsigned long long)(L.off[s][v + 1] - L.off[s][v]));
-      for (uint64_t k = L.off[s][v]; k < L.off[s][v + 1]; ++k) {
-        uint64_t e = L.ent[s][k];
-        int len = snprintf(line, sizeof(line), " %u %d", (uint32_t)(e >> 32), (int32_t)(uint32_t)e);
-        fwrite(line, 1, len, f);
-      }
-      fputc('\n', f);
-    }
-  return fclose(f) == 0 ? CHL_OK : CHL_E_IO;
-}
-
-int labels_read(Labels& L, const char* path) {
-  FILE* f = fopen(path, "r");
-  if (!f) return CHL_E_IO;
-  long long n;
-  if (fscanf(f, "%lld", &n) != 1 || n < 0) { fclose(f); return CHL_E_FORMAT; }
-  L.n = n;
-  for (int s = 0; s < 2; ++s) { L.off[s].assign(n + 1, 0); L.ent[s].clear(); }
-  for (long long v = 0; v < n; ++v)
-    for (int s = 0; s < 2; ++s) {
-      long long k;
-      if (fscanf(f, "%lld", &k) != 1 || k < 0) { fclose(f); return CHL_E_FORMAT; }
-      for (long long i = 0; i < k; ++i) {
-        long long h, d;
-        if (fscanf(f, "%lld %lld", &h, &d) != 2) { fclose(f); return CHL_E_FORMAT; }
-        L.ent[s].push_back(((uint64_t)(uint32_t)h << 32) | (uint32_t)(int32_t)d);
-      }
-      L.off[s][v + 1] = L.ent[s].size();
-    }
-  fclose(f);
-  return CHL_OK;
-}
+// Label files (labeling.hpp:167-208): label_io.cpp.
 
 }  // namespace chl

@@ -184,6 +184,67 @@ def test_betweenness_order_pinned(chl):
         assert np.array_equal(order, want_order)
 
 
+def _read_status(chl, path, reader):
+    """0 when the file parses (the labels then reach the device, or the call stops
+    at the missing device on a host without one); else the error code."""
+    try:
+        reader(str(path))
+        return 0
+    except chl.ChlError as e:
+        return 0 if e.code == -7 else e.code
+
+
+LABEL_TEXT = "3\n2 0 0 1 4\n2 0 0 1 4\n1 1 0\n1 1 0\n2 0 7 2 0\n2 0 7 2 0\n"
+
+
+def test_label_text_reader_rules(chl, tmp_path):
+    # labeling.hpp:185-204 reads whitespace-separated tokens in any layout and
+    # accepts only whitespace after the last label
+    cases = {
+        "rows": (LABEL_TEXT, 0),
+        "one_line": (LABEL_TEXT.replace("\n", " "), 0),
+        "split_rows": (LABEL_TEXT.replace(" 4\n", "\n4\n").replace("2 0 7", "2\n0\t7"), 0),
+        "blank_lines": (LABEL_TEXT.replace("\n", "\n\n"), 0),
+        "crlf": (LABEL_TEXT.replace("\n", "\r\n"), 0),
+        "no_final_newline": (LABEL_TEXT.rstrip("\n"), 0),
+        "trailing_blank": (LABEL_TEXT + "  \n\n", 0),
+        "trailing_garbage": (LABEL_TEXT + "x\n", -3),
+        "extra_token": (LABEL_TEXT + "5\n", -3),
+        "negative_count": (LABEL_TEXT.replace("1 1 0\n1 1 0", "-1 1 0\n1 1 0"), -3),
+        "truncated": (LABEL_TEXT[:-6], -3),
+        "letter": (LABEL_TEXT.replace("0 1 4", "0 a 4", 1), -3),
+        "empty": ("", -3),
+    }
+    for name, (text, want) in cases.items():
+        p = tmp_path / f"{name}.txt"
+        p.write_bytes(text.encode())
+        assert _read_status(chl, p, chl.Labels.read) == want, name
+
+
+def test_label_binary_checks(chl, tmp_path):
+    # a well-formed sidecar (shared sides) and damaged copies of it
+    n = 3
+    off = np.array([0, 2, 3, 5], np.uint64)
+    ent = np.array([(0 << 32) | 0, (1 << 32) | 4, (1 << 32) | 0, (0 << 32) | 7, (2 << 32) | 0], np.uint64)
+    row_of = np.array([0, 1, 2], np.uint32)
+    head = b"CHLLABv1" + np.array([n], np.uint64).tobytes() + np.array([1, 0], np.uint32).tobytes() + \
+        np.array([5, 0], np.uint64).tobytes()
+    good = head + off.tobytes() + ent.tobytes() + row_of.tobytes()
+    bad = {
+        "good": (good, 0),
+        "magic": (b"CHLLABv2" + good[8:], -3),
+        "short": (good[:-4], -3),
+        "long": (good + b"\0", -3),
+        "offsets": (head + np.array([0, 2, 1, 5], np.uint64).tobytes() + ent.tobytes() + row_of.tobytes(), -3),
+        "hub_order": (head + off.tobytes() + ent[[1, 0, 2, 3, 4]].tobytes() + row_of.tobytes(), -3),
+        "row_map": (head + off.tobytes() + ent.tobytes() + np.array([0, 0, 2], np.uint32).tobytes(), -3),
+    }
+    for name, (data, want) in bad.items():
+        p = tmp_path / f"{name}.bin"
+        p.write_bytes(data)
+        assert _read_status(chl, p, chl.Labels.load) == want, name
+
+
 def test_library_exports_header(chl):
     lib = chl.lib()
     names = chl.header_symbols()
