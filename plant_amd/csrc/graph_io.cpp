@@ -7,7 +7,13 @@
 // arcs keep their minimum length (graph.hpp:107-114). Where the reference would
 // abort later on a non-positive length (assert at shared_memory/dijkstra.hpp:91)
 // the engine rejects the graph at load time with CHL_E_WEIGHT.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cctype>
 #include <cstdio>
 #include <cstdlib>
@@ -27,7 +33,9 @@ int host_threads() {
   return h > 16 ? 16 : (int)h;
 }
 
-// One range: rows of (head, min length) sorted by head.
+// One range: rows of (head, min length) sorted by head. The scatter into rows is
+// sequential (a row's order before sorting does not matter); sorting, the
+// per-row deduplication and the final copy run on all host threads.
 static void build_range(int64_t n, std::vector<Arc3>& a, bool by_tail, std::vector<uint64_t>& off,
                         std::vector<uint32_t>& head, std::vector<int32_t>& len) {
   // key (row, col, w) with row = tail for out-arcs, head for in-arcs
@@ -40,21 +48,32 @@ static void build_range(int64_t n, std::vector<Arc3>& a, bool by_tail, std::vect
     uint32_t r = by_tail ? x.u : x.v, c = by_tail ? x.v : x.u;
     tmp[pos[r]++] = {c, x.w};
   }
-  off.assign(n + 1, 0);
-  head.clear();
-  len.clear();
-  head.reserve(a.size());
-  len.reserve(a.size());
-  for (int64_t v = 0; v < n; ++v) {
+  const int T = host_threads();
+  // sort each row; count its distinct heads (the first of equal heads has the
+  // minimum length)
+  std::vector<uint64_t> keep(n + 1, 0);
+  parallel_for(n, T, 4096, [&](int64_t v) {
     auto b = tmp.begin() + cnt[v], e = tmp.begin() + cnt[v + 1];
     std::sort(b, e);
-    for (auto it = b; it != e; ++it) {
-      if (it != b && it->first == (it - 1)->first) continue;  // keep the minimum length
-      head.push_back(it->first);
-      len.push_back(it->second);
-    }
-    off[v + 1] = head.size();
-  }
+    uint64_t k = 0;
+    for (auto it = b; it != e; ++it)
+      if (it == b || it->first != (it - 1)->first) ++k;
+    keep[v + 1] = k;
+  });
+  off.assign(n + 1, 0);
+  for (int64_t v = 0; v < n; ++v) off[v + 1] = off[v] + keep[v + 1];
+  head.resize(off[n]);
+  len.resize(off[n]);
+  parallel_for(n, T, 4096, [&](int64_t v) {
+    auto b = tmp.begin() + cnt[v], e = tmp.begin() + cnt[v + 1];
+    uint64_t o = off[v];
+    for (auto it = b; it != e; ++it)
+      if (it == b || it->first != (it - 1)->first) {
+        head[o] = it->first;
+        len[o] = it->second;
+        ++o;
+      }
+  });
 }
 
 int graph_from_arcs(Graph& g, int64_t n, std::vector<Arc3>& arcs, bool undirected) {
@@ -79,6 +98,100 @@ int graph_from_arcs(Graph& g, int64_t n, std::vector<Arc3>& arcs, bool undirecte
 }
 
 // ---- readers ---------------------------------------------------------------
+
+namespace {
+inline bool blank_char(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+// A decimal token as sscanf's %lld takes it (leading blanks, optional sign);
+// false for anything this fast path leaves to sscanf (more than 18 digits).
+inline bool scan_ll(const char*& p, const char* e, long long& out) {
+  while (p < e && blank_char(*p)) ++p;
+  bool neg = false;
+  if (p < e && (*p == '-' || *p == '+')) neg = *p++ == '-';
+  if (p >= e || *p < '0' || *p > '9') return false;
+  long long x = 0;
+  int d = 0;
+  while (p < e && *p >= '0' && *p <= '9') {
+    x = x * 10 + (*p++ - '0');
+    if (++d > 18) return false;
+  }
+  out = neg ? -x : x;
+  return true;
+}
+}  // namespace
+
+// DIMACS on all host threads, for files in the plain layout: every line ends in
+// a newline, no NUL byte, comment lines first and one 'p sp n m' line before any
+// arc, then 'a u v w' and comment lines, non-comment lines under 512 bytes.
+// Returns 1 with the same arcs, error and n as parse_dimacs, or 0 when the file
+// is anything else: parse_dimacs then decides (accept or reject) as the
+// reference reader does.
+static int parse_dimacs_fast(const char* data, size_t size, int64_t& n, std::vector<Arc3>& arcs, int& err) {
+  if (size == 0 || data[size - 1] != '\n' || memchr(data, 0, size)) return 0;
+  const char* e = data + size;
+  const char* q = data;
+  // header: comment lines, then the one 'p' line
+  for (;;) {
+    if (q >= e) return 0;
+    const char* nl = (const char*)memchr(q, '\n', (size_t)(e - q));
+    if (*q == 'c') {
+      q = nl + 1;
+      continue;
+    }
+    if (*q != 'p' || nl - q + 1 > 511) return 0;
+    char buf[512];
+    memcpy(buf, q, (size_t)(nl - q + 1));
+    buf[nl - q + 1] = 0;
+    long long nn, mm;
+    if (sscanf(buf, "p sp %lld %lld", &nn, &mm) != 2) return 0;
+    n = nn;
+    q = nl + 1;
+    break;
+  }
+  const int T = host_threads();
+  std::vector<const char*> cut(T + 1, e);
+  cut[0] = q;
+  for (int t = 1; t < T; ++t) {
+    const char* c = q + (size_t)(e - q) * t / T;
+    if (c < cut[t - 1]) c = cut[t - 1];
+    const char* nl = c < e ? (const char*)memchr(c, '\n', (size_t)(e - c)) : nullptr;
+    cut[t] = nl ? nl + 1 : e;
+  }
+  std::vector<std::vector<Arc3>> part(T);
+  std::vector<int> perr(T, CHL_OK);
+  std::atomic<bool> plain(true);
+  const long long nmax = n;
+  parallel_for(T, T, 1, [&](int64_t t) {
+    std::vector<Arc3>& out = part[t];
+    for (const char* p = cut[t]; p < cut[t + 1] && plain.load(std::memory_order_relaxed);) {
+      const char* nl = (const char*)memchr(p, '\n', (size_t)(cut[t + 1] - p));
+      const char* le = nl;
+      if (*p == 'c') {
+        p = nl + 1;
+        continue;
+      }
+      long long u, v, w;
+      const char* r = p + 1;
+      if (*p != 'a' || nl - p + 1 > 511 || !scan_ll(r, le, u) || !scan_ll(r, le, v) || !scan_ll(r, le, w) ||
+          u < 1 || v < 1 || u > nmax || v > nmax) {
+        plain = false;
+        break;
+      }
+      if (w <= 0 || w > 2147483647LL) perr[t] = CHL_E_WEIGHT;
+      out.push_back({(uint32_t)(u - 1), (uint32_t)(v - 1), (int32_t)w});
+      p = nl + 1;
+    }
+  });
+  if (!plain) return 0;
+  size_t total = 0;
+  for (auto& v : part) total += v.size();
+  arcs.clear();
+  arcs.reserve(total);
+  for (int t = 0; t < T; ++t) {
+    arcs.insert(arcs.end(), part[t].begin(), part[t].end());
+    if (perr[t] != CHL_OK) err = perr[t];
+  }
+  return 1;
+}
 
 static bool parse_dimacs(FILE* f, int64_t& n, std::vector<Arc3>& arcs, int& err) {
   char buf[512];
@@ -219,6 +332,29 @@ static bool parse_edgelist(FILE* f, int64_t& n, std::vector<Arc3>& arcs, int& er
 }
 
 int graph_read(Graph& g, const char* path, bool undirected) {
+  {
+    // DIMACS in the plain layout, on all host threads
+    int fd = open(path, O_RDONLY);
+    if (fd < 0) return CHL_E_IO;
+    struct stat st;
+    if (fstat(fd, &st) == 0 && st.st_size > 0) {
+      void* m = mmap(nullptr, (size_t)st.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+      if (m != MAP_FAILED) {
+        int64_t n = 0;
+        int err = CHL_OK;
+        std::vector<Arc3> arcs;
+        const int ok = parse_dimacs_fast((const char*)m, (size_t)st.st_size, n, arcs, err);
+        munmap(m, (size_t)st.st_size);
+        if (ok) {
+          close(fd);
+          if (err != CHL_OK) return err;
+          int r = graph_from_arcs(g, n, arcs, undirected);
+          return r == CHL_OK ? 1 : r;
+        }
+      }
+    }
+    close(fd);
+  }
   FILE* f = fopen(path, "r");
   if (!f) return CHL_E_IO;
   for (int fmt = 1; fmt <= 4; ++fmt) {

@@ -56,6 +56,46 @@ def test_unterminated_dimacs_rejected(chl, orc, tmp_path):
     assert og is None and r < 0
 
 
+DIMACS_LAYOUTS = {
+    # the parallel reader's plain layout, and every way out of it, against the oracle
+    "plain": "c x\np sp 3 3\na 1 2 4\na 2 3 5\nc y\na 3 1 6\n",
+    "trailing_junk": "p sp 3 2\na 1 2 4 junk\na 2 3 5\n",
+    "tabs_crlf": "p sp 3 2\r\na\t1\t2\t4\r\na 2 3 5\r\n",
+    "no_space_after_a": "p sp 3 2\na1 2 4\na 2 3 5\n",
+    "long_comment": "c " + "x" * 900 + "\np sp 3 2\na 1 2 4\nc " + "y" * 700 + "\na 2 3 5\n",
+    "long_arc_line": "p sp 3 2\na 1 2 4" + " " * 600 + "\na 2 3 5\n",
+    "line_511": "p sp 3 2\na 1 2 4" + " " * (511 - 8 - 1) + "\na 2 3 5\n",
+    "line_512": "p sp 3 2\na 1 2 4" + " " * (512 - 8 - 1) + "\na 2 3 5\n",
+    "blank_line": "p sp 3 2\na 1 2 4\n\na 2 3 5\n",
+    "second_p": "p sp 3 2\na 1 2 4\np sp 3 2\na 2 3 5\n",
+    "arc_before_p": "a 1 2 4\np sp 3 2\na 2 3 5\n",
+    "missing_length": "p sp 3 2\na 1 2\na 2 3 5\n",
+    "out_of_range": "p sp 3 2\na 1 4 4\na 2 3 5\n",
+    "signed": "p sp 3 2\na +1 2 4\na 2 3 +5\n",
+    "huge_number": "p sp 3 2\na 1 2 12345678901234567890\na 2 3 5\n",
+}
+
+
+@pytest.mark.parametrize("name", sorted(DIMACS_LAYOUTS))
+def test_dimacs_layouts_match_oracle(chl, orc, tmp_path, name):
+    path = write(tmp_path, name, DIMACS_LAYOUTS[name])
+    og, r = orc.OGraph.read(path)
+    try:
+        g = chl.Graph.load(path)
+    except chl.ChlError as e:
+        if e.code == -4:
+            # a length the reference keeps as a non-positive int and asserts on in
+            # Dijkstra (dijkstra.hpp:91) is rejected at load time
+            assert og is not None and min(og.csr(1)[2].min(), og.csr(0)[2].min()) <= 0, name
+        else:
+            assert og is None, name
+        return
+    assert og is not None and g.n == og.n, name
+    for rng in (0, 1):
+        for a, b in zip(g.csr(rng), og.csr(rng)):
+            assert np.array_equal(a, b), name
+
+
 def test_zero_weight_rejected(chl, tmp_path):
     with pytest.raises(chl.ChlError) as e:
         chl.Graph.load(write(tmp_path, "z", "p sp 2 1\na 1 2 0\n"))
