@@ -150,11 +150,12 @@ struct TreeParams {
   uint32_t R;
   uint32_t delta;
   // per-slot scratch, slot = workgroup index: cell holds 2 n words per slot
-  // (state word, stamp), the lists n entries per slot
+  // (state word, stamp and rank), the lists and the pile n entries per slot
   uint64_t* cell;
   uint32_t* qa;
   uint32_t* qb;
   uint32_t* vis;
+  uint64_t* qd;     // deferred pile, n entries per slot
   uint32_t* gkey;   // global root-label hash per slot (gcap entries) when the row is too long for LDS
   uint32_t* gval;
   uint32_t gcap;

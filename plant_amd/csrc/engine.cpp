@@ -211,6 +211,7 @@ struct Builder {
 
   DBuf<uint64_t> cell;
   DBuf<uint32_t> qa, qb, vis, gkey, gval, grank;
+  DBuf<uint64_t> qd;  // deferred piles
   std::vector<uint32_t> id_of_rank;  // rank -> device id
   std::vector<uint32_t> hids;        // staging of root ids
   uint32_t slots = 0, gcap = 0;
@@ -266,7 +267,7 @@ struct Builder {
   int alloc_slots(uint32_t want) {
     size_t freeb = 0, totalb = 0;
     CHL_HIP(hipMemGetInfo(&freeb, &totalb));
-    const size_t per = (size_t)n * 28;  // 16-byte cell + three u32 lists per vertex
+    const size_t per = (size_t)n * 36;  // 16-byte cell, three u32 lists and a u64 pile per vertex
     // a third of the free memory, at most a quarter of the device: on large
     // graphs the label store (twice, while a batch is merged) needs the rest
     size_t budget = std::min(freeb / 3, totalb / 4);
@@ -277,6 +278,7 @@ struct Builder {
     CHL_HIP(qa.reserve(cells));
     CHL_HIP(qb.reserve(cells));
     CHL_HIP(vis.reserve(cells));
+    CHL_HIP(qd.reserve(cells));
     CHL_HIP(launch_init_cells(cell.p, cells, grank.p, (uint32_t)n, stream));
     return CHL_OK;
   }
@@ -375,6 +377,7 @@ struct Builder {
     p.qa = qa.p;
     p.qb = qb.p;
     p.vis = vis.p;
+    p.qd = qd.p;
     p.gkey = gkey.p;
     p.gval = gval.p;
     p.gcap = gcap;

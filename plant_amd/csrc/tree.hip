@@ -145,7 +145,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
   constexpr int kWaves = THREADS / 64;
   __shared__ uint32_t s_key[PRUNE ? kLdsHash : 1];
   __shared__ uint32_t s_val[PRUNE ? kLdsHash : 1];
-  __shared__ uint32_t s_ri, s_fsize, s_nsize, s_nvis, s_below, s_clean, s_minnext, s_emit;
+  __shared__ uint32_t s_ri, s_fsize, s_nsize, s_nvis, s_below, s_clean, s_minnext, s_emit, s_dsize, s_dmin;
   __shared__ unsigned long long s_base;
   __shared__ unsigned long long s_ctr[4];
 
@@ -156,6 +156,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
   const size_t slot = blockIdx.x;
   uint64_t* cell = p.cell + slot * 2 * n;
   uint32_t* vis = p.vis + slot * n;
+  uint64_t* D = p.qd + slot * n;  // deferred pile: (vertex << 32) | stamp at deferral
 
   unsigned long long c_proc = 0, c_arcs = 0, c_prune = 0, c_rounds = 0;
   if (tid < 4) s_ctr[tid] = 0;
@@ -211,6 +212,8 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
       s_below = 1;
       s_clean = 1;
       s_minnext = 0;
+      s_dsize = 0;
+      s_dmin = kEmpty;
     }
     uint32_t B = p.delta;
     uint32_t round = 1;
@@ -218,34 +221,83 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
 
     for (;;) {
       const uint32_t fsize = s_fsize;
-      if (fsize == 0) break;
-      if (s_below == 0) {
+      const uint32_t dsize = min(s_dsize, (uint32_t)n);
+      if (fsize == 0 && dsize == 0) break;
+      // a bucket advance (nothing queued below B) also scans the deferred pile
+      const bool adv = s_below == 0;
+      if (adv) {
         // bucket [.., B) is final; every queued vertex is at >= B
         if (FLAGS && s_clean == 0) break;
-        B = s_minnext + p.delta;
+        B = min(s_minnext, s_dmin) + p.delta;
       }
+      const uint32_t dscan = adv ? dsize : 0u;
       __syncthreads();
       if (tid == 0) {
         s_nsize = 0;
         s_below = 0;
         s_clean = 0;
         s_minnext = kEmpty;
+        if (adv) {
+          s_dsize = 0;
+          s_dmin = kEmpty;
+        }
       }
       __syncthreads();
-      const uint32_t nstamp = round + 1;
+      // stamps of this round: queued for the next round, or put on the pile
+      // (distinct, so that an improvement of a vertex put on the pile in the same
+      // round still queues it for the next round)
+      const uint32_t nstamp = 2 * round;
+      const uint32_t pstamp = 2 * round + 1;
       Book bk;
+      uint32_t dmin = kEmpty;
 
       if (!PRUNE) {
-        // one lane per frontier entry
-        for (uint32_t k = tid; k < fsize; k += kThreads) {
-          const uint32_t u = F[k];
-          const uint64_t x = cell[2 * (size_t)u];
+        // one lane per entry of the frontier and, on a bucket advance, of the
+        // deferred pile. Entries at or beyond B wait in the pile until the next
+        // advance instead of being carried from round to round; a pile entry is
+        // live only while the vertex's stamp is the one it was deferred with
+        // (a later queueing supersedes it).
+        for (uint32_t k = tid; k < fsize + dscan; k += kThreads) {
+          uint32_t u;
+          ulonglong2 c;
+          if (k < fsize) {
+            u = F[k];
+            c = load_cell(cell, u);
+          } else {
+            const uint64_t e = D[k - fsize];
+            u = (uint32_t)(e >> 32);
+            c = load_cell(cell, u);
+            if ((uint32_t)c.y != (uint32_t)e) continue;
+          }
+          const uint64_t x = c.x;
           const uint32_t ab = p.off[u], ae = p.off[u + 1];  // read beside the state
           const uint32_t d = (uint32_t)(x >> 32);
           const uint32_t lo = (uint32_t)x;
           if (d >= B) {
-            bk.note(d, B);
-            if (atomicExch(stamp_of(cell, u), nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = u;
+            uint32_t* st = stamp_of(cell, u);
+            bool to_next = adv;
+            if (!adv) {
+              // onto the pile unless queued already in this round
+              uint32_t o = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              while (o != nstamp && o != pstamp) {
+                const uint32_t prev = atomicCAS(st, o, pstamp);
+                if (prev == o) {
+                  const uint32_t j = atomicAdd(&s_dsize, 1u);
+                  if (j < n) {
+                    D[j] = ((uint64_t)u << 32) | pstamp;
+                    dmin = min(dmin, d);
+                  } else {
+                    to_next = true;  // the pile is full: the next frontier takes it
+                  }
+                  break;
+                }
+                o = prev;
+              }
+            }
+            if (to_next) {
+              bk.note(d, B);
+              if (atomicExch(st, nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = u;
+            }
             continue;
           }
           if (!FLAGS && rank_of(cell, u) < skip) continue;
@@ -315,18 +367,28 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
       const bool any_below = __any(bk.below != 0);
       uint32_t mn = bk.mn;
       for (int off = 32; off > 0; off >>= 1) mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
+      for (int off = 32; off > 0; off >>= 1) dmin = min(dmin, (uint32_t)__shfl_xor((int)dmin, off));
       if (lane == 0) {
         if (any_below) s_below = 1;
         if (mn != kEmpty) atomicMin(&s_minnext, mn);
+        if (dmin != kEmpty) atomicMin(&s_dmin, dmin);
       }
       __syncthreads();
       const uint32_t nsize = s_nsize;
       if (FLAGS && s_below == 0) {
-        // bucket boundary: is some queued vertex still reached through the root
-        // alone? (stops at the first one found)
-        for (uint32_t k = tid; k < nsize; k += kThreads) {
+        // bucket boundary: is some queued vertex (next frontier or live pile
+        // entry) still reached through the root alone? (stops at the first one
+        // found)
+        const uint32_t pile = min(s_dsize, (uint32_t)n);
+        for (uint32_t k = tid; k < nsize + pile; k += kThreads) {
           if (s_clean) break;
-          if ((uint32_t)cell[2 * (size_t)N[k]] == rr) s_clean = 1;
+          if (k < nsize) {
+            if ((uint32_t)cell[2 * (size_t)N[k]] == rr) s_clean = 1;
+          } else {
+            const uint64_t e = D[k - nsize];
+            const ulonglong2 c = load_cell(cell, (uint32_t)(e >> 32));
+            if ((uint32_t)c.y == (uint32_t)e && (uint32_t)c.x == rr) s_clean = 1;
+          }
         }
         __syncthreads();
       }
