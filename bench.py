@@ -119,7 +119,7 @@ def serving(chl, L, g, rng, seconds):
     nreq = int(2e8 * 0.33 / 4096)
     replay(lib, srv, rng.integers(0, n, (nreq * 4096, 2), dtype=np.uint32), np.full(nreq, 4096),
            np.cumsum(rng.exponential(4096.0 / 2e8 * 1e6, nreq)))
-    for rate in (1.0e8, 2.0e8, 2.5e8, 3.0e8):
+    for rate in (1.0e8, 2.0e8, 2.2e8, 2.5e8, 3.0e8):
         nreq = int(rate * seconds / 4096)
         arrivals = np.cumsum(rng.exponential(4096.0 / rate * 1e6, nreq))
         pairs = rng.integers(0, n, (nreq * 4096, 2), dtype=np.uint32)
