@@ -230,6 +230,7 @@ struct Builder {
   SideBatch sb[2];
   uint64_t out_cap = 0;
   uint64_t next_need = 0;  // output estimate of the next batch (labels)
+  uint64_t out_limit = ~0ull;  // output capacity the memory cap allows for the current batch
   double tail_labels = -1;  // labels per root at the top ranks of the last batch
   hipEvent_t ev[8];
 
@@ -797,6 +798,7 @@ struct Builder {
         const double per_root = tail_labels >= 0 ? std::max(1.0, 1.5 * tail_labels) : (double)n;
         const double cap = (double)freeb / 96.0 / per_root / nsides;
         bw = (uint32_t)std::max<double>(1.0, std::min<double>(bw, cap));
+        out_limit = std::max<uint64_t>((uint64_t)1 << 20, (uint64_t)(freeb / 96 / nsides));
       }
       const uint32_t R = (uint32_t)std::min<int64_t>(bw, n - r0);
       const uint32_t prefix = (uint32_t)r0;
@@ -817,6 +819,10 @@ struct Builder {
         const uint64_t e = hooks->out_estimate(hooks->ctx, hroots.data(), R);
         if (e) out_cap = std::max<uint64_t>(e, (uint64_t)1 << 16);
       }
+      // the output buffer within the memory cap as well: an estimate made for a
+      // wider batch than the cap allowed would not fit (a batch that needs more
+      // spills and grows the buffer by what it measured)
+      out_cap = std::min(out_cap, out_limit);
       hids.resize(R);
       for (uint32_t k = 0; k < R; ++k) hids[k] = id_of_rank[hroots[k]];
       CHL_HIP(hipMemcpyAsync(roots.p, hids.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, stream));
