@@ -1,0 +1,83 @@
+// Test harness only: the tree kernel (plant_amd/csrc/tree.hip) run on the host
+// through hip/hip_runtime.h of this directory, one persistent workgroup taking
+// every root of one launch. Input and output are little-endian binary files:
+//   in:  u32 n, m, R, delta, threads, slots, flags; u32 off[n + 1]; u64 arc[m];
+//        u32 rank[n]; u32 roots[R] (vertex ids)
+//   out: u64 labels; (u64 key, u64 ent) per label; u32 1 when every search
+//        cell came back to its unreached state (rank kept), else 0
+#include "../../plant_amd/csrc/tree.hip"
+
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+using namespace chl;
+
+template <class T>
+static bool get(FILE* f, std::vector<T>& v, size_t count) {
+  v.resize(count);
+  return count == 0 || fread(v.data(), sizeof(T), count, f) == count;
+}
+
+int main(int argc, char** argv) {
+  if (argc != 3) return 2;
+  FILE* f = fopen(argv[1], "rb");
+  if (!f) return 2;
+  std::vector<uint32_t> hdr, off, rank, roots;
+  std::vector<uint64_t> arc;
+  if (!get(f, hdr, 7)) return 2;
+  const uint32_t n = hdr[0], m = hdr[1], R = hdr[2], delta = hdr[3], threads = hdr[4], S = hdr[5], flags = hdr[6];
+  if (!get(f, off, n + 1) || !get(f, arc, m) || !get(f, rank, n) || !get(f, roots, R)) return 2;
+  fclose(f);
+  const size_t cells = (size_t)S * n;
+  std::vector<uint64_t> cell(2 * cells), qd(cells);
+  for (size_t i = 0; i < cells; ++i) {
+    cell[2 * i] = kStateInf;
+    cell[2 * i + 1] = (uint64_t)rank[i % n] << 32;
+  }
+  std::vector<uint32_t> qa(cells), qb(cells), vis(cells), spill(R), ts(R), lc(R), gk(1), gv(1);
+  const uint64_t cap = (uint64_t)n * R + 1;
+  std::vector<uint64_t> ok(cap), oe(cap);
+  std::vector<unsigned long long> ctr(CTR_COUNT, 0);
+  ctr[CTR_SPILL_START] = ~0ull;
+  TreeParams p{};
+  p.off = off.data();
+  p.arc = arc.data();
+  p.rank = rank.data();
+  p.n = n;
+  p.roots = roots.data();
+  p.R = R;
+  p.delta = delta;
+  p.cell = cell.data();
+  p.qa = qa.data();
+  p.qb = qb.data();
+  p.vis = vis.data();
+  p.qd = qd.data();
+  p.gkey = gk.data();
+  p.gval = gv.data();
+  p.gcap = 1;
+  p.out_key = ok.data();
+  p.out_ent = oe.data();
+  p.out_cap = cap;
+  p.spill = spill.data();
+  p.tree_size = ts.data();
+  p.label_count = lc.data();
+  p.ctr = ctr.data();
+  emu_only_block = (int)S - 1;  // the last slot: the largest scratch offsets
+  if (const char* e = getenv("EMU_SEED")) emu_seed = strtoull(e, nullptr, 10);
+  launch_tree(p, flags, S, (int)threads, nullptr);
+  uint32_t restored = 1;
+  for (size_t i = 0; i < cells; ++i)
+    if (cell[2 * i] != kStateInf || cell[2 * i + 1] != ((uint64_t)rank[i % n] << 32)) restored = 0;
+  FILE* o = fopen(argv[2], "wb");
+  if (!o) return 2;
+  const uint64_t L = ctr[CTR_LABELS];
+  fwrite(&L, 8, 1, o);
+  for (uint64_t k = 0; k < L; ++k) {
+    fwrite(&ok[k], 8, 1, o);
+    fwrite(&oe[k], 8, 1, o);
+  }
+  fwrite(&restored, 4, 1, o);
+  fclose(o);
+  return ctr[CTR_OVERFLOW] ? 3 : 0;
+}
