@@ -114,43 +114,58 @@ struct Learned {
       update_budget(freeb);
     }
     // features by rank: log rank, log degree, log 2-hop degree sum, log score
+    // (on all host threads; sums in fixed chunks added in order, so the
+    // features do not depend on the thread schedule)
+    const int nt = host_threads();
     std::vector<double> f1(n), f2(n), f3(n, 0.0);
     std::vector<uint64_t> deg(n);
-    for (int64_t v = 0; v < n; ++v)
+    parallel_for(n, nt, 16384, [&](int64_t v) {
       deg[v] = (g.off[1][v + 1] - g.off[1][v]) + (g.off[0][v + 1] - g.off[0][v]);
-    parallel_for(n, host_threads(), 4096, [&](int64_t v) {
+    });
+    parallel_for(n, nt, 4096, [&](int64_t v) {
       uint64_t two = 0;
       for (uint64_t a = g.off[1][v]; a < g.off[1][v + 1]; ++a) two += deg[g.head[1][a]];
       f1[v] = std::log1p((double)deg[v]);
       f2[v] = std::log1p((double)two);
       if (o.vertex_score) f3[v] = std::log1p(std::max(0.0, (double)o.vertex_score[v]));
     });
+    constexpr int64_t kChunk = 65536;
+    const int64_t nchunks = (n + kChunk - 1) / kChunk;
+    auto chunk_sum = [&](auto term) {
+      std::vector<double> part(nchunks, 0.0);
+      parallel_for(nchunks, nt, 1, [&](int64_t c) {
+        double a = 0;
+        for (int64_t v = c * kChunk; v < std::min<int64_t>(n, (c + 1) * kChunk); ++v) a += term(v);
+        part[c] = a;
+      });
+      double total = 0;
+      for (double x : part) total += x;
+      return total;
+    };
     auto standardize = [&](std::vector<double>& f) {
-      double mu = 0, var = 0;
-      for (double x : f) mu += x;
-      mu /= (double)n;
-      for (double x : f) var += (x - mu) * (x - mu);
+      const double mu = chunk_sum([&](int64_t v) { return f[v]; }) / (double)n;
+      const double var = chunk_sum([&](int64_t v) { return (f[v] - mu) * (f[v] - mu); });
       const double sd = std::sqrt(var / (double)n);
-      for (double& x : f) x = sd > 0 ? (x - mu) / sd : 0.0;
+      parallel_for(n, nt, 16384, [&](int64_t v) { f[v] = sd > 0 ? (f[v] - mu) / sd : 0.0; });
     };
     standardize(f1);
     standardize(f2);
     standardize(f3);
     std::vector<float> hx((size_t)n * kMlpF, 0.0f);
     const double lr = std::log1p((double)n);
-    for (int64_t r = 0; r < n; ++r) {
+    parallel_for(n, nt, 16384, [&](int64_t r) {
       const uint32_t v = order[r];
       float* row = &hx[(size_t)r * kMlpF];
       row[0] = (float)(std::log1p((double)r) / lr);
       row[1] = (float)f1[v];
       row[2] = (float)f2[v];
       row[3] = (float)f3[v];
-    }
+    });
     CHL_HIP(X.reserve(hx.size()));
     CHL_HIP(hipMemcpy(X.p, hx.data(), sizeof(float) * hx.size(), hipMemcpyHostToDevice));
     hxs.swap(hx);
     lrank.resize(n);
-    for (int64_t r = 0; r < n; ++r) lrank[r] = (float)std::log1p((double)r);
+    parallel_for(n, host_threads(), 16384, [&](int64_t r) { lrank[r] = (float)std::log1p((double)r); });
     // parameters: uniform(+-sqrt(6/(fan_in+fan_out))) weights, zero biases
     std::vector<float> hp(kMlpParams, 0.0f);
     auto fill = [&](int off, int fin, int fout) {
@@ -433,41 +448,58 @@ struct Learned {
       }
       return CHL_OK;
     }
-    {
-      double sum = 0;
-      for (uint32_t k = 0; k < R; ++k) sum += ts[k];
-      if (sum > 0) ms_per_vertex = ms / sum;
-    }
+    // per-batch sums over the roots on all host threads (fixed chunks added in
+    // order: the same result on any schedule)
+    std::vector<double> y(R);
+    parallel_for(R, host_threads(), 16384, [&](int64_t k) { y[k] = std::log1p((double)ts[k]); });
+    auto sums = [&](auto term, int w, double* out) {
+      constexpr uint32_t kChunk = 65536;
+      const uint32_t nc = (R + kChunk - 1) / kChunk;
+      std::vector<double> part((size_t)nc * w, 0.0);
+      parallel_for(nc, host_threads(), 1, [&](int64_t c) {
+        for (uint32_t k = (uint32_t)c * kChunk; k < std::min<uint32_t>(R, ((uint32_t)c + 1) * kChunk); ++k)
+          term(k, &part[(size_t)c * w]);
+      });
+      for (int i = 0; i < w; ++i) out[i] = 0;
+      for (uint32_t c = 0; c < nc; ++c)
+        for (int i = 0; i < w; ++i) out[i] += part[(size_t)c * w + i];
+    };
+    double base[2];
+    sums([&](uint32_t k, double* a) { a[0] += ts[k]; a[1] += y[k]; }, 2, base);
+    if (base[0] > 0) ms_per_vertex = ms / base[0];
     // prediction error before learning from this batch
-    double mean_log = 0;
-    for (uint32_t k = 0; k < R; ++k) mean_log += std::log1p((double)ts[k]);
-    mean_log /= std::max<uint32_t>(R, 1);
+    const double mean_log = base[1] / std::max<uint32_t>(R, 1);
     if (trained && last_mean_log >= 0) {
       bool covered = true;
       for (uint32_t k = 0; k < R && covered; ++k)
         covered = roots[k] >= pred_r0 && roots[k] < pred_r0 + pred_cnt;
       if (covered) {
-        double pt = 0, pl = 0, mt = 0, ml = 0;
-        for (uint32_t k = 0; k < R; ++k) {
-          const double y = std::log1p((double)ts[k]);
-          mae_l += std::fabs(pred_tree(roots[k]) - y);
-          mae_s += std::fabs(last_mean_log - y);
-          pt += lin_tree(roots[k]);
-          pl += lin_labels(roots[k]);
-          mt += ts[k];
-          ml += lc[k];
-        }
+        double a[6];  // |error| learned, |error| static, predicted tree, predicted labels, measured both
+        const double lm = last_mean_log;
+        sums([&](uint32_t k, double* p) {
+          p[0] += std::fabs(pred_tree(roots[k]) - y[k]);
+          p[1] += std::fabs(lm - y[k]);
+          p[2] += lin_tree(roots[k]);
+          p[3] += lin_labels(roots[k]);
+          p[4] += ts[k];
+          p[5] += lc[k];
+        }, 6, a);
+        mae_l += a[0];
+        mae_s += a[1];
         mae_n += R;
-        calib_t = clamp_calib(mt / pt);
-        calib_l = clamp_calib(ml / pl);
+        calib_t = clamp_calib(a[4] / a[2]);
+        calib_l = clamp_calib(a[5] / a[3]);
       }
     }
     last_mean_log = mean_log;
-    // append (rank, targets) to the sample ring
-    for (uint32_t k = 0; k < R; ++k) {
+    // append (rank, targets) to the sample ring; training draws from the newest
+    // 16384 samples only, so a wider batch contributes an even stride of its roots
+    const uint32_t take = std::min<uint32_t>(R, 16384);
+    for (uint32_t i = 0; i < take; ++i) {
+      const uint32_t k = (uint32_t)((uint64_t)i * R / take);
       const uint64_t pos = tcount % tcap;
       hidx[pos] = roots[k];
-      hT[pos * kMlpO] = (float)std::log1p((double)ts[k]);
+      hT[pos * kMlpO] = (float)y[k];
       hT[pos * kMlpO + 1] = (float)std::log1p((double)lc[k]);
       ++tcount;
     }
