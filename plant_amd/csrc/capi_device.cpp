@@ -249,12 +249,16 @@ int chl_labels_load(const char* path, int device, chl_labels** out) {
       h.n >= 0xFFFFFFFFull || (h.shared && h.total[1] != 0))
     return set_error(CHL_E_FORMAT, "%s is not a binary label file", path);
   const int sides = h.shared ? 1 : 2;
-  // the file size must be exactly what the header announces
+  // the file size must be exactly what the header announces (counts checked
+  // against the size first, so that the sum cannot wrap)
+  struct stat st;
+  if (fstat(fileno(f), &st) != 0) return set_error(CHL_E_IO, "cannot stat %s", path);
+  const unsigned long long fsize = (unsigned long long)st.st_size;
+  if (h.n > fsize / 4 || h.total[0] > fsize / 8 || h.total[1] > fsize / 8)
+    return set_error(CHL_E_FORMAT, "%s: size does not match its header", path);
   unsigned long long want = sizeof(h) + 4ull * h.n;
   for (int s = 0; s < sides; ++s) want += 8ull * (h.n + 1) + 8ull * h.total[s];
-  struct stat st;
-  if (fstat(fileno(f), &st) != 0 || (unsigned long long)st.st_size != want)
-    return set_error(CHL_E_FORMAT, "%s: size does not match its header", path);
+  if (fsize != want) return set_error(CHL_E_FORMAT, "%s: size does not match its header", path);
   const int64_t n = (int64_t)h.n;
   std::vector<uint64_t> off[2], ent[2];
   for (int s = 0; s < sides; ++s) {

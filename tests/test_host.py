@@ -278,6 +278,10 @@ def test_label_binary_checks(chl, tmp_path):
         "offsets": (head + np.array([0, 2, 1, 5], np.uint64).tobytes() + ent.tobytes() + row_of.tobytes(), -3),
         "hub_order": (head + off.tobytes() + ent[[1, 0, 2, 3, 4]].tobytes() + row_of.tobytes(), -3),
         "row_map": (head + off.tobytes() + ent.tobytes() + np.array([0, 0, 2], np.uint32).tobytes(), -3),
+        # counts whose byte total wraps 2^64 to the file's size
+        "wrap": (b"CHLLABv1" + np.array([n], np.uint64).tobytes() + np.array([1, 0], np.uint32).tobytes() +
+                 np.array([(1 << 61) + 5, 0], np.uint64).tobytes() + off.tobytes() + ent.tobytes() +
+                 row_of.tobytes(), -3),
     }
     for name, (data, want) in bad.items():
         p = tmp_path / f"{name}.bin"
