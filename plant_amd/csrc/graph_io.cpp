@@ -210,7 +210,7 @@ static bool parse_dimacs(FILE* f, int64_t& n, std::vector<Arc3>& arcs, int& err)
       if (inited || sscanf(buf, "p sp %lld %lld", &nn, &mm) != 2) return false;
       inited = true;
       n = nn;
-      if (mm > 0) arcs.reserve((size_t)mm);
+      if (mm > 0) arcs.reserve((size_t)std::min<long long>(mm, 1 << 24));  // a hint, not trusted
     } else if (buf[0] == 'a') {
       long long u, v, w;
       if (sscanf(buf, "a %lld %lld %lld", &u, &v, &w) != 3) return false;

@@ -94,7 +94,8 @@ struct Mapped {
 int read_tokens(const char* p, const char* e, Labels& L) {
   long long n;
   while (p < e && is_ws(*p)) ++p;
-  if (!get_int(p, e, n) || n < 0) return CHL_E_FORMAT;
+  // every vertex has two rows of at least one token and a separator
+  if (!get_int(p, e, n) || n < 0 || n > (e - p) / 4 + 1) return CHL_E_FORMAT;
   L.n = n;
   for (int s = 0; s < 2; ++s) {
     L.off[s].assign(n + 1, 0);
@@ -273,7 +274,7 @@ int labels_read(Labels& L, const char* path) {
   const char* q = p;
   while (q < e && is_blank(*q)) ++q;
   long long n;
-  if (get_int(q, e, n) && n >= 0) {
+  if (get_int(q, e, n) && n >= 0 && n <= (long long)(m.size / 4) + 1) {
     while (q < e && is_blank(*q)) ++q;
     if (q < e && *q == '\n') {
       L.n = n;
