@@ -399,6 +399,11 @@ int order_read(const char* path, std::vector<uint32_t>& order) {
   if (!f) return CHL_E_IO;
   long long s;
   if (fscanf(f, "%lld", &s) != 1 || s < 0) { fclose(f); return CHL_E_FORMAT; }
+  {
+    // every id takes at least two bytes: a larger count cannot be in the file
+    struct stat st;
+    if (fstat(fileno(f), &st) != 0 || s > (long long)st.st_size / 2 + 1) { fclose(f); return CHL_E_FORMAT; }
+  }
   order.resize((size_t)s);
   for (long long i = 0; i < s; ++i) {
     long long x;
