@@ -107,7 +107,7 @@ def serving(chl, L, g, rng, seconds):
     lib = server_lib(chl)
     o = SOpts()
     lib.chl_server_opts_default(C.byref(o))
-    o.streams = 4  # the box's hardware queues; 8 streams gave a worse p99 at 2.5e8 pairs/s
+    o.streams = 4  # HIP's default hardware queues per process; 8 streams gave a worse p99 at 2.5e8 pairs/s
     o.max_inflight_pairs = 1 << 21
     o.max_batch_pairs = 1 << 16
     srv = C.c_void_p()

@@ -7,13 +7,14 @@
 // Device layout (all in HBM for the whole build):
 //   * graph in device ids, a breadth-first locality order kept with the graph
 //     handle across builds; per arc range an offset array u32[n+1] and packed
-//     arcs u64[A] = (head << 32) | length, plus this build's rank of every arc
-//     head u32[A] and of every device id u32[n]. A symmetric graph keeps one
-//     range, since both trees of a root are then the same.
+//     arcs u64[A] = (head << 32) | length, plus the rank of every device id
+//     u32[n] for the current label build (copied into the search cells). A
+//     symmetric graph keeps one range, since both trees of a root are then the
+//     same.
 //   * label store per side, in rank ids: off u64[n+1], ent u64[T] =
 //     (hub << 32) | dist, rows hub-ascending; a symmetric graph stores side 0
 //     only (side 1 is identical).
-//   * per-slot search scratch (tree.hip), S slots x 28 n bytes.
+//   * per-slot search scratch (tree.hip), S slots x 36 n bytes.
 //   * batch output: key (vertex << 32 | hub), ent; sorted, cleaned (GLL) and
 //     appended to the store after every batch.
 //
@@ -246,7 +247,7 @@ struct Builder {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  // The device graph (built once per graph handle and device) and this build's
+  // The device graph (built once per graph handle and device) and the current
   // rank of every device id.
   int upload_graph(const std::vector<uint32_t>& rank) {
     if (dg.device != o.device || dg.n != n) {
