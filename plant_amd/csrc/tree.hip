@@ -8,12 +8,14 @@
 //
 // One workgroup owns one root at a time (it pulls the next root index from a
 // device cursor) and a private slot of scratch sized n: a 16-byte cell per
-// vertex holding the packed state word and a round stamp, two frontier lists
-// and the list of touched vertices. The search is a bucketed label-correcting
-// Dijkstra (delta stepping): each round processes the frontier entries whose
-// distance is below the bucket bound B and re-queues the rest; a vertex whose
-// state improves is queued once per round (stamp) and processed again, so every
-// vertex is finally processed at its final state.
+// vertex holding the packed state word, a stamp and the vertex's rank, two
+// frontier lists, the list of touched vertices and a deferred pile. The search
+// is a bucketed label-correcting Dijkstra (delta stepping): each round
+// processes the frontier entries whose distance is below the bucket bound B;
+// the others wait on the pile (the lane-per-vertex path) or in the next
+// frontier (the pruned path) until B advances. A vertex whose state improves is
+// queued once per round (stamp) and processed again, so every vertex is finally
+// processed at its final state.
 //
 // State word = (dist << 32) | low. With TREE_FLAGS, low is the smallest rank on
 // the best path found so far, including the vertex itself; a 64-bit atomicMin
