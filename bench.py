@@ -158,7 +158,7 @@ def serving(chl, L, g, rng, seconds):
 def query_kernel_rate(chl, L, pairs, reps=5):
     """The merge-join kernel alone on device-resident pairs (HIP events around
     chl_query_device on one stream), with its algorithmic bytes: per pair the
-    pair, the result, two offset pairs, two rank lookups and both label rows."""
+    pair, the result, the two row ranges (begin, end) and both label rows."""
     hip = C.CDLL("libamdhip64.so")
     vp = C.c_void_p
     q = pairs.shape[0]
@@ -189,7 +189,7 @@ def query_kernel_rate(chl, L, pairs, reps=5):
     sec = ms.value / 1e3 / reps
     len1, len0 = L.row_lengths(1), L.row_lengths(0)
     rows = len1[pairs[:, 0]].astype(np.float64) + len0[pairs[:, 1]].astype(np.float64)
-    algo = float(q) * (8 + 4 + 32 + 8) + 8.0 * rows.sum()
+    algo = float(q) * (8 + 4 + 32) + 8.0 * rows.sum()
     gbs = algo / sec / 1e9
     return {"kernel_1M_pairs_queries_per_s": q / sec, "kernel_ms": sec * 1e3, "kernel_algorithmic_GBps": gbs,
             "kernel_hbm_frac": gbs / HBM_PEAK_GBS}

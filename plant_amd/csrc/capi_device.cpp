@@ -17,6 +17,10 @@ struct chl_labels {
   LabelStore store;
   DBuf<uint32_t> rank_of;        // vertex id -> row id
   std::vector<uint32_t> row_of;  // host copy of rank_of
+  // per vertex id the (begin, end) of its row on each side, read by queries;
+  // one table serves both sides when they are shared
+  DBuf<uint64_t> rng[2];
+  const uint64_t* d_rng(int s) const { return rng[store.shared_sides ? 0 : s].p; }
   // query staging for host-pointer calls
   DBuf<uint32_t> qpairs;
   DBuf<int32_t> qout;
@@ -34,6 +38,12 @@ int make_handle(chl_labels* l, const std::vector<uint32_t>& row_of) {
   if (!row_of.empty())
     CHL_HIP(hipMemcpy(l->rank_of.p, row_of.data(), sizeof(uint32_t) * row_of.size(), hipMemcpyHostToDevice));
   CHL_HIP(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking));
+  const uint32_t n = (uint32_t)l->store.n;
+  for (int s = 0; s < (l->store.shared_sides ? 1 : 2); ++s) {
+    CHL_HIP(l->rng[s].reserve(2 * (size_t)std::max<uint32_t>(n, 1)));
+    CHL_HIP(launch_row_ranges(l->store.d_off(s), l->rank_of.p, n, l->rng[s].p, l->stream));
+  }
+  CHL_HIP(hipStreamSynchronize(l->stream));
   return CHL_OK;
 }
 
@@ -96,13 +106,12 @@ int from_host_labels(const Labels& H, int device, chl_labels** out) {
 }  // namespace
 
 namespace chl {
-void labels_views(const chl_labels* l, const uint64_t** off1, const uint64_t** ent1, const uint64_t** off0,
-                  const uint64_t** ent0, const uint32_t** rank_of, uint32_t* n, int* device) {
-  *off1 = l->store.d_off(1);
+void labels_views(const chl_labels* l, const uint64_t** rng1, const uint64_t** ent1, const uint64_t** rng0,
+                  const uint64_t** ent0, uint32_t* n, int* device) {
+  *rng1 = l->d_rng(1);
   *ent1 = l->store.d_ent(1);
-  *off0 = l->store.d_off(0);
+  *rng0 = l->d_rng(0);
   *ent0 = l->store.d_ent(0);
-  *rank_of = l->rank_of.p;
   *n = (uint32_t)l->store.n;
   *device = l->store.device;
 }
@@ -399,8 +408,8 @@ int chl_query_device(const chl_labels* l, const uint32_t* d_pairs, int64_t q, in
   if (!l || q < 0 || (q > 0 && (!d_pairs || !d_out))) return set_error(CHL_E_INVAL, "bad arguments");
   if (q == 0) return CHL_OK;
   CHL_HIP(hipSetDevice(l->store.device));
-  CHL_HIP(launch_query(l->store.d_off(1), l->store.d_ent(1), l->store.d_off(0), l->store.d_ent(0), d_pairs,
-                       l->rank_of.p, (uint32_t)l->store.n, (uint64_t)q, d_out, (hipStream_t)stream));
+  CHL_HIP(launch_query(l->d_rng(1), l->store.d_ent(1), l->d_rng(0), l->store.d_ent(0), d_pairs, (uint32_t)l->store.n,
+                       (uint64_t)q, d_out, (hipStream_t)stream));
   return CHL_OK;
 }
 
@@ -414,8 +423,8 @@ int chl_query(const chl_labels* lc, const uint32_t* pairs, int64_t q, int32_t* o
   CHL_HIP(l->qpairs.reserve(2 * q));
   CHL_HIP(l->qout.reserve(q));
   CHL_HIP(hipMemcpyAsync(l->qpairs.p, pairs, sizeof(uint32_t) * 2 * q, hipMemcpyHostToDevice, l->stream));
-  CHL_HIP(launch_query(l->store.d_off(1), l->store.d_ent(1), l->store.d_off(0), l->store.d_ent(0), l->qpairs.p,
-                       l->rank_of.p, (uint32_t)l->store.n, (uint64_t)q, l->qout.p, l->stream));
+  CHL_HIP(launch_query(l->d_rng(1), l->store.d_ent(1), l->d_rng(0), l->store.d_ent(0), l->qpairs.p,
+                       (uint32_t)l->store.n, (uint64_t)q, l->qout.p, l->stream));
   CHL_HIP(hipMemcpyAsync(out, l->qout.p, sizeof(int32_t) * q, hipMemcpyDeviceToHost, l->stream));
   CHL_HIP(hipStreamSynchronize(l->stream));
   return CHL_OK;

@@ -237,10 +237,12 @@ hipError_t launch_clean(const uint64_t* key, const uint64_t* ent, uint64_t m, co
                         const uint64_t* oent, const uint64_t* oseg_off, uint8_t* keep, unsigned long long* ctr,
                         hipStream_t s);
 // Label merge-join query (shared_memory/labeling.hpp:56-65). pairs hold vertex
-// ids, mapped to rows by rank_of; an id >= n gives -1.
-hipError_t launch_query(const uint64_t* off1, const uint64_t* ent1, const uint64_t* off0, const uint64_t* ent0,
-                        const uint32_t* pairs, const uint32_t* rank_of, uint32_t n, uint64_t q, int32_t* out,
-                        hipStream_t s);
+// ids; rng1 / rng0 hold per vertex id the (begin, end) of its side-1 / side-0
+// row (launch_row_ranges); an id >= n gives -1.
+hipError_t launch_query(const uint64_t* rng1, const uint64_t* ent1, const uint64_t* rng0, const uint64_t* ent0,
+                        const uint32_t* pairs, uint32_t n, uint64_t q, int32_t* out, hipStream_t s);
+// rng[2 v], rng[2 v + 1] = off[row_of[v]], off[row_of[v] + 1] for v in [0, n).
+hipError_t launch_row_ranges(const uint64_t* off, const uint32_t* row_of, uint32_t n, uint64_t* rng, hipStream_t s);
 
 // Library primitives (hipCUB): radix sort of 64-bit key/value pairs, exclusive
 // scan of u32 counts into u64 offsets, stream compaction by flags.

@@ -160,9 +160,19 @@ __device__ __forceinline__ void load_chunk(const uint64_t* ent, uint64_t base, u
   }
 }
 
-__global__ __launch_bounds__(256) void query_kernel(const uint64_t* off1, const uint64_t* ent1, const uint64_t* off0,
-                                                    const uint64_t* ent0, const uint32_t* pairs,
-                                                    const uint32_t* rank_of, uint32_t n, uint64_t q, int32_t* out) {
+// rng[2 v], rng[2 v + 1] = the bounds of vertex v's row (row row_of[v]).
+__global__ void row_ranges_kernel(const uint64_t* off, const uint32_t* row_of, uint32_t n, uint64_t* rng) {
+  for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t r = row_of[v];
+    reinterpret_cast<ulonglong2*>(rng)[v] = make_ulonglong2(off[r], off[r + 1]);
+  }
+}
+
+// rng1 / rng0: per vertex id the (begin, end) of its side-1 / side-0 row, so a
+// pair reaches its rows with one 16-byte load per endpoint.
+__global__ __launch_bounds__(256) void query_kernel(const uint64_t* rng1, const uint64_t* ent1, const uint64_t* rng0,
+                                                    const uint64_t* ent0, const uint32_t* pairs, uint32_t n,
+                                                    uint64_t q, int32_t* out) {
   __shared__ uint32_t s_h[kQWaves][kQT];
   __shared__ uint32_t s_d[kQWaves][kQT];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -173,8 +183,9 @@ __global__ __launch_bounds__(256) void query_kernel(const uint64_t* off1, const 
       if (lane == 0) out[k] = -1;
       continue;
     }
-    const uint32_t u = rank_of[pu], v = rank_of[pv];
-    const uint64_t ab = off1[u], ae = off1[u + 1], bb = off0[v], be = off0[v + 1];
+    const ulonglong2 ru = reinterpret_cast<const ulonglong2*>(rng1)[pu];
+    const ulonglong2 rv = reinterpret_cast<const ulonglong2*>(rng0)[pv];
+    const uint64_t ab = ru.x, ae = ru.y, bb = rv.x, be = rv.y;
     uint32_t best = kInfDist;
     for (uint64_t t0 = ab; t0 < ae; t0 += kQT) {
       const uint32_t tl = (uint32_t)((ae - t0) < kQT ? (ae - t0) : kQT);
@@ -306,12 +317,17 @@ hipError_t launch_clean(const uint64_t* key, const uint64_t* ent, uint64_t m, co
   return hipGetLastError();
 }
 
-hipError_t launch_query(const uint64_t* off1, const uint64_t* ent1, const uint64_t* off0, const uint64_t* ent0,
-                        const uint32_t* pairs, const uint32_t* rank_of, uint32_t n, uint64_t q, int32_t* out,
-                        hipStream_t s) {
+hipError_t launch_query(const uint64_t* rng1, const uint64_t* ent1, const uint64_t* rng0, const uint64_t* ent0,
+                        const uint32_t* pairs, uint32_t n, uint64_t q, int32_t* out, hipStream_t s) {
   if (!q) return hipSuccess;
-  hipLaunchKernelGGL(query_kernel, dim3(grid_for(q * 64, 256, 8192)), dim3(256), 0, s, off1, ent1, off0, ent0, pairs,
-                     rank_of, n, q, out);
+  hipLaunchKernelGGL(query_kernel, dim3(grid_for(q * 64, 256, 8192)), dim3(256), 0, s, rng1, ent1, rng0, ent0, pairs, n,
+                     q, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_ranges(const uint64_t* off, const uint32_t* row_of, uint32_t n, uint64_t* rng, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(row_ranges_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, off, row_of, n, rng);
   return hipGetLastError();
 }
 

@@ -112,11 +112,10 @@ struct Pool {
 
 struct chl_server {
   const chl_labels* labels = nullptr;
-  const uint64_t* off1 = nullptr;
+  const uint64_t* rng1 = nullptr;  // per vertex id: (begin, end) of its side-1 row
   const uint64_t* ent1 = nullptr;
-  const uint64_t* off0 = nullptr;
+  const uint64_t* rng0 = nullptr;
   const uint64_t* ent0 = nullptr;
-  const uint32_t* rank_of = nullptr;
   uint32_t n = 0;
   int device = 0;
   chl_server_opts o;
@@ -152,8 +151,8 @@ struct chl_server {
 
 namespace chl {
 // Device views of a label handle (defined in capi_device.cpp).
-void labels_views(const chl_labels* l, const uint64_t** off1, const uint64_t** ent1, const uint64_t** off0,
-                  const uint64_t** ent0, const uint32_t** rank_of, uint32_t* n, int* device);
+void labels_views(const chl_labels* l, const uint64_t** rng1, const uint64_t** ent1, const uint64_t** rng0,
+                  const uint64_t** ent0, uint32_t* n, int* device);
 }
 
 int chl_server::launch(Slot& s) {
@@ -166,7 +165,7 @@ int chl_server::launch(Slot& s) {
     std::lock_guard<std::mutex> lk(mu);
     for (Request* r : s.group) arena_pool.release(r->arena_off, (uint64_t)r->q);
   }
-  CHL_HIP(launch_query(off1, ent1, off0, ent0, s.d_pairs, rank_of, n, s.total, s.d_out, s.stream));
+  CHL_HIP(launch_query(rng1, ent1, rng0, ent0, s.d_pairs, n, s.total, s.d_out, s.stream));
   CHL_HIP(hipEventRecord(s.done, s.stream));
   return CHL_OK;
 }
@@ -269,7 +268,7 @@ int chl_server_create(const chl_labels* l, const chl_server_opts* opts, chl_serv
   std::unique_ptr<chl_server> s(new chl_server());
   s->labels = l;
   s->o = o;
-  labels_views(l, &s->off1, &s->ent1, &s->off0, &s->ent0, &s->rank_of, &s->n, &s->device);
+  labels_views(l, &s->rng1, &s->ent1, &s->rng0, &s->ent0, &s->n, &s->device);
   CHL_HIP(hipSetDevice(s->device));
   s->pool.init((uint64_t)o.max_inflight_pairs);
   // queue arena: twice the pool (the back-pressure limit) plus one request;
@@ -296,7 +295,7 @@ int chl_server_create(const chl_labels* l, const chl_server_opts* opts, chl_serv
   // stream's first use (device queue set-up takes milliseconds)
   for (auto& sl : s->slots) {
     sl.h_pairs[0] = sl.h_pairs[1] = 0;
-    CHL_HIP(launch_query(s->off1, s->ent1, s->off0, s->ent0, sl.d_pairs, s->rank_of, s->n, 1, sl.d_out, sl.stream));
+    CHL_HIP(launch_query(s->rng1, s->ent1, s->rng0, s->ent0, sl.d_pairs, s->n, 1, sl.d_out, sl.stream));
     CHL_HIP(hipStreamSynchronize(sl.stream));
   }
   chl_server* raw = s.release();
