@@ -100,6 +100,11 @@ int chl_order_betweenness_device(const chl_graph* g, int64_t roots, uint64_t see
                                  float* score, double* device_ms);
 /* Degree order, distributed_memory/degree.cpp:62-67 when reference_ties=1. */
 int chl_order_degree(const chl_graph* g, uint64_t seed, int reference_ties, uint32_t* order);
+/* The same degree order computed on a gfx950 device (keys and stable radix
+ * sorts); replaces distributed_memory/degree.cpp:62-67 (reference_ties=1).
+ * device_ms, if not NULL, receives the device time of the keys and sorts. */
+int chl_order_degree_device(const chl_graph* g, uint64_t seed, int reference_ties, int device, uint32_t* order,
+                            double* device_ms);
 
 /* ---- label construction --------------------------------------------------- */
 enum { CHL_MODE_GLL = 0, CHL_MODE_PLANT = 1, CHL_MODE_HYBRID = 2 };

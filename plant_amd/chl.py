@@ -91,6 +91,7 @@ def lib():
             "chl_order_betweenness_scored": ([vp, i64, u64, vp, vp], C.c_int),
             "chl_order_betweenness_device": ([vp, i64, u64, C.c_int, vp, vp, vp], C.c_int),
             "chl_order_degree": ([vp, u64, C.c_int, vp], C.c_int),
+            "chl_order_degree_device": ([vp, u64, C.c_int, C.c_int, vp, vp], C.c_int),
             "chl_build_opts_default": ([C.POINTER(BuildOpts)], None),
             "chl_build": ([vp, vp, i64, C.POINTER(BuildOpts), pp, C.POINTER(BuildStats)], C.c_int),
             "chl_labels_write": ([vp, C.c_char_p], C.c_int),
@@ -244,6 +245,13 @@ class Graph:
         o = np.zeros(self.n, np.uint32)
         _check(lib().chl_order_degree(self._h, seed, int(reference_ties), _ptr(o)))
         return o
+
+    def order_degree_device(self, seed=7, reference_ties=False, device=0):
+        """The same order as order_degree, computed on the GPU. Returns (order, device ms)."""
+        o = np.zeros(self.n, np.uint32)
+        ms = C.c_double(0.0)
+        _check(lib().chl_order_degree_device(self._h, seed, int(reference_ties), device, _ptr(o), C.byref(ms)))
+        return o, ms.value
 
 
 def order_load(path):

@@ -404,6 +404,16 @@ int chl_order_betweenness_device(const chl_graph* g, int64_t roots, uint64_t see
   return CHL_OK;
 }
 
+int chl_order_degree_device(const chl_graph* g, uint64_t seed, int reference_ties, int device, uint32_t* order,
+                            double* device_ms) {
+  if (!g || !order) return set_error(CHL_E_INVAL, "bad arguments");
+  std::vector<uint32_t> o;
+  int rc = order_degree_device(g->g, device, seed, reference_ties != 0, o, device_ms);
+  if (rc) return rc;
+  if (!o.empty()) memcpy(order, o.data(), sizeof(uint32_t) * o.size());
+  return CHL_OK;
+}
+
 int chl_query_device(const chl_labels* l, const uint32_t* d_pairs, int64_t q, int32_t* d_out, void* stream) {
   if (!l || q < 0 || (q > 0 && (!d_pairs || !d_out))) return set_error(CHL_E_INVAL, "bad arguments");
   if (q == 0) return CHL_OK;

@@ -207,6 +207,15 @@ struct BcParams {
 hipError_t launch_betweenness(const BcParams& p, uint32_t slots, hipStream_t s);
 hipError_t launch_score_keys(const unsigned long long* score, const uint32_t* inv, uint32_t n, uint64_t* key,
                              uint64_t* val, hipStream_t s);
+// Degree order keys over vertex ids in [0, n); off1 / off0 are the out / in arc
+// offsets (n + 1 each) and every degree is below 2^32. reference_ties: key =
+// (degree descending, id descending); otherwise key = the seeded tie key.
+// val[v] = v.
+hipError_t launch_degree_keys(const uint64_t* off1, const uint64_t* off0, uint32_t n, uint64_t seed, int reference_ties,
+                              uint64_t* key, uint64_t* val, hipStream_t s);
+// key[i] = the descending degree key of vertex val[i] (32 bits).
+hipError_t launch_degree_rekey(const uint64_t* off1, const uint64_t* off0, const uint64_t* val, uint32_t n,
+                               uint64_t* key, hipStream_t s);
 
 // Label-store maintenance.
 // seg[v] = first index k of the sorted keys with key >> 32 >= v, for v in [0, n].

@@ -76,6 +76,10 @@ int build_labels(const Graph& g, DeviceGraph& dg, const std::vector<uint32_t>& o
 // sort time.
 int order_betweenness_device(const Graph& g, DeviceGraph& dg, int device, int64_t roots, uint64_t seed,
                              std::vector<uint32_t>& order, std::vector<uint64_t>* score, double* device_ms);
+// Degree order computed on the device; the same order as the host
+// order_degree. device_ms: key and sort time.
+int order_degree_device(const Graph& g, int device, uint64_t seed, bool reference_ties, std::vector<uint32_t>& order,
+                        double* device_ms);
 // Build under the batching policy of opts.batching (static rank order or learned packing).
 int run_build(const Graph& g, DeviceGraph& dg, const std::vector<uint32_t>& order, const chl_build_opts& opts,
               chl_build_stats* stats, LabelStore& L);

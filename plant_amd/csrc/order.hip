@@ -207,6 +207,40 @@ __global__ void score_keys_kernel(const unsigned long long* score, const uint32_
   }
 }
 
+// Degree of vertex v: out-arcs plus in-arcs (graph.hpp:324).
+__device__ __forceinline__ uint32_t degree_of(const uint64_t* off1, const uint64_t* off0, uint32_t v) {
+  return (uint32_t)((off1[v + 1] - off1[v]) + (off0[v + 1] - off0[v]));
+}
+
+// The SplitMix64 generator step (Steele, Lea and Flood, OOPSLA 2014): the
+// seeded tie key of the degree order.
+__device__ __forceinline__ uint64_t splitmix_step(uint64_t s) {
+  uint64_t z = s + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// First sort keys of the degree order, val[v] = v. reference_ties: one key
+// (degree descending, then id descending, distributed_memory/degree.cpp:62-67);
+// otherwise the seeded tie key, which a second, stable sort by degree keeps.
+__global__ void degree_keys_kernel(const uint64_t* off1, const uint64_t* off0, uint32_t n, uint64_t seed,
+                                   int reference_ties, uint64_t* key, uint64_t* val) {
+  for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < n; x += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t v = (uint32_t)x;
+    key[v] = reference_ties ? ((uint64_t)(0xFFFFFFFFu - degree_of(off1, off0, v)) << 32) | (0xFFFFFFFFu - v)
+                            : splitmix_step(seed ^ ((uint64_t)v * 0xD1B54A32D192ED03ull));
+    val[v] = v;
+  }
+}
+
+// key[i] = the degree key (32 bits, descending) of vertex val[i].
+__global__ void degree_rekey_kernel(const uint64_t* off1, const uint64_t* off0, const uint64_t* val, uint32_t n,
+                                    uint64_t* key) {
+  for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < n; x += (size_t)gridDim.x * blockDim.x)
+    key[x] = 0xFFFFFFFFu - degree_of(off1, off0, (uint32_t)val[x]);
+}
+
 unsigned grid_for(size_t count) {
   const size_t b = (count + 255) / 256;
   return (unsigned)(b < 8192 ? (b ? b : 1) : 8192);
@@ -224,6 +258,21 @@ hipError_t launch_score_keys(const unsigned long long* score, const uint32_t* in
                              uint64_t* val, hipStream_t s) {
   if (!n) return hipSuccess;
   hipLaunchKernelGGL(score_keys_kernel, dim3(grid_for(n)), dim3(256), 0, s, score, inv, n, key, val);
+  return hipGetLastError();
+}
+
+hipError_t launch_degree_keys(const uint64_t* off1, const uint64_t* off0, uint32_t n, uint64_t seed, int reference_ties,
+                              uint64_t* key, uint64_t* val, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(degree_keys_kernel, dim3(grid_for(n)), dim3(256), 0, s, off1, off0, n, seed, reference_ties, key,
+                     val);
+  return hipGetLastError();
+}
+
+hipError_t launch_degree_rekey(const uint64_t* off1, const uint64_t* off0, const uint64_t* val, uint32_t n,
+                               uint64_t* key, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(degree_rekey_kernel, dim3(grid_for(n)), dim3(256), 0, s, off1, off0, val, n, key);
   return hipGetLastError();
 }
 

@@ -1,5 +1,6 @@
-// Host side of the device sampled-betweenness order: roots, scratch slots, the
-// persistent tree launch, then a stable radix sort of (~score, id).
+// Host side of the device vertex orders. Sampled betweenness: roots, scratch
+// slots, the persistent tree launch, then a stable radix sort of (~score, id).
+// Degree: per-vertex keys, then one or two stable radix sorts.
 #include <algorithm>
 #include <cstring>
 
@@ -130,6 +131,70 @@ int order_impl(const Graph& g, DeviceGraph& dg, int device, int64_t roots, uint6
 }
 
 }  // namespace
+
+// Degree order (distributed_memory/degree.cpp:62-67 with reference ties): the
+// degrees come from the two arc-offset arrays, one stable radix sort orders
+// (degree descending, id descending); with seeded ties a first sort orders the
+// ids by their seeded key (equal keys keep ids ascending) and a second, stable
+// sort by degree keeps that order among equal degrees.
+int order_degree_device(const Graph& g, int device, uint64_t seed, bool reference_ties, std::vector<uint32_t>& order,
+                        double* device_ms) {
+  const int64_t n = g.n;
+  order.resize(n);
+  if (device_ms) *device_ms = 0;
+  if (n == 0) return CHL_OK;
+  if (n >= 0x7FFFFFFFll) return set_error(CHL_E_RANGE, "more than 2^31 - 1 vertices");
+  if (g.off[0][n] + g.off[1][n] >= 0xFFFFFFFFull) return set_error(CHL_E_RANGE, "2^32 or more arcs");
+  int rc = ensure_device(device);
+  if (rc) return rc;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  CHL_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  struct Guard {
+    hipStream_t& s;
+    hipEvent_t &a, &b;
+    ~Guard() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+      if (s) (void)hipStreamDestroy(s);
+    }
+  } guard{stream, e0, e1};
+  CHL_HIP(hipEventCreate(&e0));
+  CHL_HIP(hipEventCreate(&e1));
+  DBuf<uint64_t> off1, off0, key, val, skey, sval;
+  DBuf<uint8_t> tmp;
+  CHL_HIP(off1.reserve(n + 1));
+  CHL_HIP(off0.reserve(n + 1));
+  CHL_HIP(key.reserve(n));
+  CHL_HIP(val.reserve(n));
+  CHL_HIP(skey.reserve(n));
+  CHL_HIP(sval.reserve(n));
+  CHL_HIP(hipMemcpyAsync(off1.p, g.off[1].data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, stream));
+  CHL_HIP(hipMemcpyAsync(off0.p, g.off[0].data(), sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, stream));
+  size_t tb = 0, tb2 = 0;
+  CHL_HIP(sort_pairs_u64(nullptr, tb, key.p, skey.p, val.p, sval.p, (uint64_t)n, 64, stream));
+  CHL_HIP(sort_pairs_u64(nullptr, tb2, key.p, skey.p, sval.p, val.p, (uint64_t)n, 32, stream));
+  CHL_HIP(tmp.reserve(std::max<size_t>({tb, tb2, 1})));
+  CHL_HIP(hipEventRecord(e0, stream));
+  CHL_HIP(launch_degree_keys(off1.p, off0.p, (uint32_t)n, seed, reference_ties ? 1 : 0, key.p, val.p, stream));
+  CHL_HIP(sort_pairs_u64(tmp.p, tb, key.p, skey.p, val.p, sval.p, (uint64_t)n, 64, stream));
+  const uint64_t* result = sval.p;
+  if (!reference_ties) {
+    // second pass: degree keys of the tie-ordered ids, sorted stably (32 bits)
+    CHL_HIP(launch_degree_rekey(off1.p, off0.p, sval.p, (uint32_t)n, key.p, stream));
+    CHL_HIP(sort_pairs_u64(tmp.p, tb2, key.p, skey.p, sval.p, val.p, (uint64_t)n, 32, stream));
+    result = val.p;
+  }
+  CHL_HIP(hipEventRecord(e1, stream));
+  std::vector<uint64_t> hv(n);
+  CHL_HIP(hipMemcpyAsync(hv.data(), result, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, stream));
+  CHL_HIP(hipStreamSynchronize(stream));
+  for (int64_t i = 0; i < n; ++i) order[i] = (uint32_t)hv[i];
+  float ms = 0;
+  CHL_HIP(hipEventElapsedTime(&ms, e0, e1));
+  if (device_ms) *device_ms = ms;
+  return CHL_OK;
+}
 
 int order_betweenness_device(const Graph& g, DeviceGraph& dg, int device, int64_t roots, uint64_t seed,
                              std::vector<uint32_t>& order, std::vector<uint64_t>* score, double* device_ms) {
