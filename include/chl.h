@@ -61,6 +61,9 @@ int chl_graph_from_arcs(int64_t n, int64_t m, const uint32_t* src, const uint32_
 /* Replaces Graph::write (graph.hpp:338-343): DIMACS, one "a" line per out-arc. */
 int chl_graph_write(const chl_graph* g, const char* path);
 int64_t chl_graph_n(const chl_graph* g);
+/* Replaces Graph::get_m (graph.hpp:314): arcs as the input listed them (an
+ * undirected input edge counts twice), before parallel arcs are merged. */
+int64_t chl_graph_m(const chl_graph* g);
 /* Arcs in a range: 1 = out-arcs, 0 = in-arcs (graph.hpp:319-320). */
 int64_t chl_graph_arcs(const chl_graph* g, int range);
 /* 1 if every arc has an equal-length reverse arc (labels of both sides agree). */

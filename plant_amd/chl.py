@@ -78,6 +78,7 @@ def lib():
             "chl_graph_from_arcs": ([i64, i64, vp, vp, vp, C.c_int, pp], C.c_int),
             "chl_graph_write": ([vp, C.c_char_p], C.c_int),
             "chl_graph_n": ([vp], i64),
+            "chl_graph_m": ([vp], i64),
             "chl_graph_arcs": ([vp, C.c_int], i64),
             "chl_graph_symmetric": ([vp], C.c_int),
             "chl_graph_csr": ([vp, C.c_int, vp, vp, vp], C.c_int),
@@ -196,6 +197,11 @@ class Graph:
     @property
     def n(self):
         return lib().chl_graph_n(self._h)
+
+    @property
+    def m(self):
+        """Arcs as the input listed them (graph.hpp:314 get_m)."""
+        return lib().chl_graph_m(self._h)
 
     def arcs(self, rng=1):
         return lib().chl_graph_arcs(self._h, rng)

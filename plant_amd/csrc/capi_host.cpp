@@ -85,6 +85,7 @@ int chl_graph_write(const chl_graph* g, const char* path) {
 }
 
 int64_t chl_graph_n(const chl_graph* g) { return g ? g->g.n : -1; }
+int64_t chl_graph_m(const chl_graph* g) { return g ? g->g.m_input : -1; }
 int64_t chl_graph_arcs(const chl_graph* g, int range) {
   if (!g || range < 0 || range > 1) return -1;
   return (int64_t)g->g.head[range].size();

@@ -8,7 +8,13 @@ reference hhl on them with -l, and stores inputs and label files here, plus a
 manifest with sha256 sums. For the 100x100 grid workload only the sha256 and
 sizes of the reference's label file are kept (the file is ~12 MB).
 
+The reference's degree-order tool (distributed_memory/degree.cpp, built by
+oracle/Makefile into oracle/_ref/degree) is run on every golden graph file; its
+order goes to <name>.degree and its stdout line into the manifest
+("degree_stdout").
+
 Run from the repository root:  python tests/golden/make_golden.py
+(python tests/golden/make_golden.py degree  redoes only the degree fixtures)
 """
 import hashlib
 import json
@@ -25,6 +31,7 @@ sys.path.insert(0, ROOT)
 from plant_amd import chl  # noqa: E402
 
 HHL = os.path.join(ROOT, "oracle", "_ref", "hhl")
+DEGREE = os.path.join(ROOT, "oracle", "_ref", "degree")
 
 
 def sha(path):
@@ -83,10 +90,32 @@ def main():
     for p in (gp, op, lp):
         os.remove(p)
     os.rmdir(tmp)
+    degree_fixtures(manifest)
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
     print(json.dumps(manifest, indent=1))
 
 
+def degree_fixtures(manifest):
+    """<name>.degree and the stdout line of the reference degree tool, per golden graph."""
+    if not os.path.exists(DEGREE):
+        raise SystemExit(f"{DEGREE} missing: build it with make -C oracle (needs the reference tree)")
+    for name in sorted(manifest):
+        gp = os.path.join(HERE, f"{name}.gr")
+        if not os.path.exists(gp):
+            continue
+        dp = os.path.join(HERE, f"{name}.degree")
+        r = subprocess.run([DEGREE, "-o", dp, gp], check=True, capture_output=True, text=True)
+        manifest[name]["degree_stdout"] = r.stdout
+        manifest[name]["degree_sha256"] = sha(dp)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["degree"]:
+        with open(os.path.join(HERE, "manifest.json")) as f:
+            m = json.load(f)
+        degree_fixtures(m)
+        with open(os.path.join(HERE, "manifest.json"), "w") as f:
+            json.dump(m, f, indent=1, sort_keys=True)
+    else:
+        main()

@@ -1,6 +1,8 @@
 """Vertex orders computed on the GPU.
 
-Degree order: equal to a numpy restatement of distributed_memory/degree.cpp:62-67
+Degree order: equal to the reference degree tool's output on the golden graphs
+(its order files and stdout, and plant_amd/bin/degree byte for byte), to a numpy
+restatement of distributed_memory/degree.cpp:62-67
 (degree = out-arcs + in-arcs descending, ties by higher id first) and, with seeded
 ties, of (degree descending, SplitMix64 key ascending, id ascending); and equal to
 the host order_degree.
@@ -8,10 +10,19 @@ the host order_degree.
 Sampled betweenness: the GPU order equals the host order (gen.cpp
 order_betweenness, pinned in test_host.py against scipy's Dijkstra and a closed
 form): the same scores and the same tie-broken order."""
+import json
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+DEGREE_TOOL = os.path.join(os.path.dirname(GOLD), "..", "plant_amd", "bin", "degree")
+with open(os.path.join(GOLD, "manifest.json")) as _f:
+    MANIFEST = json.load(_f)
 
 skip_bc = pytest.mark.skip(reason="device betweenness order switched off: its first GPU run ended with a device "
                            "fault whose cause is not found (chl_order_betweenness_device returns CHL_E_STATE)")
@@ -67,6 +78,22 @@ def test_degree_order(chl, reference_ties):
     g = chl.Graph.from_arcs(4, np.array([0, 1], np.uint32), np.array([1, 2], np.uint32), np.array([1, 1], np.int32))
     got, _ = g.order_degree_device(reference_ties=True)
     assert list(got) == [1, 2, 0, 3]
+
+
+@pytest.mark.parametrize("name", sorted(k for k, v in MANIFEST.items() if "degree_stdout" in v))
+def test_degree_tool_matches_reference_tool(chl, name, tmp_path):
+    """plant_amd/bin/degree (device order) prints the reference tool's line and
+    writes its order file byte for byte (fixtures from distributed_memory/degree.cpp)."""
+    gr = os.path.join(GOLD, name + ".gr")
+    fixture = os.path.join(GOLD, name + ".degree")
+    got, _ = chl.Graph.load(gr).order_degree_device(reference_ties=True)
+    assert np.array_equal(got, chl.order_load(fixture))
+    out = tmp_path / "o.txt"
+    print("degree -o", out, gr, flush=True)
+    r = subprocess.run([DEGREE_TOOL, "-o", str(out), gr], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == MANIFEST[name]["degree_stdout"]
+    assert out.read_bytes() == open(fixture, "rb").read()
 
 
 def test_degree_order_1M(chl):

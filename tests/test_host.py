@@ -1,6 +1,7 @@
 """Host-side parts of libchl.so that need no GPU: graph readers (against the
 oracle's restatement of shared_memory/graph.hpp, itself checked through the
 reference's own hhl), generators, order files, and the exported C ABI."""
+import json
 import os
 import subprocess
 
@@ -8,6 +9,15 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc_mod
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+DEGREE_TOOL = os.path.join(os.path.dirname(GOLD), "..", "plant_amd", "bin", "degree")
+REF_DEGREE = os.path.join(os.path.dirname(orc_mod.REF_HHL), "degree")
+
+
+def golden_manifest():
+    with open(os.path.join(GOLD, "manifest.json")) as f:
+        return json.load(f)
 
 FIXTURES = {
     # duplicate arcs (min length kept), unequal reverse lengths, a self-loop, comments
@@ -354,3 +364,44 @@ def test_local_pairs_stay_within_the_hop_ball(chl):
     assert set(np.unique(d).tolist()) == set(range(-3, 4))
     assert np.array_equal(p, chl.local_pairs(g, 5000, 3, seed=4).astype(np.int64))  # seeded
     assert np.array_equal(chl.local_pairs(g, 10, 0, seed=1)[:, 0], chl.local_pairs(g, 10, 0, seed=1)[:, 1])
+
+
+@pytest.mark.parametrize("name", sorted(k for k, v in golden_manifest().items() if "degree_stdout" in v))
+def test_degree_order_matches_reference_tool(chl, name):
+    """The host degree order, n and m equal what the reference's degree tool
+    (distributed_memory/degree.cpp:54-67) printed and wrote for the same graph file."""
+    man = golden_manifest()[name]
+    g = chl.Graph.load(os.path.join(GOLD, name + ".gr"))
+    assert f"Graph has {g.n} vertices and {g.m} arcs\n" == man["degree_stdout"]
+    want = chl.order_load(os.path.join(GOLD, name + ".degree"))
+    assert np.array_equal(g.order_degree(reference_ties=True), want)
+
+
+DEGREE_USAGE_CASES = [[], ["-h"], ["-o"], ["-o", "x"], ["g.gr"], ["-o", "x", "a", "b"], ["-q", "g"],
+                      ["-o", "x", "--", "g"], ["--"]]
+
+
+@pytest.mark.parametrize("args", DEGREE_USAGE_CASES)
+def test_degree_tool_usage(args):
+    """Command-line errors of plant_amd/bin/degree: the reference tool's usage text
+    on stdout and exit status 1 (degree.cpp:32-52), before any device use."""
+    if not os.path.exists(DEGREE_TOOL):
+        pytest.skip("plant_amd/bin/degree not built")
+    r = subprocess.run([DEGREE_TOOL] + args, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert r.stdout == f"Usage: {DEGREE_TOOL} -o ordering graph\n  -o ordering\tFile with the vertex order\n"
+    if os.path.exists(REF_DEGREE):
+        ref = subprocess.run([REF_DEGREE] + args, capture_output=True, text=True, timeout=60)
+        assert ref.returncode == r.returncode
+        assert ref.stdout.replace(REF_DEGREE, DEGREE_TOOL) == r.stdout
+
+
+def test_degree_tool_unreadable_graph(tmp_path):
+    if not os.path.exists(DEGREE_TOOL):
+        pytest.skip("plant_amd/bin/degree not built")
+    missing = str(tmp_path / "none.gr")
+    r = subprocess.run([DEGREE_TOOL, "-o", str(tmp_path / "o"), missing], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert r.stdout == ""
+    assert r.stderr.endswith(f"Unable to read graph from file {missing}\n")
+    assert not (tmp_path / "o").exists()
