@@ -1,5 +1,7 @@
-// Graph ingest (DIMACS, METIS, SNAP, edgelist with autodetection), CSR build,
-// order files and label files for the CHL engine.
+// Graph ingest (DIMACS, METIS, SNAP, edgelist with autodetection), CSR build
+// and order files for the CHL engine. DIMACS, SNAP and edge-list files in the
+// plain layout are parsed on all host threads; every other file takes the
+// sequential readers.
 //
 // Semantics follow the reference readers (shared_memory/graph.hpp:164-335):
 // the same formats are tried in the same order, ids are 1-based for DIMACS/METIS
@@ -117,6 +119,31 @@ inline bool scan_ll(const char*& p, const char* e, long long& out) {
   out = neg ? -x : x;
   return true;
 }
+
+// [q, e) split into T ranges of whole lines (e ends in a newline).
+std::vector<const char*> line_cuts(const char* q, const char* e, int T) {
+  std::vector<const char*> cut(T + 1, e);
+  cut[0] = q;
+  for (int t = 1; t < T; ++t) {
+    const char* c = q + (size_t)(e - q) * t / T;
+    if (c < cut[t - 1]) c = cut[t - 1];
+    const char* nl = c < e ? (const char*)memchr(c, '\n', (size_t)(e - c)) : nullptr;
+    cut[t] = nl ? nl + 1 : e;
+  }
+  return cut;
+}
+
+// Per-thread arc lists joined in range order.
+void join_parts(std::vector<std::vector<Arc3>>& part, std::vector<Arc3>& arcs) {
+  size_t total = 0;
+  for (auto& v : part) total += v.size();
+  arcs.clear();
+  arcs.reserve(total);
+  for (auto& v : part) {
+    arcs.insert(arcs.end(), v.begin(), v.end());
+    std::vector<Arc3>().swap(v);
+  }
+}
 }  // namespace
 
 // DIMACS on all host threads, for files in the plain layout: every line ends in
@@ -148,14 +175,7 @@ static int parse_dimacs_fast(const char* data, size_t size, int64_t& n, std::vec
     break;
   }
   const int T = host_threads();
-  std::vector<const char*> cut(T + 1, e);
-  cut[0] = q;
-  for (int t = 1; t < T; ++t) {
-    const char* c = q + (size_t)(e - q) * t / T;
-    if (c < cut[t - 1]) c = cut[t - 1];
-    const char* nl = c < e ? (const char*)memchr(c, '\n', (size_t)(e - c)) : nullptr;
-    cut[t] = nl ? nl + 1 : e;
-  }
+  const std::vector<const char*> cut = line_cuts(q, e, T);
   std::vector<std::vector<Arc3>> part(T);
   std::vector<int> perr(T, CHL_OK);
   std::atomic<bool> plain(true);
@@ -182,14 +202,135 @@ static int parse_dimacs_fast(const char* data, size_t size, int64_t& n, std::vec
     }
   });
   if (!plain) return 0;
-  size_t total = 0;
-  for (auto& v : part) total += v.size();
-  arcs.clear();
-  arcs.reserve(total);
-  for (int t = 0; t < T; ++t) {
-    arcs.insert(arcs.end(), part[t].begin(), part[t].end());
+  join_parts(part, arcs);
+  for (int t = 0; t < T; ++t)
     if (perr[t] != CHL_OK) err = perr[t];
+  return 1;
+}
+
+// SNAP on all host threads, for files in the plain layout: the first byte is
+// '#' (so the DIMACS and METIS readers reject the file), every line ends in a
+// newline, no NUL byte, '#' lines only before the first data line, exactly one
+// of them reading "# Nodes: n Edges: m", every line under 512 bytes, and each
+// data line two ids in [0, n) (anything may follow them, as sscanf allows).
+// Returns 1 with the arcs and n that parse_snap gives, or 0 when the file is
+// anything else: the sequential readers then decide.
+static int parse_snap_fast(const char* data, size_t size, int64_t& n, std::vector<Arc3>& arcs) {
+  if (size == 0 || data[0] != '#' || data[size - 1] != '\n' || memchr(data, 0, size)) return 0;
+  const char* e = data + size;
+  const char* q = data;
+  bool inited = false;
+  while (q < e && *q == '#') {
+    const char* nl = (const char*)memchr(q, '\n', (size_t)(e - q));
+    if (nl - q + 1 > 511) return 0;
+    char buf[512];
+    memcpy(buf, q, (size_t)(nl - q + 1));
+    buf[nl - q + 1] = 0;
+    long long nn, mm;
+    if (sscanf(buf, "# Nodes: %lld Edges: %lld", &nn, &mm) == 2) {
+      if (inited) return 0;
+      inited = true;
+      n = nn;
+    }
+    q = nl + 1;
   }
+  if (!inited) return 0;
+  const int T = host_threads();
+  const std::vector<const char*> cut = line_cuts(q, e, T);
+  std::vector<std::vector<Arc3>> part(T);
+  std::atomic<bool> plain(true);
+  const long long nmax = n;
+  parallel_for(T, T, 1, [&](int64_t t) {
+    std::vector<Arc3>& out = part[t];
+    for (const char* p = cut[t]; p < cut[t + 1] && plain.load(std::memory_order_relaxed);) {
+      const char* nl = (const char*)memchr(p, '\n', (size_t)(cut[t + 1] - p));
+      long long u, v;
+      const char* r = p;
+      if (*p == '#' || nl - p + 1 > 511 || !scan_ll(r, nl, u) || !scan_ll(r, nl, v) || u < 0 || v < 0 || u >= nmax ||
+          v >= nmax) {
+        plain = false;
+        break;
+      }
+      out.push_back({(uint32_t)u, (uint32_t)v, 1});
+      p = nl + 1;
+    }
+  });
+  if (!plain) return 0;
+  join_parts(part, arcs);
+  return 1;
+}
+
+// Edge list on all host threads, for files in the plain layout: every line ends
+// in a newline, no NUL byte, every line under 512 bytes holding three numbers
+// u v w with 0 <= u, v < 2^31 (anything may follow them, as sscanf allows), the
+// ids exactly 0 .. max, and a first line that the METIS header check rejects
+// (a sign, or a third number that is no METIS fmt code, graph.hpp:224-236); the
+// DIMACS and SNAP readers reject any file whose first line starts with a number.
+// Returns 1 with the arcs, n and error that parse_edgelist gives, or 0 when the
+// file is anything else: the sequential readers then decide.
+static int parse_edgelist_fast(const char* data, size_t size, int64_t& n, std::vector<Arc3>& arcs, int& err) {
+  if (size == 0 || data[size - 1] != '\n' || memchr(data, 0, size)) return 0;
+  const char* e = data + size;
+  {
+    const char* nl = (const char*)memchr(data, '\n', size);
+    const char* r = data;
+    long long u, v, w;
+    if (!scan_ll(r, nl, u) || !scan_ll(r, nl, v) || !scan_ll(r, nl, w)) return 0;
+    const bool sign = memchr(data, '-', (size_t)(r - data)) || memchr(data, '+', (size_t)(r - data));
+    const bool fmt_bad = w < 0 || w % 10 > 1 || (w / 10) % 10 > 1 || w > 111;
+    if (!sign && !fmt_bad) return 0;
+  }
+  const int T = host_threads();
+  const std::vector<const char*> cut = line_cuts(data, e, T);
+  std::vector<std::vector<Arc3>> part(T);
+  std::vector<int> perr(T, CHL_OK);
+  std::vector<long long> pmax(T, -1);
+  std::atomic<bool> plain(true);
+  parallel_for(T, T, 1, [&](int64_t t) {
+    std::vector<Arc3>& out = part[t];
+    long long mx = -1;
+    for (const char* p = cut[t]; p < cut[t + 1] && plain.load(std::memory_order_relaxed);) {
+      const char* nl = (const char*)memchr(p, '\n', (size_t)(cut[t + 1] - p));
+      long long u, v, w;
+      const char* r = p;
+      if (nl - p + 1 > 511 || !scan_ll(r, nl, u) || !scan_ll(r, nl, v) || !scan_ll(r, nl, w) || u < 0 || v < 0 ||
+          u > 0x7FFFFFFFll || v > 0x7FFFFFFFll) {
+        plain = false;
+        break;
+      }
+      if (w <= 0 || w > 2147483647LL) perr[t] = CHL_E_WEIGHT;
+      mx = std::max(mx, std::max(u, v));
+      out.push_back({(uint32_t)u, (uint32_t)v, (int32_t)w});
+      p = nl + 1;
+    }
+    pmax[t] = mx;
+  });
+  if (!plain) return 0;
+  long long mx = -1;
+  size_t total = 0;
+  for (int t = 0; t < T; ++t) {
+    mx = std::max(mx, pmax[t]);
+    total += part[t].size();
+  }
+  // n counts the distinct ids (graph.hpp:297-304); the reader accepts only ids
+  // below n, that is ids 0 .. max all present
+  if (mx < 0 || (uint64_t)mx >= 2 * (uint64_t)total) return 0;
+  std::vector<uint8_t> seen((size_t)mx + 1, 0);
+  parallel_for(T, T, 1, [&](int64_t t) {
+    for (const Arc3& a : part[t]) {
+      __atomic_store_n(&seen[a.u], (uint8_t)1, __ATOMIC_RELAXED);
+      __atomic_store_n(&seen[a.v], (uint8_t)1, __ATOMIC_RELAXED);
+    }
+  });
+  std::atomic<bool> full(true);
+  parallel_for((int64_t)seen.size(), T, 1 << 16, [&](int64_t i) {
+    if (!seen[i]) full.store(false, std::memory_order_relaxed);
+  });
+  if (!full) return 0;
+  n = mx + 1;
+  join_parts(part, arcs);
+  for (int t = 0; t < T; ++t)
+    if (perr[t] != CHL_OK) err = perr[t];
   return 1;
 }
 
@@ -333,7 +474,7 @@ static bool parse_edgelist(FILE* f, int64_t& n, std::vector<Arc3>& arcs, int& er
 
 int graph_read(Graph& g, const char* path, bool undirected) {
   {
-    // DIMACS in the plain layout, on all host threads
+    // DIMACS, SNAP or an edge list in the plain layout, on all host threads
     int fd = open(path, O_RDONLY);
     if (fd < 0) return CHL_E_IO;
     struct stat st;
@@ -343,13 +484,17 @@ int graph_read(Graph& g, const char* path, bool undirected) {
         int64_t n = 0;
         int err = CHL_OK;
         std::vector<Arc3> arcs;
-        const int ok = parse_dimacs_fast((const char*)m, (size_t)st.st_size, n, arcs, err);
-        munmap(m, (size_t)st.st_size);
-        if (ok) {
+        const char* data = (const char*)m;
+        const size_t size = (size_t)st.st_size;
+        int fmt = parse_dimacs_fast(data, size, n, arcs, err) ? 1 : 0;
+        if (!fmt && parse_snap_fast(data, size, n, arcs)) fmt = 3;
+        if (!fmt && parse_edgelist_fast(data, size, n, arcs, err)) fmt = 4;
+        munmap(m, size);
+        if (fmt) {
           close(fd);
           if (err != CHL_OK) return err;
           int r = graph_from_arcs(g, n, arcs, undirected);
-          return r == CHL_OK ? 1 : r;
+          return r == CHL_OK ? fmt : r;
         }
       }
     }

@@ -106,6 +106,85 @@ def test_dimacs_layouts_match_oracle(chl, orc, tmp_path, name):
             assert np.array_equal(a, b), name
 
 
+SNAP_HEAD = "# Directed graph\n# Nodes: 4 Edges: 4\n"
+LINE_LAYOUTS = {
+    # the parallel SNAP and edge-list readers' plain layouts, and ways out of them,
+    # against the oracle (format detected, n, arcs, or rejection)
+    "snap_plain": SNAP_HEAD + "0 1\n1 2\n2 3\n3 0\n",
+    "snap_junk_after_ids": SNAP_HEAD + "0 1 junk\n1 2\t7\n2 3\n3 0\n",
+    "snap_crlf": "# Nodes: 4 Edges: 2\r\n0 1\r\n2 3\r\n",
+    "snap_comment_in_body": SNAP_HEAD + "0 1\n# note\n2 3\n",
+    "snap_second_nodes_line": SNAP_HEAD + "0 1\n# Nodes: 4 Edges: 4\n2 3\n",
+    "snap_nodes_after_comments": "# a\n# b\n# Nodes: 3 Edges: 1\n# c\n0 2\n",
+    "snap_no_nodes_line": "# Directed graph\n0 1\n1 2\n",
+    "snap_out_of_range": SNAP_HEAD + "0 4\n",
+    "snap_negative": SNAP_HEAD + "0 -1\n",
+    "snap_blank_line": SNAP_HEAD + "0 1\n\n2 3\n",
+    "snap_no_final_newline": SNAP_HEAD + "0 1\n2 3",
+    "snap_long_comment": "# " + "x" * 700 + "\n# Nodes: 4 Edges: 1\n0 1\n",
+    "snap_long_line": SNAP_HEAD + "0 1" + " " * 600 + "\n2 3\n",
+    "snap_line_511": SNAP_HEAD + "0 1" + " " * (511 - 4) + "\n2 3\n",
+    "snap_line_512": SNAP_HEAD + "0 1" + " " * (512 - 4) + "\n2 3\n",
+    "snap_huge_id": SNAP_HEAD + "0 12345678901234567890\n",
+    "snap_header_only": "# Nodes: 3 Edges: 0\n",
+    "el_plain": "0 1 5\n1 2 7\n2 0 3\n",
+    "el_fmt_like_first_line": "0 1 1\n1 2 7\n2 0 3\n",
+    "el_fmt_like_10": "2 0 10\n0 1 4\n1 2 5\n",
+    "el_signed_first": "+0 1 1\n1 2 7\n2 0 3\n",
+    "el_junk_after": "0 1 5 x\n1 2 7 y\n2 0 3\n",
+    "el_gap_in_ids": "0 1 5\n1 3 7\n3 0 3\n",
+    "el_zero_weight": "0 1 5\n1 2 0\n2 0 3\n",
+    "el_negative_id": "0 1 5\n-1 2 7\n2 0 3\n",
+    "el_two_numbers": "0 1 5\n1 2\n2 0 3\n",
+    "el_no_final_newline": "0 1 5\n1 2 7\n2 0 3",
+    "el_long_line": "0 1 5" + " " * 600 + "\n1 2 7\n2 0 3\n",
+    "el_line_512": "0 1 5" + " " * (512 - 6) + "\n1 2 7\n2 0 3\n",
+    "el_huge_weight": "0 1 5\n1 2 3000000000\n2 0 3\n",
+    "el_tabs": "0\t1\t5\n1\t2\t7\n2\t0\t3\n",
+    "el_big_id": "0 1 5\n1 3000000000 7\n",
+}
+
+
+def check_against_oracle(chl, orc, path, name):
+    og, r = orc.OGraph.read(path)
+    try:
+        g = chl.Graph.load(path)
+    except chl.ChlError as e:
+        if e.code == -4:
+            assert og is not None and min(og.csr(1)[2].min(), og.csr(0)[2].min()) <= 0, name
+        else:
+            assert og is None or og.n == 0, name
+        return
+    assert og is not None and g.n == og.n and g.format == r, name
+    for rng in (0, 1):
+        for a, b in zip(g.csr(rng), og.csr(rng)):
+            assert np.array_equal(a, b), name
+
+
+@pytest.mark.parametrize("name", sorted(LINE_LAYOUTS))
+def test_line_format_layouts_match_oracle(chl, orc, tmp_path, name):
+    check_against_oracle(chl, orc, write(tmp_path, name, LINE_LAYOUTS[name]), name)
+
+
+def test_line_formats_at_size_match_oracle(chl, orc, tmp_path):
+    """Files of 300K lines, read in ranges on every host thread."""
+    rng = np.random.default_rng(3)
+    n, m = 50000, 300000
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    u[:n] = np.arange(n)  # every id present
+    w = rng.integers(2, 1000, m)
+    w[0] = 7  # a first line the METIS header check rejects
+    el = tmp_path / "big.el"
+    el.write_text("".join(f"{a} {b} {c}\n" for a, b, c in zip(u, v, w)))
+    check_against_oracle(chl, orc, str(el), "edgelist")
+    assert chl.Graph.load(str(el)).format == 4
+    sn = tmp_path / "big.snap"
+    sn.write_text(f"# big\n# Nodes: {n} Edges: {m}\n" + "".join(f"{a}\t{b}\n" for a, b in zip(u, v)))
+    check_against_oracle(chl, orc, str(sn), "snap")
+    assert chl.Graph.load(str(sn)).format == 3
+
+
 def test_zero_weight_rejected(chl, tmp_path):
     with pytest.raises(chl.ChlError) as e:
         chl.Graph.load(write(tmp_path, "z", "p sp 2 1\na 1 2 0\n"))
