@@ -1,5 +1,5 @@
 // Graph ingest (DIMACS, METIS, SNAP, edgelist with autodetection), CSR build
-// and order files for the CHL engine. DIMACS, SNAP and edge-list files in the
+// and order files for the CHL engine. Files of any of the four formats in its
 // plain layout are parsed on all host threads; every other file takes the
 // sequential readers.
 //
@@ -208,6 +208,139 @@ static int parse_dimacs_fast(const char* data, size_t size, int64_t& n, std::vec
   return 1;
 }
 
+// A character that ends a METIS token on a line (isspace without the newline).
+inline bool metis_space(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+
+// The tokens of one METIS line [p, le): runs of digits separated by spaces,
+// each passed to f(index, value) (f returns false to stop). Returns the token
+// count, or -1 for any other character, a run over 18 digits or a stop.
+template <class F>
+int metis_tokens(const char* p, const char* le, F f) {
+  int i = 0;
+  while (p < le) {
+    if (metis_space(*p)) {
+      ++p;
+      continue;
+    }
+    if (*p < '0' || *p > '9') return -1;
+    long long x = 0;
+    int d = 0;
+    while (p < le && *p >= '0' && *p <= '9') {
+      x = x * 10 + (*p++ - '0');
+      if (++d > 18) return -1;
+    }
+    if (!f(i, x)) return -1;
+    ++i;
+  }
+  return i;
+}
+
+// A line whose first non-space character is '%' (a METIS comment, graph.hpp:220).
+inline bool metis_comment(const char* p, const char* le) {
+  while (p < le && metis_space(*p)) ++p;
+  return p < le && *p == '%';
+}
+
+// METIS on all host threads, for files in the plain layout: the first byte is
+// not 'c', 'p' or 'a' (so the DIMACS reader rejects the file), the file ends in
+// a newline, the first non-comment line is a valid header with n > 0, and every
+// later line is a comment or a vertex line that parse_metis accepts. Vertex
+// lines are numbered in a first pass, so ranges of lines parse independently.
+// Returns 1 with the arcs, n and error that parse_metis gives, or 0 when the
+// file is anything else: the sequential readers then decide.
+static int parse_metis_fast(const char* data, size_t size, int64_t& n, std::vector<Arc3>& arcs, int& err) {
+  if (size == 0 || data[size - 1] != '\n' || data[0] == 'c' || data[0] == 'p' || data[0] == 'a') return 0;
+  const char* e = data + size;
+  const char* q = data;
+  while (q < e) {  // comment lines before the header
+    const char* nl = (const char*)memchr(q, '\n', (size_t)(e - q));
+    if (!metis_comment(q, nl)) break;
+    q = nl + 1;
+  }
+  if (q >= e) return 0;
+  const char* hl = (const char*)memchr(q, '\n', (size_t)(e - q));
+  long long hv[4] = {0, 0, 0, 0};
+  const int hc = metis_tokens(q, hl, [&](int i, long long x) {
+    if (i >= 4) return false;
+    hv[i] = x;
+    return true;
+  });
+  if (hc < 1) return 0;
+  const long long nn = hv[0];
+  long long fmt = 0, skip = 0;
+  if (hc >= 3) {
+    fmt = hv[2];
+    if (fmt % 10 > 1 || (fmt / 10) % 10 > 1 || fmt > 111) return 0;
+    skip = (fmt >= 100) + (fmt % 100 >= 10);
+  }
+  if (hc == 4) {
+    if (fmt % 100 < 10 || hv[3] > 1000) return 0;
+    skip = (fmt >= 100) + hv[3];
+  }
+  if (nn <= 0 || nn > 0x7FFFFFFFll) return 0;
+  const bool weighted = fmt % 10 == 1;
+  q = hl + 1;
+  const int T = host_threads();
+  const std::vector<const char*> cut = line_cuts(q, e, T);
+  // pass 1: vertex lines per range
+  std::vector<int64_t> first(T + 1, 0);
+  parallel_for(T, T, 1, [&](int64_t t) {
+    int64_t c = 0;
+    for (const char* p = cut[t]; p < cut[t + 1];) {
+      const char* nl = (const char*)memchr(p, '\n', (size_t)(cut[t + 1] - p));
+      if (!metis_comment(p, nl)) ++c;
+      p = nl + 1;
+    }
+    first[t + 1] = c;
+  });
+  for (int t = 0; t < T; ++t) first[t + 1] += first[t];
+  // pass 2: arcs; v is the 1-based vertex of a line, as in parse_metis
+  std::vector<std::vector<Arc3>> part(T);
+  std::vector<int> perr(T, CHL_OK);
+  std::atomic<bool> plain(true);
+  parallel_for(T, T, 1, [&](int64_t t) {
+    std::vector<Arc3>& out = part[t];
+    int64_t v = first[t] + 1;
+    for (const char* p = cut[t]; p < cut[t + 1] && plain.load(std::memory_order_relaxed);) {
+      const char* nl = (const char*)memchr(p, '\n', (size_t)(cut[t + 1] - p));
+      if (metis_comment(p, nl)) {
+        p = nl + 1;
+        continue;
+      }
+      long long headv = -1;
+      const int c = metis_tokens(p, nl, [&](int i, long long x) {
+        if (i < skip) return true;
+        if (x > 0x7FFFFFFFll) return false;  // the sequential reader decides
+        if (weighted) {
+          if ((i - skip) % 2 == 0) {
+            headv = x - 1;
+            return true;
+          }
+          if (headv < 0 || headv >= nn || v - 1 >= nn) return false;
+          if (x <= 0 || x > 2147483647LL) perr[t] = CHL_E_WEIGHT;
+          out.push_back({(uint32_t)(v - 1), (uint32_t)headv, (int32_t)x});
+          return true;
+        }
+        if (x < 1 || x > nn || v - 1 >= nn) return false;
+        out.push_back({(uint32_t)(v - 1), (uint32_t)(x - 1), 1});
+        return true;
+      });
+      if (c < 0 || c < skip || (weighted && (c - skip) % 2 == 1)) {
+        plain = false;
+        break;
+      }
+      ++v;
+      p = nl + 1;
+    }
+  });
+  if (!plain) return 0;
+  n = nn;
+  join_parts(part, arcs);
+  for (int t = 0; t < T; ++t)
+    if (perr[t] != CHL_OK) err = perr[t];
+  return 1;
+}
+
 // SNAP on all host threads, for files in the plain layout: the first byte is
 // '#' (so the DIMACS and METIS readers reject the file), every line ends in a
 // newline, no NUL byte, '#' lines only before the first data line, exactly one
@@ -365,22 +498,25 @@ static bool parse_dimacs(FILE* f, int64_t& n, std::vector<Arc3>& arcs, int& err)
   return true;
 }
 
+// graph.hpp:211-257. The reference's newline flag is never cleared, so a '%'
+// anywhere starts a comment that runs to the end of the line, and that newline
+// ends no line: the next line continues the current one (a token cut by the
+// comment continues too). At the end of the file an unterminated line's tokens
+// count, but its line-end check is not made. Ids are taken as the reference's
+// 32-bit Vertex parameters take them (elem - 1, modulo 2^32).
 static bool parse_metis(FILE* f, int64_t& n, std::vector<Arc3>& arcs, int& err) {
-  int c, i = 0, fmt = 0, ncon = 0, skip = 0;
+  int c, i = 0, fmt = 0, skip = 0;
   long long elem = 0;
-  int64_t v = 0;
-  long long headv = -1;
+  uint32_t v = 0, headv = 0xFFFFFFFFu;
   bool inelem = false;
   n = 0;
   do {
     c = fgetc(f);
-    if (c == '%' && i == 0 && !inelem) {  // comment line (graph.hpp:220)
+    if (c == '%') {
       while (c != '\n' && c != EOF) c = fgetc(f);
-      continue;
-    }
-    if (isdigit(c)) {
+    } else if (isdigit(c)) {
       inelem = true;
-      elem = elem * 10 + (c - '0');
+      elem = (long long)((unsigned long long)elem * 10 + (unsigned)(c - '0'));
     } else if (c == EOF || isspace(c)) {
       if (inelem) {
         if (v == 0) {
@@ -391,28 +527,27 @@ static bool parse_metis(FILE* f, int64_t& n, std::vector<Arc3>& arcs, int& err) 
             skip = (fmt >= 100) + (fmt % 100 >= 10);
           } else if (i == 3) {
             if (fmt % 100 < 10) return false;
-            ncon = (int)elem;
-            skip = (fmt >= 100) + ncon;
+            skip = (fmt >= 100) + (int)elem;
           } else if (i > 3) return false;
         } else if (i >= skip) {
           if (fmt % 10 == 1) {
-            if ((i - skip) % 2 == 0) headv = elem - 1;
+            if ((i - skip) % 2 == 0) headv = (uint32_t)(elem - 1);
             else {
-              if (headv < 0 || headv >= n || v - 1 >= n) return false;
+              if ((int64_t)(v - 1) >= n || (int64_t)headv >= n) return false;
               if (elem <= 0 || elem > 2147483647LL) err = CHL_E_WEIGHT;
-              arcs.push_back({(uint32_t)(v - 1), (uint32_t)headv, (int32_t)elem});
+              arcs.push_back({v - 1, headv, (int32_t)elem});
             }
           } else {
-            if (elem < 1 || elem > n || v - 1 >= n) return false;
-            arcs.push_back({(uint32_t)(v - 1), (uint32_t)(elem - 1), 1});
+            const uint32_t h = (uint32_t)(elem - 1);
+            if ((int64_t)(v - 1) >= n || (int64_t)h >= n) return false;
+            arcs.push_back({v - 1, h, 1});
           }
         }
         ++i;
         elem = 0;
         inelem = false;
       }
-      if (c == '\n' || c == EOF) {
-        if (c == EOF && i == 0) break;
+      if (c == '\n') {
         if (v > 0 && (i < skip || (fmt % 10 == 1 && (i - skip) % 2 == 1))) return false;
         ++v;
         i = 0;
@@ -474,7 +609,7 @@ static bool parse_edgelist(FILE* f, int64_t& n, std::vector<Arc3>& arcs, int& er
 
 int graph_read(Graph& g, const char* path, bool undirected) {
   {
-    // DIMACS, SNAP or an edge list in the plain layout, on all host threads
+    // DIMACS, METIS, SNAP or an edge list in the plain layout, on all host threads
     int fd = open(path, O_RDONLY);
     if (fd < 0) return CHL_E_IO;
     struct stat st;
@@ -487,6 +622,7 @@ int graph_read(Graph& g, const char* path, bool undirected) {
         const char* data = (const char*)m;
         const size_t size = (size_t)st.st_size;
         int fmt = parse_dimacs_fast(data, size, n, arcs, err) ? 1 : 0;
+        if (!fmt && parse_metis_fast(data, size, n, arcs, err)) fmt = 2;
         if (!fmt && parse_snap_fast(data, size, n, arcs)) fmt = 3;
         if (!fmt && parse_edgelist_fast(data, size, n, arcs, err)) fmt = 4;
         munmap(m, size);

@@ -108,7 +108,7 @@ def test_dimacs_layouts_match_oracle(chl, orc, tmp_path, name):
 
 SNAP_HEAD = "# Directed graph\n# Nodes: 4 Edges: 4\n"
 LINE_LAYOUTS = {
-    # the parallel SNAP and edge-list readers' plain layouts, and ways out of them,
+    # the parallel METIS, SNAP and edge-list readers' plain layouts, and ways out of them,
     # against the oracle (format detected, n, arcs, or rejection)
     "snap_plain": SNAP_HEAD + "0 1\n1 2\n2 3\n3 0\n",
     "snap_junk_after_ids": SNAP_HEAD + "0 1 junk\n1 2\t7\n2 3\n3 0\n",
@@ -142,6 +142,35 @@ LINE_LAYOUTS = {
     "el_huge_weight": "0 1 5\n1 2 3000000000\n2 0 3\n",
     "el_tabs": "0\t1\t5\n1\t2\t7\n2\t0\t3\n",
     "el_big_id": "0 1 5\n1 3000000000 7\n",
+    # METIS (the parallel reader's plain layout and ways out of it)
+    "metis_plain": "% c\n4 4 1\n2 5 3 7\n1 5\n1 7 4 2\n3 2\n",
+    "metis_unweighted": "3 2\n2\n1 3\n2\n",
+    "metis_comments_between": "3 2\n% x\n2\n  % y\n1 3\n2\n",
+    "metis_blank_vertex_line": "3 1\n2\n1\n\n",
+    "metis_fewer_lines": "5 1\n2\n1\n",
+    "metis_extra_blank_lines": "2 1\n2\n1\n\n\n",
+    "metis_extra_vertex_line": "2 1\n2\n1\n1\n",
+    "metis_vertex_weights": "3 2 10\n9 2\n9 1 3\n9 2\n",
+    "metis_sizes_weights_edges": "3 2 111\n4 9 2 6\n4 9 1 6 3 8\n4 9 2 8\n",
+    "metis_ncon": "3 2 10 2\n9 9 2\n9 9 1 3\n9 9 2\n",
+    "metis_ncon_without_flag": "3 2 1 2\n2 5\n1 5\n",
+    "metis_bad_fmt": "3 2 2\n2\n1 3\n2\n",
+    "metis_odd_pairs": "3 2 1\n2 5 3\n1 5\n2 4\n",
+    "metis_zero_weight": "3 2 1\n2 0\n1 0 3 4\n2 4\n",
+    "metis_out_of_range": "3 2\n4\n1 3\n2\n",
+    "metis_zero_id": "3 2\n0\n1 3\n2\n",
+    "metis_tabs_crlf": "3 2\r\n2\r\n1\t3\r\n2\r\n",
+    "metis_percent_mid_line": "3 2\n2 % x\n1 3\n2\n",  # the next line continues vertex 1
+    "metis_percent_mid_token": "3 2\n2\n1%x\n3\n2\n",
+    "metis_percent_weighted": "3 2 1\n2 % x\n5\n1 5 3 4\n2 4\n",
+    "metis_odd_pairs_at_eof": "3 2 1\n2 5\n1 5 3 4\n2 4 1",
+    "metis_too_few_at_eof": "3 2 10\n9 2\n9 1 3\n",
+    "metis_header_comment": "3 2 % c\n2\n1 3\n2\n",
+    "metis_letter": "3 2\n2\n1 x3\n2\n",
+    "metis_no_final_newline": "3 2\n2\n1 3\n2",
+    "metis_blank_before_header": "\n3 2\n2\n1 3\n2\n",
+    "metis_huge": "3 2\n2\n1 123456789012345678901\n2\n",
+    "metis_n_zero": "0 0\n",
 }
 
 
@@ -167,7 +196,7 @@ def test_line_format_layouts_match_oracle(chl, orc, tmp_path, name):
 
 
 def test_line_formats_at_size_match_oracle(chl, orc, tmp_path):
-    """Files of 300K lines, read in ranges on every host thread."""
+    """Files of 20K to 300K lines, read in ranges on every host thread."""
     rng = np.random.default_rng(3)
     n, m = 50000, 300000
     u = rng.integers(0, n, m)
@@ -183,6 +212,21 @@ def test_line_formats_at_size_match_oracle(chl, orc, tmp_path):
     sn.write_text(f"# big\n# Nodes: {n} Edges: {m}\n" + "".join(f"{a}\t{b}\n" for a, b in zip(u, v)))
     check_against_oracle(chl, orc, str(sn), "snap")
     assert chl.Graph.load(str(sn)).format == 3
+    # METIS: weighted adjacency lines of a symmetric graph, comments sprinkled
+    g = chl.Graph.knn(20000, 3, seed=2)
+    off, head, ln = g.csr(1)
+    lines = [f"{g.n} {g.arcs(1) // 2} 1"]
+    for x in range(g.n):
+        if x % 997 == 0:
+            lines.append("% every 997th line a comment")
+        lines.append(" ".join(f"{h + 1} {w}" for h, w in zip(head[off[x]:off[x + 1]], ln[off[x]:off[x + 1]])))
+    mt = tmp_path / "big.metis"
+    mt.write_text("\n".join(lines) + "\n")
+    check_against_oracle(chl, orc, str(mt), "metis")
+    gm = chl.Graph.load(str(mt))
+    assert gm.format == 2
+    for a, b in zip(gm.csr(1), g.csr(1)):
+        assert np.array_equal(a, b)
 
 
 def test_zero_weight_rejected(chl, tmp_path):
@@ -192,11 +236,27 @@ def test_zero_weight_rejected(chl, tmp_path):
 
 
 def test_oracle_readers_agree_with_reference_binary(chl, orc, tmp_path):
+    """The oracle's readers against the reference's own hhl on every reader case of
+    this file: a file the oracle rejects makes hhl fail, with "Unable to read graph"
+    (shared_memory/hhl.cpp:46-48) or, for an edge list whose ids are not 0 .. n-1, by
+    a crash (read_edgelist has no bounds check, graph.hpp:285-305); a file it accepts
+    (n > 0, lengths > 0, where hhl runs to the end) gives the same labels."""
     if not os.path.exists(orc_mod.REF_HHL):
         pytest.skip("reference hhl not built (reference tree absent)")
-    for name, text in FIXTURES.items():
+    cases = dict(FIXTURES)
+    cases.update(DIMACS_LAYOUTS)
+    cases.update(LINE_LAYOUTS)
+    for name, text in cases.items():
         path = write(tmp_path, name, text)
         og, _ = orc.OGraph.read(path)
+        if og is None:
+            r = subprocess.run([orc_mod.REF_HHL, "-g", path, "-o", "none", "-p", "2"], capture_output=True,
+                               text=True, timeout=60)
+            assert r.returncode != 0, name
+            assert r.returncode < 0 or "Unable to read graph" in r.stderr, name
+            continue
+        if og.n == 0 or min(og.csr(1)[2].min(initial=1), og.csr(0)[2].min(initial=1)) <= 0:
+            continue  # the reference asserts on such a graph (dijkstra.hpp:91)
         order = np.arange(og.n, dtype=np.uint32)[::-1].copy()
         op = str(tmp_path / f"{name}.order")
         chl.order_write(op, order)
