@@ -177,6 +177,43 @@ def test_query_rejects_out_of_range(chl, grid):
         L.query(np.array([[0, g.n]], np.uint32))
 
 
+def test_query_device_entry_point(chl, grid):
+    """chl_query_device on device-resident pairs, for pair counts from one up to
+    several pairs per wavefront of the kernel's grid: ids >= n give -1, the rest
+    equal the oracle's queries."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")  # device buffers through the HIP runtime the library uses
+    vp = C.c_void_p
+    g, order, _, ref = grid
+    L = chl.build(g, order)
+    rng = np.random.default_rng(8)
+    for q in (1, 2, 63, 64, 65, 1000, 32767, 32768, 32769, 100003, 300000):
+        pairs = rng.integers(0, g.n, (q, 2)).astype(np.uint32)
+        bad = rng.random((q, 2)) < 0.03
+        pairs[bad] = g.n + rng.integers(0, 3, int(bad.sum())).astype(np.uint32)
+        pairs[-1, 1] = 0xFFFFFFFF if q > 2 else pairs[-1, 1]
+        ok = (pairs < g.n).all(1)
+        want = np.full(q, -1, np.int32)
+        if ok.any():
+            want[ok] = ref.query(pairs[ok])
+        got = np.full(q, -7, np.int32)
+        dp, do = vp(), vp()
+        assert hip.hipMalloc(C.byref(dp), C.c_size_t(pairs.nbytes)) == 0
+        assert hip.hipMalloc(C.byref(do), C.c_size_t(got.nbytes)) == 0
+        try:
+            assert hip.hipMemcpy(dp, pairs.ctypes.data_as(vp), C.c_size_t(pairs.nbytes), 1) == 0  # to device
+            assert hip.hipMemcpy(do, got.ctypes.data_as(vp), C.c_size_t(got.nbytes), 1) == 0
+            print("chl_query_device q =", q, flush=True)
+            assert chl.lib().chl_query_device(L._h, dp, q, do, None) == 0
+            assert hip.hipDeviceSynchronize() == 0
+            assert hip.hipMemcpy(got.ctypes.data_as(vp), do, C.c_size_t(got.nbytes), 2) == 0  # to host
+        finally:
+            hip.hipFree(dp)
+            hip.hipFree(do)
+        assert np.array_equal(got, want), q
+
+
 def test_label_file_roundtrip(chl, grid, tmp_path):
     g, order, _, ref = grid
     L = chl.build(g, order)
