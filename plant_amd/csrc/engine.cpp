@@ -242,6 +242,9 @@ struct Builder {
     for (auto& e : ev) e = nullptr;
   }
   ~Builder() {
+    // an error return can leave work queued: let it finish before the member
+    // buffers go back to the allocation cache, where another call may take them
+    if (stream) (void)hipStreamSynchronize(stream);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);

@@ -11,6 +11,14 @@ namespace chl {
 
 namespace {
 
+// Waits for a stream's queued work when it leaves scope.
+struct StreamDrain {
+  hipStream_t s;
+  ~StreamDrain() {
+    if (s) (void)hipStreamSynchronize(s);
+  }
+};
+
 int order_impl(const Graph& g, DeviceGraph& dg, int device, int64_t roots, uint64_t seed,
                std::vector<uint32_t>& order, std::vector<uint64_t>* score, double* device_ms, size_t* scratch) {
   const int64_t n = g.n;
@@ -63,6 +71,7 @@ int order_impl(const Graph& g, DeviceGraph& dg, int device, int64_t roots, uint6
   DBuf<uint32_t> qa, qb, vis, inv, droots;
   DBuf<unsigned long long> ctr;
   DBuf<uint8_t> tmp;
+  StreamDrain drain{stream};
   CHL_HIP(cell.reserve(cells));  // u32 pairs (distance, stamp)
   CHL_HIP(acc.reserve(cells));
   CHL_HIP(qa.reserve(cells));
@@ -163,6 +172,9 @@ int order_degree_device(const Graph& g, int device, uint64_t seed, bool referenc
   CHL_HIP(hipEventCreate(&e1));
   DBuf<uint64_t> off1, off0, key, val, skey, sval;
   DBuf<uint8_t> tmp;
+  // declared after the buffers, so it runs first on every return: queued work
+  // ends before the buffers go back to the allocation cache
+  StreamDrain drain{stream};
   CHL_HIP(off1.reserve(n + 1));
   CHL_HIP(off0.reserve(n + 1));
   CHL_HIP(key.reserve(n));
