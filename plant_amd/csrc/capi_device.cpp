@@ -344,10 +344,10 @@ int chl_labels_copy(const chl_labels* l, int side, uint64_t* off, uint32_t* hub,
   int rc = to_host(l, side, o, e);
   if (rc) return rc;
   memcpy(off, o.data(), sizeof(uint64_t) * o.size());
-  for (size_t k = 0; k < e.size(); ++k) {
+  parallel_for((int64_t)e.size(), host_threads(), (int64_t)1 << 20, [&](int64_t k) {
     if (hub) hub[k] = (uint32_t)(e[k] >> 32);
     if (dist) dist[k] = (int32_t)(uint32_t)e[k];
-  }
+  });
   return CHL_OK;
 }
 

@@ -1,8 +1,11 @@
-"""Parity at the configurations' own sizes, through properties that do not need the
-oracle's labels: the reference's label file for the 100x100 grid (byte-identical,
-every mode), the device digest of the label set under different batch schedules,
-and query distances against the oracle's plain Dijkstra on sampled pairs. The
-digest itself is checked against a host computation over the oracle's labels."""
+"""Parity at the configurations' own sizes: the reference's label file for the
+100x100 grid (byte-identical, every mode); at 1M vertices, the device digest of
+the label set under different batch schedules, query distances against the
+oracle's plain Dijkstra on sampled pairs, and the labels of whole hub-rank
+windows bit for bit against the oracle's PLaNT trees of those ranks (independent
+trees, so a window's oracle labels are exactly the canonical labels whose hub lies
+in it). The digest itself is checked against a host computation over the oracle's
+labels."""
 import hashlib
 import json
 import os
@@ -40,6 +43,26 @@ def oracle_dijkstra_check(orc, g, L, sources=4, targets=5000, seed=3):
     assert np.array_equal(L.query(pairs), want)
     v = rng.integers(0, g.n, 1000).astype(np.uint32)
     assert np.all(L.query(np.stack([v, v], 1)) == 0)
+
+
+def rank_window_check(orc, g, order, L, windows, threads=16):
+    """For each hub-rank window [r0, r1): the GPU labels with hub in the window
+    equal, row by row, the labels of the oracle's PLaNT trees of those ranks
+    (distributed_memory/akiba.hpp:52-104 restated)."""
+    src, dst, w = g.arc_list(1)
+    og = orc.OGraph.from_arcs(g.n, src, dst, w)
+    off, hub, dist = L.copy(0)
+    off = np.asarray(off, np.int64)
+    for r0, r1 in windows:
+        idx = np.flatnonzero((hub >= r0) & (hub < r1))
+        vert = np.searchsorted(off, idx, side="right") - 1
+        cnt = np.bincount(vert, minlength=g.n)
+        got = (np.concatenate([[0], np.cumsum(cnt)]).astype(np.uint64), hub[idx], dist[idx])
+        ref, _, _ = orc.plant_range(og, order, r0, r1, threads=threads)
+        print(f"rank window [{r0}, {r1}): {idx.size} labels", flush=True)
+        assert idx.size > 0
+        for a, b, what in zip(got, ref.copy(0), ("off", "hub", "dist")):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), (r0, r1, what)
 
 
 def test_device_digest_matches_host(chl, orc):
@@ -84,6 +107,9 @@ def test_c2_road_1M_schedules_agree_and_match_dijkstra(chl, orc):
     assert learned.stats.labels == static.stats.labels
     assert learned.checksum(0) == static.checksum(0)
     oracle_dijkstra_check(orc, g, learned)
+    del static
+    n = g.n
+    rank_window_check(orc, g, order, learned, [(0, 48), (3000, 3128), (300000, 300512), (n - 1024, n)])
 
 
 @pytest.mark.parametrize("wmax", [1, 16])
@@ -96,3 +122,7 @@ def test_c3_rmat20_hybrid_schedules_agree_and_match_dijkstra(chl, orc, wmax):
     assert learned.stats.labels == static.stats.labels
     assert learned.checksum(0) == static.checksum(0)
     oracle_dijkstra_check(orc, g, learned)
+    # windows on both sides of the switch to pruned trees
+    n, sr = g.n, max(int(learned.stats.switch_rank), 0)
+    windows = [(0, 32), (sr, sr + 256), (400000, 400512), (n - 1024, n)]
+    rank_window_check(orc, g, order, learned, windows)
