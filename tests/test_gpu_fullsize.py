@@ -31,7 +31,8 @@ def host_digest(off, hub, dist):
     return int(z.sum(dtype=np.uint64))
 
 
-def oracle_dijkstra_check(orc, g, L, sources=4, targets=5000, seed=3):
+def oracle_dijkstra_check(orc, g, L, sources=20, targets=5000, seed=3):
+    """sources x targets = 1e5 seeded pairs against the oracle's plain Dijkstra."""
     src, dst, w = g.arc_list(1)
     og = orc.OGraph.from_arcs(g.n, src, dst, w)
     rng = np.random.default_rng(seed)
@@ -80,7 +81,7 @@ def test_device_digest_matches_host(chl, orc):
     assert L.checksum(0) != host_digest(off, hub, dist)
 
 
-def test_c1_grid100_label_file_is_the_reference_file(chl, tmp_path):
+def test_c1_grid100_label_file_is_the_reference_file(chl, orc, tmp_path):
     # C1: the label file the reference's own hhl wrote for this graph and order
     # (sha256 in the manifest), from the graph file read as hhl reads it
     m = MANIFEST["grid100"]
@@ -96,6 +97,11 @@ def test_c1_grid100_label_file_is_the_reference_file(chl, tmp_path):
             lp = tmp_path / f"l{mode}{batching}.txt"
             L.write(str(lp))
             assert hashlib.sha256(lp.read_bytes()).hexdigest() == m["labels_sha256"], (mode, batching)
+    # 1e5 uniform random pairs against all-pairs Dijkstra
+    src, dst, w = g2.arc_list(1)
+    D = orc.OGraph.from_arcs(g2.n, src, dst, w).dijkstra_multi(np.arange(g2.n, dtype=np.uint32), threads=16)
+    pairs = np.random.default_rng(5).integers(0, g2.n, (100000, 2)).astype(np.uint32)
+    assert np.array_equal(L.query(pairs), D[pairs[:, 0], pairs[:, 1]])
 
 
 def test_c2_road_1M_schedules_agree_and_match_dijkstra(chl, orc):
