@@ -213,7 +213,9 @@ void chl_labels_free(chl_labels* l);
  * (distributed_memory/query_funcs.hpp:23-30): out[k] = d(pairs[2k] -> pairs[2k+1]),
  * CHL_INFTY when no hub is shared. Host pointers; synchronous. */
 int chl_query(const chl_labels* l, const uint32_t* pairs, int64_t q, int32_t* out);
-/* Same on device pointers, enqueued on `stream` (a hipStream_t; NULL = default). */
+/* Same on device pointers, enqueued on `stream` (a hipStream_t; NULL = default);
+ * returns once enqueued. A pair with an id >= n gives -1 (chl_query rejects it
+ * with CHL_E_RANGE instead). */
 int chl_query_device(const chl_labels* l, const uint32_t* d_pairs, int64_t q, int32_t* d_out, void* stream);
 
 /* Query sources and results of distributed_memory/query.hpp:18-51:
