@@ -500,8 +500,10 @@ struct Builder {
       stats->arcs_scanned += (int64_t)hc[CTR_ARCS];
       stats->prune_entries += (int64_t)hc[CTR_PRUNE_ENT];
       stats->candidates += (int64_t)hc[CTR_LABELS];
+      // per processed vertex its row bounds and state, per scanned arc the arc and
+      // the head's state, per prune probe one label entry, per label its entry
       stats->relax_bytes +=
-          16.0 * hc[CTR_PROCESSED] + 16.0 * hc[CTR_ARCS] + 8.0 * hc[CTR_PRUNE_ENT] + 16.0 * hc[CTR_LABELS];
+          16.0 * hc[CTR_PROCESSED] + 16.0 * hc[CTR_ARCS] + 8.0 * hc[CTR_PRUNE_ENT] + 8.0 * hc[CTR_LABELS];
     }
     if (hooks && hooks->on_tree_launch) hooks->on_tree_launch(hooks->ctx, side, R, (float)ms, hc);
     return CHL_OK;
