@@ -143,14 +143,15 @@ typedef struct chl_build_stats {
   int64_t batches;
   int64_t trees_plant, trees_pruned;
   int64_t switch_rank;          /* first rank built with pruned trees (-1: none) */
-  int64_t settled;              /* vertices processed over all trees */
+  int64_t settled;              /* vertices processed over all trees (re-processing included) */
   int64_t arcs_scanned;
   int64_t candidates;           /* labels emitted before cleaning */
   int64_t prune_entries;        /* label entries read by prune checks */
   int64_t clean_entries;        /* label entries read by the cleaning merge-join */
   int64_t delta_used, slots_used, batch_used;
   double relax_ms, clean_ms;    /* summed device time of the tree kernels / cleaning + append */
-  double relax_bytes;           /* algorithmic bytes of the tree kernels */
+  double relax_bytes;           /* algorithmic bytes of the tree kernels: the minimal work, 16 B per distinct
+                                   settled vertex and per arc it scans, 8 B per prune probe and per label */
   double mlp_ms;                /* device time of MLP training/inference */
   /* learned batching: mean absolute error of log1p(tree size) per root, of the MLP
      prediction and of the static estimate (the previous batch's mean), over the
@@ -164,7 +165,13 @@ typedef struct chl_build_stats {
   int64_t tree_launches;        /* tree kernel launches (relax_ms is their summed device time) */
   double clean_bytes;           /* algorithmic bytes of the cleaning merge-join */
   double clean_kernel_ms;       /* its device time */
-  double reserved[3];
+  int64_t settled_distinct;     /* distinct vertices settled over all trees (each tree's final bucket) */
+  int64_t settled_arcs;         /* arcs those vertices scan once each */
+  double relax_bytes_processed; /* relax_bytes counted over every processing instead (re-processing included) */
+  double segment_ms;            /* device time building, cleaning and merging label segments */
+  double store_bytes;           /* device bytes of the finished label store */
+  int64_t segments;             /* segments per side of the finished store */
+  double reserved[4];
 } chl_build_stats;
 
 void chl_build_opts_default(chl_build_opts* o);

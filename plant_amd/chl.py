@@ -51,7 +51,10 @@ class BuildStats(C.Structure):
                 ("pred_mae_learned", C.c_double), ("pred_mae_static", C.c_double), ("pred_roots", C.c_int64),
                 ("granularity_used", C.c_int64), ("autotune_ms", C.c_double), ("switch_trials", C.c_int64),
                 ("policy_ms", C.c_double), ("tree_launches", C.c_int64),
-                ("clean_bytes", C.c_double), ("clean_kernel_ms", C.c_double), ("reserved", C.c_double * 3)]
+                ("clean_bytes", C.c_double), ("clean_kernel_ms", C.c_double),
+                ("settled_distinct", C.c_int64), ("settled_arcs", C.c_int64), ("relax_bytes_processed", C.c_double),
+                ("segment_ms", C.c_double), ("store_bytes", C.c_double), ("segments", C.c_int64),
+                ("reserved", C.c_double * 4)]
 
     def as_dict(self):
         return {k: (getattr(self, k) if k != "reserved" else None) for k, _ in self._fields_ if k != "reserved"}

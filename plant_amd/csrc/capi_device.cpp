@@ -17,10 +17,6 @@ struct chl_labels {
   LabelStore store;
   DBuf<uint32_t> rank_of;        // vertex id -> row id
   std::vector<uint32_t> row_of;  // host copy of rank_of
-  // per vertex id the (begin, end) of its row on each side, read by queries;
-  // one table serves both sides when they are shared
-  DBuf<uint64_t> rng[2];
-  const uint64_t* d_rng(int s) const { return rng[store.shared_sides ? 0 : s].p; }
   // query staging for host-pointer calls
   DBuf<uint32_t> qpairs;
   DBuf<int32_t> qout;
@@ -32,19 +28,15 @@ struct chl_labels {
 
 namespace {
 
+// Row map and the per-segment row-range tables the queries read (per vertex id
+// the (begin, end) of its row in each segment).
 int make_handle(chl_labels* l, const std::vector<uint32_t>& row_of) {
   l->row_of = row_of;
-  CHL_HIP(l->rank_of.reserve(row_of.size()));
+  CHL_HIP(l->rank_of.reserve(std::max<size_t>(row_of.size(), 1)));
   if (!row_of.empty())
     CHL_HIP(hipMemcpy(l->rank_of.p, row_of.data(), sizeof(uint32_t) * row_of.size(), hipMemcpyHostToDevice));
   CHL_HIP(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking));
-  const uint32_t n = (uint32_t)l->store.n;
-  for (int s = 0; s < (l->store.shared_sides ? 1 : 2); ++s) {
-    CHL_HIP(l->rng[s].reserve(2 * (size_t)std::max<uint32_t>(n, 1)));
-    CHL_HIP(launch_row_ranges(l->store.d_off(s), l->rank_of.p, n, l->rng[s].p, l->stream));
-  }
-  CHL_HIP(hipStreamSynchronize(l->stream));
-  return CHL_OK;
+  return l->store.build_ranges(l->rank_of.p, l->stream);
 }
 
 // Host rows in vertex order from the device store.
@@ -66,15 +58,6 @@ int to_host(const chl_labels* l, int side, std::vector<uint64_t>& off, std::vect
   return CHL_OK;
 }
 
-int upload_side(LabelStore& S, int side, const std::vector<uint64_t>& off, const std::vector<uint64_t>& ent) {
-  CHL_HIP(S.off[side].reserve(S.n + 1));
-  CHL_HIP(hipMemcpy(S.off[side].p, off.data(), sizeof(uint64_t) * (S.n + 1), hipMemcpyHostToDevice));
-  CHL_HIP(S.ent[side].reserve(std::max<size_t>(ent.size(), 1)));
-  if (!ent.empty()) CHL_HIP(hipMemcpy(S.ent[side].p, ent.data(), sizeof(uint64_t) * ent.size(), hipMemcpyHostToDevice));
-  S.total[side] = ent.size();
-  return CHL_OK;
-}
-
 bool rows_sorted(const std::vector<uint64_t>& off, const std::vector<uint64_t>& ent) {
   for (size_t v = 0; v + 1 < off.size(); ++v)
     for (uint64_t k = off[v] + 1; k < off[v + 1]; ++k)
@@ -88,10 +71,9 @@ int from_host_labels(const Labels& H, int device, chl_labels** out) {
   int rc = ensure_device(device);
   if (rc) return rc;
   chl_labels* l = new chl_labels();
-  l->store.n = H.n;
-  l->store.device = device;
-  l->store.shared_sides = false;
-  for (int s = 0; s < 2 && rc == CHL_OK; ++s) rc = upload_side(l->store, s, H.off[s], H.ent[s]);
+  rc = l->store.init(H.n, false, device);
+  for (int s = 0; s < 2 && rc == CHL_OK; ++s) rc = l->store.upload(s, H.off[s], H.ent[s]);
+  if (rc == CHL_OK) rc = l->store.sync_views();
   std::vector<uint32_t> ident(H.n);
   for (int64_t v = 0; v < H.n; ++v) ident[v] = (uint32_t)v;
   if (rc == CHL_OK) rc = make_handle(l, ident);
@@ -106,12 +88,11 @@ int from_host_labels(const Labels& H, int device, chl_labels** out) {
 }  // namespace
 
 namespace chl {
-void labels_views(const chl_labels* l, const uint64_t** rng1, const uint64_t** ent1, const uint64_t** rng0,
-                  const uint64_t** ent0, uint32_t* n, int* device) {
-  *rng1 = l->d_rng(1);
-  *ent1 = l->store.d_ent(1);
-  *rng0 = l->d_rng(0);
-  *ent0 = l->store.d_ent(0);
+void labels_views(const chl_labels* l, const SegView** v1, uint32_t* k1, const SegView** v0, uint32_t* k0,
+                  uint32_t* n, int* device) {
+  *v1 = l->store.d_views(1);
+  *v0 = l->store.d_views(0);
+  *k1 = *k0 = l->store.nseg();
   *n = (uint32_t)l->store.n;
   *device = l->store.device;
 }
@@ -293,10 +274,9 @@ int chl_labels_load(const char* path, int device, chl_labels** out) {
   int rc = ensure_device(device);
   if (rc) return rc;
   chl_labels* l = new chl_labels();
-  l->store.n = n;
-  l->store.device = device;
-  l->store.shared_sides = h.shared != 0;
-  for (int s = 0; s < sides && rc == CHL_OK; ++s) rc = upload_side(l->store, s, off[s], ent[s]);
+  rc = l->store.init(n, h.shared != 0, device);
+  for (int s = 0; s < sides && rc == CHL_OK; ++s) rc = l->store.upload(s, off[s], ent[s]);
+  if (rc == CHL_OK) rc = l->store.sync_views();
   if (rc == CHL_OK) rc = make_handle(l, row_of);
   if (rc) {
     delete l;
@@ -354,13 +334,13 @@ int chl_labels_copy(const chl_labels* l, int side, uint64_t* off, uint32_t* hub,
 int chl_labels_row_lengths(const chl_labels* l, int side, uint32_t* len) {
   if (!l || side < 0 || side > 1 || !len) return set_error(CHL_E_INVAL, "bad arguments");
   const int64_t n = l->store.n;
-  std::vector<uint64_t> roff(n + 1);
+  std::vector<uint64_t> roff(n + 1), tot(n, 0);
   CHL_HIP(hipSetDevice(l->store.device));
-  CHL_HIP(hipMemcpy(roff.data(), l->store.d_off(side), sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost));
-  for (int64_t v = 0; v < n; ++v) {
-    const uint32_t r = l->row_of[v];
-    len[v] = (uint32_t)(roff[r + 1] - roff[r]);
+  for (auto& g : l->store.side(side)) {
+    CHL_HIP(hipMemcpy(roff.data(), g->off.p, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost));
+    for (int64_t r = 0; r < n; ++r) tot[r] += roff[r + 1] - roff[r];
   }
+  for (int64_t v = 0; v < n; ++v) len[v] = (uint32_t)tot[l->row_of[v]];
   return CHL_OK;
 }
 
@@ -369,10 +349,9 @@ int chl_labels_checksum(const chl_labels* l, int side, uint64_t* sum) {
   CHL_HIP(hipSetDevice(l->store.device));
   DBuf<unsigned long long> acc;
   CHL_HIP(acc.reserve(1));
-  CHL_HIP(hipMemset(acc.p, 0, sizeof(unsigned long long)));
-  if (l->store.n)
-    CHL_HIP(launch_checksum(l->store.d_off(side), l->store.d_ent(side), l->rank_of.p, (uint32_t)l->store.n, acc.p,
-                            nullptr));
+  CHL_HIP(launch_fill_u64((uint64_t*)acc.p, 0, 1, nullptr));
+  for (auto& g : l->store.side(side))
+    CHL_HIP(launch_checksum(g->view(), l->rank_of.p, (uint32_t)l->store.n, acc.p, nullptr));
   unsigned long long h = 0;
   CHL_HIP(hipMemcpy(&h, acc.p, sizeof(h), hipMemcpyDeviceToHost));
   *sum = h;
@@ -418,8 +397,8 @@ int chl_query_device(const chl_labels* l, const uint32_t* d_pairs, int64_t q, in
   if (!l || q < 0 || (q > 0 && (!d_pairs || !d_out))) return set_error(CHL_E_INVAL, "bad arguments");
   if (q == 0) return CHL_OK;
   CHL_HIP(hipSetDevice(l->store.device));
-  CHL_HIP(launch_query(l->d_rng(1), l->store.d_ent(1), l->d_rng(0), l->store.d_ent(0), d_pairs, (uint32_t)l->store.n,
-                       (uint64_t)q, d_out, (hipStream_t)stream));
+  CHL_HIP(launch_query(l->store.d_views(1), l->store.nseg(), l->store.d_views(0), l->store.nseg(), d_pairs,
+                       (uint32_t)l->store.n, (uint64_t)q, d_out, (hipStream_t)stream));
   return CHL_OK;
 }
 
@@ -433,7 +412,7 @@ int chl_query(const chl_labels* lc, const uint32_t* pairs, int64_t q, int32_t* o
   CHL_HIP(l->qpairs.reserve(2 * q));
   CHL_HIP(l->qout.reserve(q));
   CHL_HIP(hipMemcpyAsync(l->qpairs.p, pairs, sizeof(uint32_t) * 2 * q, hipMemcpyHostToDevice, l->stream));
-  CHL_HIP(launch_query(l->d_rng(1), l->store.d_ent(1), l->d_rng(0), l->store.d_ent(0), l->qpairs.p,
+  CHL_HIP(launch_query(l->store.d_views(1), l->store.nseg(), l->store.d_views(0), l->store.nseg(), l->qpairs.p,
                        (uint32_t)l->store.n, (uint64_t)q, l->qout.p, l->stream));
   CHL_HIP(hipMemcpyAsync(out, l->qout.p, sizeof(int32_t) * q, hipMemcpyDeviceToHost, l->stream));
   CHL_HIP(hipStreamSynchronize(l->stream));

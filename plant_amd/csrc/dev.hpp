@@ -110,6 +110,45 @@ struct DBuf {
   }
 };
 
+// ---- segmented label store ------------------------------------------------
+// A label side is a list of segments. A segment holds the labels whose hub rank
+// lies in [h0, h1) (one batch, or several merged) as a CSR over row ids (the
+// rank for built labels, the vertex id for labels read from a file), rows
+// hub-ascending. Entry formats:
+//   SEG_F5: key u32 = ((hub - h0) << 8) | (dist >> 8) and lo u8 = dist & 0xFF,
+//           5 bytes; needs hub - h0 < 2^24 and dist <= 65535. key is ascending
+//           along a row (the hub is in its top bits).
+//   SEG_F8: ent u64 = (hub << 32) | dist, 8 bytes, any value (the escape).
+// Hub ranges of the segments of a side are disjoint, so a merge-join over a
+// vertex's whole row is the minimum of the merge-joins over its segments.
+enum : uint32_t { SEG_F5 = 5, SEG_F8 = 8 };
+constexpr uint32_t kSegMax = 48;  // segments per side at most (more are merged first)
+constexpr uint32_t kF5MaxDist = 65535;
+constexpr uint32_t kF5MaxWidth = 1u << 24;
+
+struct SegView {
+  const uint64_t* off;  // row offsets, n + 1, segment-relative
+  const uint64_t* rng;  // per vertex id (begin, end) of its row; null where not built
+  const uint32_t* key;  // SEG_F5
+  const uint8_t* lo;    // SEG_F5
+  const uint64_t* ent;  // SEG_F8
+  uint32_t h0;
+  uint32_t fmt;
+};
+
+// Entry k of a segment as (hub, dist).
+__device__ __forceinline__ void seg_entry(const SegView& s, uint64_t k, uint32_t& h, uint32_t& d) {
+  if (s.fmt == SEG_F8) {
+    const uint64_t e = s.ent[k];
+    h = (uint32_t)(e >> 32);
+    d = (uint32_t)e;
+  } else {
+    const uint32_t x = s.key[k];
+    h = s.h0 + (x >> 8);
+    d = ((x & 0xFFu) << 8) | s.lo[k];
+  }
+}
+
 // Tree-kernel mode bits.
 enum : uint32_t {
   TREE_FLAGS = 1u,  // propagate the most important rank on the path (PLaNT, distributed_memory/akiba.hpp:71,89)
@@ -122,14 +161,21 @@ enum : uint32_t {
 static constexpr uint64_t kStateInf = ~0ull;
 
 // Counters of algorithmic work, summed over a launch.
+// CTR_PROCESSED / CTR_ARCS: vertices processed and arcs scanned, every
+// processing counted (a label-correcting search processes a vertex again when
+// its state improves). CTR_SETTLED / CTR_SETTLED_ARCS: distinct vertices
+// settled (final distance below the last bucket bound) and the arcs a search
+// scans when it processes each of them once: the minimal work of the trees.
 // CTR_LABELS: labels written; CTR_OVERFLOW: roots whose labels did not fit (their
 // indices go to spill[]); CTR_SPILL: labels those roots still have to write.
 // CTR_SPILL_START: smallest output offset reserved by a root that did not fit
 // (all ones when none), so [0, CTR_SPILL_START) is the written prefix.
 // CTR_SKIPPED: roots not started because the output was already full (they are
-// in spill[] too, with unknown label counts).
+// in spill[] too, with unknown label counts). CTR_MAXD: largest label distance
+// written.
 enum { CTR_PROCESSED = 0, CTR_ARCS = 1, CTR_PRUNE_ENT = 2, CTR_LABELS = 3, CTR_OVERFLOW = 4, CTR_NEXT = 5,
-       CTR_SPILL = 6, CTR_ROUNDS = 7, CTR_SPILL_START = 8, CTR_SKIPPED = 9, CTR_COUNT = 10 };
+       CTR_SPILL = 6, CTR_ROUNDS = 7, CTR_SPILL_START = 8, CTR_SKIPPED = 9, CTR_MAXD = 10, CTR_SETTLED = 11,
+       CTR_SETTLED_ARCS = 12, CTR_COUNT = 13 };
 
 struct TreeParams {
   // graph in device ids (a locality order): arcs of the range walked by the tree
@@ -139,11 +185,12 @@ struct TreeParams {
   const uint64_t* arc;
   const uint32_t* rank;
   uint32_t n;
-  // label store (rows in rank ids, hub-sorted, all hubs < prefix)
-  const uint64_t* scan_off;  // side read at a visited vertex
-  const uint64_t* scan_ent;  // (hub << 32) | dist
-  const uint64_t* root_off;  // side read at the root
-  const uint64_t* root_ent;
+  // label store (rows in rank ids, hub-sorted, all hubs < prefix): device
+  // arrays of segment views of the side read at a visited vertex and of the
+  // side read at the root
+  const SegView* scan_view;
+  const SegView* root_view;
+  uint32_t scan_nseg, root_nseg;
   uint32_t prefix;
   // batch (roots as device ids)
   const uint32_t* roots;
@@ -159,9 +206,9 @@ struct TreeParams {
   uint32_t* gkey;   // global root-label hash per slot (gcap entries) when the row is too long for LDS
   uint32_t* gval;
   uint32_t gcap;
-  // output: emitted labels, key = (vertex << 32) | hub, ent = (hub << 32) | dist
-  uint64_t* out_key;
-  uint64_t* out_ent;
+  // output: emitted labels, row id (the vertex's rank) and (hub << 32) | dist
+  uint32_t* out_row;
+  uint64_t* out_hd;
   uint64_t out_cap;
   uint32_t* spill;  // [R] indices into roots whose labels did not fit
   // per root statistics and launch counters
@@ -217,49 +264,100 @@ hipError_t launch_degree_keys(const uint64_t* off1, const uint64_t* off0, uint32
 hipError_t launch_degree_rekey(const uint64_t* off1, const uint64_t* off0, const uint64_t* val, uint32_t n,
                                uint64_t* key, hipStream_t s);
 
-// Label-store maintenance.
-// seg[v] = first index k of the sorted keys with key >> 32 >= v, for v in [0, n].
-hipError_t launch_seg_offsets(const uint64_t* keys, uint64_t m, uint32_t n, uint64_t* seg, hipStream_t s);
-// new_off[v] = old_off[v] + excl[v] for v in [0, n].
-hipError_t launch_merge_offsets(const uint64_t* old_off, const uint64_t* excl, uint32_t n, uint64_t* new_off,
-                                hipStream_t s);
-// out[0] = max over v of off[v+1] - off[v] (out must be zeroed).
-hipError_t launch_max_row(const uint64_t* off, uint32_t n, unsigned long long* out, hipStream_t s);
+// Device array allocated and freed directly (never cached): segment storage.
+template <class T>
+struct BigBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  BigBuf() = default;
+  BigBuf(const BigBuf&) = delete;
+  BigBuf& operator=(const BigBuf&) = delete;
+  ~BigBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      dev_cache_flush();
+      e = hipMalloc((void**)&p, count * sizeof(T));
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return e;
+    }
+    n = count;
+    return hipSuccess;
+  }
+};
+
+// Batch pipeline: the tree kernel writes, per label, its row id (row[k]) and
+// (hub << 32) | dist (hd[k]).
+// cnt[row[k]] += 1 for k in [0, m) (cnt zeroed first).
+hipError_t launch_count_rows(const uint32_t* row, uint64_t m, uint32_t* cnt, hipStream_t s);
+// Exclusive scan: out[i] = sum of in[0..i) for i in [0, count], so out[count]
+// is the total. tmp: scan_tmp_words(count) words.
+size_t scan_tmp_words(uint64_t count);
+hipError_t launch_scan_u32(const uint32_t* in, uint64_t count, uint64_t* out, uint64_t* tmp, hipStream_t s);
+// Each label to its row of a new segment: position = atomicAdd(cur[row], 1)
+// (cur = a copy of the segment offsets); fmt SEG_F5 (base h0) or SEG_F8.
+hipError_t launch_scatter_labels(const uint32_t* row, const uint64_t* hd, uint64_t m, uint64_t* cur, uint32_t fmt,
+                                 uint32_t h0, uint32_t* key, uint8_t* lo, uint64_t* ent, hipStream_t s);
+// Sorts every row of a segment by hub. longq: n + 1 words of scratch (a list
+// of the rows too long for one thread).
+hipError_t launch_sort_rows(const uint64_t* off, uint32_t rows, uint32_t fmt, uint32_t* key, uint8_t* lo,
+                            uint64_t* ent, uint32_t* longq, hipStream_t s);
+// In-batch cleaning (shared_memory/labeling.hpp:78-88 over the batch labels):
+// keep[k] = 0 when entry k = (h, d) of row v of seg (this side) is covered by a
+// common hub x < h of row v and row h of oseg (the other side of the batch):
+// L(v)[x] + L(h)[x] <= d. One wavefront per row. ctr[CTR_PRUNE_ENT] += entries
+// of the other rows read.
+hipError_t launch_clean(const SegView& seg, const SegView& oseg, uint32_t rows, uint8_t* keep,
+                        unsigned long long* ctr, hipStream_t s);
+// cnt[r] = kept entries of row r.
+hipError_t launch_count_kept(const uint64_t* off, const uint8_t* keep, uint32_t rows, uint32_t* cnt, hipStream_t s);
+// The kept entries of every row into a new segment of the same format (offsets
+// noff); row order and entry order are kept.
+hipError_t launch_copy_kept(const SegView& seg, const uint8_t* keep, uint32_t rows, const uint64_t* noff,
+                            uint32_t* key, uint8_t* lo, uint64_t* ent, hipStream_t s);
+// Merge of adjacent segments (ascending hub ranges) into one: cnt[r] = sum of
+// the row lengths (row_total), then rows concatenated in segment order
+// (concat_rows), each entry re-encoded to the output format. views: device
+// array of k views.
+hipError_t launch_row_total(const SegView* views, uint32_t k, uint32_t rows, uint32_t* cnt, hipStream_t s);
+hipError_t launch_concat_rows(const SegView* views, uint32_t k, uint32_t rows, const uint64_t* noff, uint32_t fmt,
+                              uint32_t h0, uint32_t* key, uint8_t* lo, uint64_t* ent, hipStream_t s);
+// out[0] = max over rows of the summed row length over the k segments (out zeroed).
+hipError_t launch_max_row(const SegView* views, uint32_t k, uint32_t rows, unsigned long long* out, hipStream_t s);
 // Switch test of the hybrid build (distributed_memory/akiba.hpp:297-303): sets
 // *flag when some root's tree_size / label_count exceeds ratio.
 hipError_t launch_switch_test(const uint32_t* tree_size, const uint32_t* label_count, uint32_t R, double ratio,
                               unsigned long long* flag, hipStream_t s);
-// In-place merge of one vertex block [a, b): its old rows, copied to tmp from
-// old_off[a] on, go to new_off[v], each followed by the batch's rows; blocks are
-// merged from the last one down. And a plain device copy.
-hipError_t launch_place_rows(const uint64_t* old_off, const uint64_t* tmp, uint64_t tmp_base, const uint64_t* new_off,
-                             const uint64_t* excl, const uint64_t* bat, uint32_t a, uint32_t b, uint64_t* ent,
-                             hipStream_t s);
-hipError_t launch_copy_u64(const uint64_t* src, uint64_t count, uint64_t* dst, hipStream_t s);
-// *out += sum over vertices v and entries e of row row_of[v] of mix64(e ^ v * golden)
-// (out must be zeroed): an order-independent digest of one label side.
-hipError_t launch_checksum(const uint64_t* off, const uint64_t* ent, const uint32_t* row_of, uint32_t n,
-                           unsigned long long* out, hipStream_t s);
-// In-batch cleaning (shared_memory/labeling.hpp:78-88 over the batch labels, which
-// are sorted by (vertex, hub)): keep[k] = 0 when candidate k is covered.
-hipError_t launch_clean(const uint64_t* key, const uint64_t* ent, uint64_t m, const uint64_t* seg_off,
-                        const uint64_t* oent, const uint64_t* oseg_off, uint8_t* keep, unsigned long long* ctr,
-                        hipStream_t s);
-// Label merge-join query (shared_memory/labeling.hpp:56-65). pairs hold vertex
-// ids; rng1 / rng0 hold per vertex id the (begin, end) of its side-1 / side-0
-// row (launch_row_ranges); an id >= n gives -1.
-hipError_t launch_query(const uint64_t* rng1, const uint64_t* ent1, const uint64_t* rng0, const uint64_t* ent0,
-                        const uint32_t* pairs, uint32_t n, uint64_t q, int32_t* out, hipStream_t s);
+// *out += sum over vertices v and entries (h, d) of row row_of[v] of a segment
+// of mix64(((h << 32) | d) ^ v * golden) (out zeroed first): an order-
+// independent digest; summed over the segments it is the side's digest.
+hipError_t launch_checksum(const SegView& seg, const uint32_t* row_of, uint32_t n, unsigned long long* out,
+                           hipStream_t s);
 // rng[2 v], rng[2 v + 1] = off[row_of[v]], off[row_of[v] + 1] for v in [0, n).
 hipError_t launch_row_ranges(const uint64_t* off, const uint32_t* row_of, uint32_t n, uint64_t* rng, hipStream_t s);
+// (hub << 32) | dist of every entry of a segment, in entry order.
+hipError_t launch_decode(const SegView& seg, uint64_t count, uint64_t* out, hipStream_t s);
+// Label merge-join query (shared_memory/labeling.hpp:56-65) over segmented
+// sides. v1 / v0: device arrays of the side-1 / side-0 views (rng set), k1 / k0
+// segments; pairs hold vertex ids; an id >= n gives -1.
+hipError_t launch_query(const SegView* v1, uint32_t k1, const SegView* v0, uint32_t k0, const uint32_t* pairs,
+                        uint32_t n, uint64_t q, int32_t* out, hipStream_t s);
+hipError_t launch_copy_u64(const uint64_t* src, uint64_t count, uint64_t* dst, hipStream_t s);
 
-// Library primitives (hipCUB): radix sort of 64-bit key/value pairs, exclusive
-// scan of u32 counts into u64 offsets, stream compaction by flags.
+// Library primitive (hipCUB) used by the device degree order: radix sort of
+// 64-bit key/value pairs.
 hipError_t sort_pairs_u64(void* tmp, size_t& tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint64_t* vin,
                           uint64_t* vout, uint64_t m, int end_bit, hipStream_t s);
-hipError_t exclusive_scan_u32_u64(void* tmp, size_t& tmp_bytes, const uint32_t* in, uint64_t* out, uint64_t m,
-                                  hipStream_t s);
-hipError_t select_flagged_u64(void* tmp, size_t& tmp_bytes, const uint64_t* in, const uint8_t* flags, uint64_t* out,
-                              uint64_t* nsel, uint64_t m, hipStream_t s);
 
 }  // namespace chl

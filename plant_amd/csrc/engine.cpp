@@ -11,12 +11,15 @@
 //     u32[n] for the current label build (copied into the search cells). A
 //     symmetric graph keeps one range, since both trees of a root are then the
 //     same.
-//   * label store per side, in rank ids: off u64[n+1], ent u64[T] =
-//     (hub << 32) | dist, rows hub-ascending; a symmetric graph stores side 0
-//     only (side 1 is identical).
+//   * label store per side, in rank ids: a list of segments, one per batch
+//     (merged as they accumulate), each a CSR over rank rows with 5-byte
+//     entries (8 bytes where a distance or the hub span does not fit); a
+//     symmetric graph stores side 0 only (side 1 is identical).
 //   * per-slot search scratch (tree.hip), S slots x 36 n bytes.
-//   * batch output: key (vertex << 32 | hub), ent; sorted, cleaned (GLL) and
-//     appended to the store after every batch.
+//   * batch output: row u32 and (hub << 32) | dist u64 per label (12 bytes);
+//     counted per row, scattered into the batch's new segment, rows sorted,
+//     cleaned (GLL) and appended to the store as one segment: no existing row
+//     moves.
 //
 // A batch is a contiguous rank range [r0, r1). Its trees read only labels of
 // hubs < r0, which are final, so the labels do not depend on how roots are
@@ -159,36 +162,17 @@ int DeviceGraph::build(const Graph& g, int dev) {
   return CHL_OK;
 }
 
-int LabelStore::init(int64_t nn, bool shared, int dev) {
-  n = nn;
-  shared_sides = shared;
-  device = dev;
-  for (int s = 0; s < (shared ? 1 : 2); ++s) {
-    CHL_HIP(off[s].reserve(n + 1));
-    CHL_HIP(hipMemset(off[s].p, 0, sizeof(uint64_t) * (n + 1)));
-    CHL_HIP(ent[s].reserve(1));
-    total[s] = 0;
-  }
-  return CHL_OK;
-}
-
-int LabelStore::download(int side, std::vector<uint64_t>& hoff, std::vector<uint64_t>& hent) const {
-  const int s = shared_sides ? 0 : side;
-  hoff.resize(n + 1);
-  CHL_HIP(hipMemcpy(hoff.data(), off[s].p, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost));
-  hent.resize(hoff[n]);
-  if (hoff[n]) CHL_HIP(hipMemcpy(hent.data(), ent[s].p, sizeof(uint64_t) * hoff[n], hipMemcpyDeviceToHost));
-  return CHL_OK;
-}
-
 // ---------------------------------------------------------------------------
 // Builder
 // ---------------------------------------------------------------------------
 struct SideBatch {
-  DBuf<uint64_t> key, ent, skey, sent, seg, ckey, cent;
-  DBuf<uint8_t> keep;
+  DBuf<uint32_t> row;  // tree output: row id per label
+  DBuf<uint64_t> hd;   // (hub << 32) | dist per label
+  DBuf<uint8_t> keep;  // GLL cleaning flags
+  std::unique_ptr<Segment> seg;  // the batch's new segment
   uint64_t m = 0;       // labels emitted (before cleaning)
   uint64_t kept = 0;    // labels appended
+  uint32_t maxd = 0;    // largest label distance
 };
 
 // Search slots for a granularity: base_slots workgroups of 256 threads, as
@@ -223,12 +207,12 @@ struct Builder {
   int64_t stats_batches = 0;
   DBuf<unsigned long long> ctr;
   DBuf<unsigned long long> misc;  // [0] switch flag, [1] max row
-  DBuf<uint64_t> noff;
-  DBuf<uint64_t> mtmp;          // staging of the in-place merge
-  std::vector<uint64_t> ho;     // host copy of the store offsets
-  static constexpr uint64_t kMergeTmp = (uint64_t)1 << 26;  // entries staged per block
-  DBuf<uint8_t> tmp;
+  // segment building: per-row counts, scan scratch, scatter cursors, the list
+  // of long rows of the row sort
+  DBuf<uint32_t> cnt, longq;
+  DBuf<uint64_t> scan_tmp, cur;
   SideBatch sb[2];
+  static constexpr double kBatchBytes = 32.0;  // device bytes per batch label in flight
   uint64_t out_cap = 0;
   uint64_t next_need = 0;  // output estimate of the next batch (labels)
   uint64_t out_limit = ~0ull;  // output capacity the memory cap allows for the current batch
@@ -299,17 +283,18 @@ struct Builder {
   }
 
   int ensure_out(SideBatch& b, uint64_t cap) {
-    CHL_HIP(b.key.reserve(cap));
-    CHL_HIP(b.ent.reserve(cap));
+    CHL_HIP(b.row.reserve(cap));
+    CHL_HIP(b.hd.reserve(cap));
     return CHL_OK;
   }
 
   // Grows a batch output buffer to cap entries keeping its first `used` entries.
-  int grow_keep(DBuf<uint64_t>& buf, uint64_t cap, uint64_t used) {
+  template <class T>
+  int grow_keep(DBuf<T>& buf, uint64_t cap, uint64_t used) {
     if (buf.n >= cap) return CHL_OK;
-    DBuf<uint64_t> nb;
+    DBuf<T> nb;
     CHL_HIP(nb.reserve(cap));
-    if (used) CHL_HIP(hipMemcpyAsync(nb.p, buf.p, sizeof(uint64_t) * used, hipMemcpyDeviceToDevice, stream));
+    if (used) CHL_HIP(hipMemcpyAsync(nb.p, buf.p, sizeof(T) * used, hipMemcpyDeviceToDevice, stream));
     buf.swap(nb);
     CHL_HIP(hipStreamSynchronize(stream));
     return CHL_OK;
@@ -334,17 +319,9 @@ struct Builder {
     };
     CHL_HIP(hipStreamSynchronize(stream));
     for (auto& b : sb) {
-      trim(b.key);
-      trim(b.ent);
-      trim(b.skey);
-      trim(b.sent);
-      trim(b.ckey);
-      trim(b.cent);
+      trim(b.row);
+      trim(b.hd);
       trim(b.keep);
-    }
-    if (tmp.n > 16 * keep) {
-      freed += tmp.bytes;
-      tmp.release();
     }
     if (freed) {
       out_cap = std::min<uint64_t>(out_cap, keep);
@@ -369,10 +346,9 @@ struct Builder {
     p.off = dg.off[sym ? 1 : range].p;
     p.arc = dg.arc[sym ? 1 : range].p;
     p.n = (uint32_t)n;
-    p.scan_off = L.d_off(side);
-    p.scan_ent = L.d_ent(side);
-    p.root_off = L.d_off(1 - side);
-    p.root_ent = L.d_ent(1 - side);
+    p.scan_view = L.d_views(side);
+    p.root_view = L.d_views(1 - side);
+    p.scan_nseg = p.root_nseg = L.nseg();
     p.prefix = prefix;
     p.roots = roots.p;
     p.R = R;
@@ -386,9 +362,9 @@ struct Builder {
     p.gkey = gkey.p;
     p.gval = gval.p;
     p.gcap = gcap;
-    p.out_key = b.key.p;
-    p.out_ent = b.ent.p;
-    p.out_cap = std::min<uint64_t>(b.key.n, b.ent.n);
+    p.out_row = b.row.p;
+    p.out_hd = b.hd.p;
+    p.out_cap = std::min<uint64_t>(b.row.n, b.hd.n);
     p.spill = spill.p;
     p.tree_size = tree_size.p;
     p.touched = o.verbose ? touched.p : nullptr;
@@ -447,9 +423,9 @@ struct Builder {
       const double est = std::min((double)freeb / 48.0, 1.5 * mean * (double)skipped);
       const uint64_t need = written + hc[CTR_SPILL] + (uint64_t)est + 4096;
       out_cap = std::max<uint64_t>(out_cap, need + need / 8);
-      rc = grow_keep(b.key, out_cap, written);
+      rc = grow_keep(b.row, out_cap, written);
       if (rc) return rc;
-      rc = grow_keep(b.ent, out_cap, written);
+      rc = grow_keep(b.hd, out_cap, written);
       if (rc) return rc;
       hids.resize(ns);
       for (uint32_t j = 0; j < ns; ++j) hids[j] = id_of_rank[cur[j]];
@@ -457,8 +433,8 @@ struct Builder {
       CHL_HIP(hipMemcpy(origd.p, orig.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
       p.roots = roots2.p;
       p.R = ns;
-      p.out_key = b.key.p;
-      p.out_ent = b.ent.p;
+      p.out_row = b.row.p;
+      p.out_hd = b.hd.p;
       p.out_cap = out_cap;
       p.tree_size = tree_size.p + W;  // scratch, scattered back after the launch
       p.label_count = label_count.p + W;
@@ -495,14 +471,21 @@ struct Builder {
               wfit[1] / wsum, wfit[2] / wsum, wfit[3] / wsum);
     }
     b.m = hc[CTR_LABELS];
+    b.maxd = (uint32_t)hc[CTR_MAXD];
     if (stats) {
       stats->settled += (int64_t)hc[CTR_PROCESSED];
       stats->arcs_scanned += (int64_t)hc[CTR_ARCS];
+      stats->settled_distinct += (int64_t)hc[CTR_SETTLED];
+      stats->settled_arcs += (int64_t)hc[CTR_SETTLED_ARCS];
       stats->prune_entries += (int64_t)hc[CTR_PRUNE_ENT];
       stats->candidates += (int64_t)hc[CTR_LABELS];
-      // per processed vertex its row bounds and state, per scanned arc the arc and
-      // the head's state, per prune probe one label entry, per label its entry
-      stats->relax_bytes +=
+      // the minimal work of the trees: per distinct settled vertex its row
+      // bounds and state, per arc it scans (once) the arc and the head's
+      // state, per prune probe one label entry, per label its entry
+      stats->relax_bytes += 16.0 * hc[CTR_SETTLED] + 16.0 * hc[CTR_SETTLED_ARCS] + 8.0 * hc[CTR_PRUNE_ENT] +
+                            8.0 * hc[CTR_LABELS];
+      // the same counted over every processing, re-processing included
+      stats->relax_bytes_processed +=
           16.0 * hc[CTR_PROCESSED] + 16.0 * hc[CTR_ARCS] + 8.0 * hc[CTR_PRUNE_ENT] + 8.0 * hc[CTR_LABELS];
     }
     if (hooks && hooks->on_tree_launch) hooks->on_tree_launch(hooks->ctx, side, R, (float)ms, hc);
@@ -572,128 +555,110 @@ struct Builder {
     return CHL_OK;
   }
 
-  int sort_side(int side, double& ms) {
-    SideBatch& b = sb[side];
-    const uint64_t m = b.m;
-    CHL_HIP(b.skey.reserve(std::max<uint64_t>(m, 1)));
-    CHL_HIP(b.sent.reserve(std::max<uint64_t>(m, 1)));
-    CHL_HIP(b.seg.reserve(n + 1));
-    CHL_HIP(hipEventRecord(ev[2], stream));
-    if (m) {
-      int bits = 0;
-      while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)n) ++bits;
-      size_t tb = 0;
-      CHL_HIP(sort_pairs_u64(nullptr, tb, b.key.p, b.skey.p, b.ent.p, b.sent.p, m, 32 + bits, stream));
-      CHL_HIP(tmp.reserve(tb));
-      CHL_HIP(sort_pairs_u64(tmp.p, tb, b.key.p, b.skey.p, b.ent.p, b.sent.p, m, 32 + bits, stream));
-    }
-    CHL_HIP(launch_seg_offsets(b.skey.p, m, (uint32_t)n, b.seg.p, stream));
-    CHL_HIP(hipEventRecord(ev[3], stream));
-    CHL_HIP(hipEventSynchronize(ev[3]));
+  // ms += the device time between two events recorded on the stream.
+  int elapsed(hipEvent_t a, hipEvent_t b, double& ms) {
+    CHL_HIP(hipEventSynchronize(b));
     float t = 0;
-    CHL_HIP(hipEventElapsedTime(&t, ev[2], ev[3]));
+    CHL_HIP(hipEventElapsedTime(&t, a, b));
     ms += t;
+    if (stats) stats->segment_ms += t;
     return CHL_OK;
   }
 
-  // GLL cleaning of one side against the batch labels of both sides.
+  // The batch's labels of one side into a new segment for the hub range
+  // [r0, r0 + R): counted per row, offsets by a scan, scattered into the rows,
+  // rows sorted by hub. O(batch): no stored row moves. SEG_F5 unless a distance
+  // exceeds 65535 or the range is wider than 2^24 ranks.
+  int build_segment(int side, uint32_t r0, uint32_t R, double& ms) {
+    SideBatch& b = sb[side];
+    const uint64_t m = b.m;
+    CHL_HIP(hipEventRecord(ev[2], stream));
+    std::unique_ptr<Segment> g(new Segment());
+    g->h0 = r0;
+    g->h1 = r0 + R;
+    const bool f5 = b.maxd <= kF5MaxDist && R <= kF5MaxWidth;
+    int rc = g->alloc((uint64_t)n, m, f5 ? SEG_F5 : SEG_F8);
+    if (rc) return rc;
+    CHL_HIP(launch_fill_u32(cnt.p, 0, (size_t)n, stream));
+    CHL_HIP(launch_count_rows(b.row.p, m, cnt.p, stream));
+    CHL_HIP(launch_scan_u32(cnt.p, (uint64_t)n, g->off.p, scan_tmp.p, stream));
+    if (m) {
+      CHL_HIP(launch_copy_u64(g->off.p, (uint64_t)n, cur.p, stream));
+      CHL_HIP(launch_scatter_labels(b.row.p, b.hd.p, m, cur.p, g->fmt, g->h0, g->key.p, g->lo.p, g->ent.p, stream));
+      CHL_HIP(launch_sort_rows(g->off.p, (uint32_t)n, g->fmt, g->key.p, g->lo.p, g->ent.p, longq.p, stream));
+    }
+    CHL_HIP(hipEventRecord(ev[3], stream));
+    b.seg = std::move(g);
+    return elapsed(ev[2], ev[3], ms);
+  }
+
+  // GLL cleaning of one side's new segment against the batch labels of both
+  // sides (clean_cover over the superstep's labels, shared_memory/akiba.hpp:
+  // 197-208 with labeling.hpp:78-88).
   int clean_side(int side, double& ms) {
     SideBatch& b = sb[side];
     SideBatch& ob = sb[sym ? side : 1 - side];
     const uint64_t m = b.m;
-    if (m) {
-      CHL_HIP(b.keep.reserve(m));
-      CHL_HIP(b.ckey.reserve(m));
-      CHL_HIP(b.cent.reserve(m));
-      CHL_HIP(launch_fill_u64((uint64_t*)ctr.p + CTR_PRUNE_ENT, 0, 1, stream));
-    }
+    if (!m) return CHL_OK;
+    CHL_HIP(b.keep.reserve(m));
+    CHL_HIP(launch_fill_u64((uint64_t*)ctr.p + CTR_PRUNE_ENT, 0, 1, stream));
     CHL_HIP(hipEventRecord(ev[4], stream));
-    if (m) CHL_HIP(launch_clean(b.skey.p, b.sent.p, m, b.seg.p, ob.sent.p, ob.seg.p, b.keep.p, ctr.p, stream));
+    CHL_HIP(launch_clean(b.seg->view(), ob.seg->view(), (uint32_t)n, b.keep.p, ctr.p, stream));
     CHL_HIP(hipEventRecord(ev[5], stream));
-    CHL_HIP(hipEventSynchronize(ev[5]));
-    float t = 0;
-    CHL_HIP(hipEventElapsedTime(&t, ev[4], ev[5]));
+    double t = 0;
+    int rc = elapsed(ev[4], ev[5], t);
+    if (rc) return rc;
     ms += t;
-    if (m && stats) {
-      // merge steps (two entries each), plus the candidate's key, entry and flag
-      unsigned long long steps = 0;
-      CHL_HIP(hipMemcpy(&steps, ctr.p + CTR_PRUNE_ENT, sizeof(steps), hipMemcpyDeviceToHost));
-      stats->clean_entries += (int64_t)(2 * steps);
-      stats->clean_bytes += 16.0 * (double)steps + 17.0 * (double)m;
+    if (stats) {
+      // per candidate: its entry, its flag and the two offsets of the other
+      // row; per entry of the other rows read: the entry
+      unsigned long long reads = 0;
+      CHL_HIP(hipMemcpy(&reads, ctr.p + CTR_PRUNE_ENT, sizeof(reads), hipMemcpyDeviceToHost));
+      const double eb = b.seg->fmt == SEG_F8 ? 8.0 : 5.0, oeb = ob.seg->fmt == SEG_F8 ? 8.0 : 5.0;
+      stats->clean_entries += (int64_t)reads;
+      stats->clean_bytes += (eb + 1.0 + 16.0) * (double)m + oeb * (double)reads;
       stats->clean_kernel_ms += t;
     }
     return CHL_OK;
   }
 
+  // Drops the covered candidates: a new segment of the kept entries.
   int compact_side(int side, double& ms) {
     SideBatch& b = sb[side];
-    const uint64_t m = b.m;
-    if (!m) return CHL_OK;
+    if (!b.m) return CHL_OK;
     CHL_HIP(hipEventRecord(ev[4], stream));
-    DBuf<uint64_t> nsel;
-    CHL_HIP(nsel.reserve(1));
-    size_t tb = 0;
-    CHL_HIP(select_flagged_u64(nullptr, tb, b.skey.p, b.keep.p, b.ckey.p, nsel.p, m, stream));
-    CHL_HIP(tmp.reserve(tb));
-    CHL_HIP(select_flagged_u64(tmp.p, tb, b.skey.p, b.keep.p, b.ckey.p, nsel.p, m, stream));
-    CHL_HIP(select_flagged_u64(tmp.p, tb, b.sent.p, b.keep.p, b.cent.p, nsel.p, m, stream));
+    const Segment& o = *b.seg;
+    CHL_HIP(launch_count_kept(o.off.p, b.keep.p, (uint32_t)n, cnt.p, stream));
+    CHL_HIP(launch_scan_u32(cnt.p, (uint64_t)n, cur.p, scan_tmp.p, stream));
     uint64_t kept = 0;
-    CHL_HIP(hipMemcpyAsync(&kept, nsel.p, sizeof(kept), hipMemcpyDeviceToHost, stream));
+    CHL_HIP(hipMemcpyAsync(&kept, cur.p + n, sizeof(kept), hipMemcpyDeviceToHost, stream));
     CHL_HIP(hipStreamSynchronize(stream));
-    b.skey.swap(b.ckey);
-    b.sent.swap(b.cent);
-    b.m = kept;
-    CHL_HIP(launch_seg_offsets(b.skey.p, kept, (uint32_t)n, b.seg.p, stream));
+    std::unique_ptr<Segment> g(new Segment());
+    g->h0 = o.h0;
+    g->h1 = o.h1;
+    int rc = g->alloc((uint64_t)n, kept, o.fmt);
+    if (rc) return rc;
+    CHL_HIP(launch_copy_u64(cur.p, (uint64_t)n + 1, g->off.p, stream));
+    CHL_HIP(launch_copy_kept(o.view(), b.keep.p, (uint32_t)n, g->off.p, g->key.p, g->lo.p, g->ent.p, stream));
     CHL_HIP(hipEventRecord(ev[5], stream));
-    CHL_HIP(hipEventSynchronize(ev[5]));
-    float t = 0;
-    CHL_HIP(hipEventElapsedTime(&t, ev[4], ev[5]));
-    ms += t;
+    rc = elapsed(ev[4], ev[5], ms);
+    if (rc) return rc;
+    b.seg = std::move(g);
+    b.m = kept;
     return CHL_OK;
   }
 
-  // Appends the batch's sorted labels (segments b.seg, entries b.sent) to the
-  // store rows in place: the store grows without moving, then its rows are
-  // shifted up block by block from the last vertex down (each block's old rows
-  // staged in a temporary first), so the store is never held twice.
-  int merge_side(int side, LabelStore& L, double& ms) {
-    SideBatch& b = sb[side];
-    const uint64_t m = b.m;
-    b.kept = m;
-    if (!m) return CHL_OK;
+  // Appends the batch's segments to the store (no stored row moves), then lets
+  // the store merge small segments.
+  int append_batch(LabelStore& L, double& ms) {
+    for (int s = 0; s < nsides; ++s) sb[s].kept = sb[s].m;
     CHL_HIP(hipEventRecord(ev[6], stream));
-    const uint64_t oldt = L.total[side], newt = oldt + m;
-    CHL_HIP(L.ent[side].reserve(newt, oldt, stream));
-    CHL_HIP(noff.reserve(n + 1));
-    CHL_HIP(launch_merge_offsets(L.off[side].p, b.seg.p, (uint32_t)n, noff.p, stream));
-    if (oldt == 0) {
-      CHL_HIP(launch_copy_u64(b.sent.p, m, L.ent[side].p, stream));
-    } else {
-      ho.resize(n + 1);
-      CHL_HIP(hipMemcpyAsync(ho.data(), L.off[side].p, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, stream));
-      CHL_HIP(hipStreamSynchronize(stream));
-      const uint64_t cap = std::min<uint64_t>(kMergeTmp, oldt);
-      CHL_HIP(mtmp.reserve(cap));
-      for (uint64_t hi = (uint64_t)n; hi > 0;) {
-        // the widest block [lo, hi) whose old rows fit the temporary
-        uint64_t lo = (uint64_t)(std::lower_bound(ho.begin(), ho.begin() + hi, ho[hi] - std::min(ho[hi], cap)) -
-                                 ho.begin());
-        if (lo == hi) lo = hi - 1;  // one row longer than the temporary: cannot happen (rows < n entries)
-        if (ho[hi] - ho[lo] > cap) return set_error(CHL_E_RANGE, "label row of %llu entries", (unsigned long long)(ho[hi] - ho[lo]));
-        CHL_HIP(launch_copy_u64(L.ent[side].p + ho[lo], ho[hi] - ho[lo], mtmp.p, stream));
-        CHL_HIP(launch_place_rows(L.off[side].p, mtmp.p, ho[lo], noff.p, b.seg.p, b.sent.p, (uint32_t)lo,
-                                  (uint32_t)hi, L.ent[side].p, stream));
-        hi = lo;
-      }
-    }
-    L.off[side].swap(noff);
-    L.total[side] = newt;
+    int rc = L.append(std::move(sb[0].seg), nsides == 2 ? std::move(sb[1].seg) : nullptr);
+    if (rc) return rc;
+    rc = L.tidy(0, stream);
+    if (rc) return rc;
     CHL_HIP(hipEventRecord(ev[7], stream));
-    CHL_HIP(hipEventSynchronize(ev[7]));
-    float t = 0;
-    CHL_HIP(hipEventElapsedTime(&t, ev[6], ev[7]));
-    ms += t;
-    return CHL_OK;
+    return elapsed(ev[6], ev[7], ms);
   }
 
   int run(LabelStore& L) {
@@ -738,16 +703,20 @@ struct Builder {
     CHL_HIP(ctr.reserve(CTR_COUNT));
     CHL_HIP(misc.reserve(4));
     CHL_HIP(launch_fill_u64((uint64_t*)misc.p, 0, 4, stream));
+    CHL_HIP(cnt.reserve(std::max<int64_t>(n, 1)));
+    CHL_HIP(cur.reserve(n + 1));
+    CHL_HIP(longq.reserve(n + 1));
+    CHL_HIP(scan_tmp.reserve(scan_tmp_words((uint64_t)n)));
     // first batch: the top roots label a large part of the graph
     if (o.out_capacity > 0) {
       out_cap = (uint64_t)o.out_capacity;
     } else {
       // the top roots label a large part of the graph: width * n / 4, bounded
-      // by an eighth of the free memory (16 bytes per label)
+      // by an eighth of the free memory (12 bytes per label)
       size_t freeb = 0, totalb = 0;
       CHL_HIP(hipMemGetInfo(&freeb, &totalb));
       const uint64_t want = (uint64_t)width * (uint64_t)n / 4;
-      out_cap = std::max<uint64_t>((uint64_t)1 << 20, std::min<uint64_t>(want, freeb / 8 / 16));
+      out_cap = std::max<uint64_t>((uint64_t)1 << 20, std::min<uint64_t>(want, freeb / 8 / 12));
     }
     if (hooks) hooks->slots = slots;
     rc = L.init(n, sym, o.device);
@@ -794,17 +763,18 @@ struct Builder {
       rc = trim_batch_buffers(next_need);
       if (rc) return rc;
       {
-        // memory cap under either policy: a batch's labels (output, sorted
-        // copy, sort scratch: about 96 bytes each) within the free memory, at
+        // memory cap under either policy: a batch's labels (output 12 bytes,
+        // new segment up to 8, cleaning flag and compacted copy 9: kBatchBytes
+        // each) within the free memory, at
         // 1.5 times the labels per root of the last batch's highest eighth of
         // ranks (labels per root fall with rank); before any batch, a root
         // may label every vertex
         size_t freeb = 0, totalb = 0;
         CHL_HIP(hipMemGetInfo(&freeb, &totalb));
         const double per_root = tail_labels >= 0 ? std::max(1.0, 1.5 * tail_labels) : (double)n;
-        const double cap = (double)freeb / 96.0 / per_root / nsides;
+        const double cap = (double)freeb / kBatchBytes / per_root / nsides;
         bw = (uint32_t)std::max<double>(1.0, std::min<double>(bw, cap));
-        out_limit = std::max<uint64_t>((uint64_t)1 << 20, (uint64_t)(freeb / 96 / nsides));
+        out_limit = std::max<uint64_t>((uint64_t)1 << 20, (uint64_t)(freeb / kBatchBytes / nsides));
       }
       const uint32_t R = (uint32_t)std::min<int64_t>(bw, n - r0);
       const uint32_t prefix = (uint32_t)r0;
@@ -839,13 +809,13 @@ struct Builder {
         // unpruned trees without flags label every vertex they reach
         size_t freeb = 0, totalb = 0;
         CHL_HIP(hipMemGetInfo(&freeb, &totalb));
-        out_cap = std::max<uint64_t>(out_cap, std::min<uint64_t>((uint64_t)R * (uint64_t)n + 4096, freeb / 4 / 16));
+        out_cap = std::max<uint64_t>(out_cap, std::min<uint64_t>((uint64_t)R * (uint64_t)n + 4096, freeb / 4 / 12));
       }
       if (flags & TREE_PRUNE) {
         uint64_t mr = 0;
         for (int s = 0; s < nsides; ++s) {
           CHL_HIP(launch_fill_u64((uint64_t*)misc.p + 1, 0, 1, stream));
-          CHL_HIP(launch_max_row(L.off[s].p, (uint32_t)n, misc.p + 1, stream));
+          CHL_HIP(launch_max_row(L.d_views(s), L.nseg(), (uint32_t)n, misc.p + 1, stream));
           unsigned long long h = 0;
           CHL_HIP(hipMemcpyAsync(&h, misc.p + 1, sizeof(h), hipMemcpyDeviceToHost, stream));
           CHL_HIP(hipStreamSynchronize(stream));
@@ -871,7 +841,7 @@ struct Builder {
       if (device_switch)
         CHL_HIP(launch_switch_test(tree_size.p, label_count.p, R, o.switch_ratio, misc.p, stream));
       for (int s = 0; s < nsides; ++s) {
-        rc = sort_side(s, stt);
+        rc = build_segment(s, (uint32_t)r0, R, stt);
         if (rc) return rc;
       }
       if (!(flags & TREE_FLAGS)) {
@@ -884,10 +854,8 @@ struct Builder {
           if (rc) return rc;
         }
       }
-      for (int s = 0; s < nsides; ++s) {
-        rc = merge_side(s, L, ct);
-        if (rc) return rc;
-      }
+      rc = append_batch(L, ct);
+      if (rc) return rc;
       if (stats) {
         stats->batches++;
         if (flags & TREE_FLAGS) stats->trees_plant += (int64_t)R * nsides;
@@ -907,9 +875,9 @@ struct Builder {
       }
       if (o.verbose) {
         uint64_t emitted = sb[0].kept + (nsides == 2 ? sb[1].kept : 0);
-        fprintf(stderr, "batch %lld: ranks [%lld, %lld) flags %u labels %llu store %llu\n", (long long)stats_batches,
-                (long long)r0, (long long)(r0 + R), flags, (unsigned long long)emitted,
-                (unsigned long long)L.total[0]);
+        fprintf(stderr, "batch %lld: ranks [%lld, %lld) flags %u labels %llu store %llu (%u segments, %.2f GB)\n",
+                (long long)stats_batches, (long long)r0, (long long)(r0 + R), flags, (unsigned long long)emitted,
+                (unsigned long long)L.total[0], L.nseg(), L.bytes() / 1e9);
       }
       ++stats_batches;
       {
@@ -936,8 +904,31 @@ struct Builder {
       r0 += R;
     }
     CHL_HIP(hipStreamSynchronize(stream));
+    {
+      // the search slots and batch buffers go back to the device, then the
+      // store's segments are merged into one where memory allows (queries then
+      // merge-join one row pair per side)
+      for (auto* b : {&cell}) b->release();
+      for (auto* b : {&qa, &qb, &vis, &gkey, &gval}) b->release();
+      qd.release();
+      for (auto& b : sb) {
+        b.row.release();
+        b.hd.release();
+        b.keep.release();
+        b.seg.reset();
+      }
+      dev_cache_flush();
+      CHL_HIP(hipEventRecord(ev[6], stream));
+      rc = L.consolidate(stream);
+      if (rc) return rc;
+      CHL_HIP(hipEventRecord(ev[7], stream));
+      rc = elapsed(ev[6], ev[7], ct);
+      if (rc) return rc;
+    }
     const double t1 = now_s();
     if (stats) {
+      stats->segments = L.nseg();
+      stats->store_bytes = (double)L.bytes();
       stats->lt = lt / 1e3;
       stats->st = stt / 1e3;
       stats->ct = ct / 1e3;

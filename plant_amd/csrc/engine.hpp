@@ -1,27 +1,57 @@
 // Host-side interface of the device engine (label store, build, hooks).
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "dev.hpp"
 
 namespace chl {
 
+// One segment of a label side: the labels of hub ranks [h0, h1) as a CSR over
+// row ids (off, rows + 1 offsets), entries in SEG_F5 (key, lo) or SEG_F8 (ent)
+// form; rng: per vertex id (begin, end) of its row, built for queries.
+struct Segment {
+  uint32_t h0 = 0, h1 = 0;
+  uint32_t fmt = SEG_F8;
+  uint64_t total = 0;
+  BigBuf<uint64_t> off;
+  BigBuf<uint32_t> key;
+  BigBuf<uint8_t> lo;
+  BigBuf<uint64_t> ent;
+  BigBuf<uint64_t> rng;
+  SegView view() const;
+  uint64_t entry_bytes() const;
+  int alloc(uint64_t rows, uint64_t count, uint32_t fmt);
+};
+
 // Labels resident on the device. Rows are indexed by row id (the rank for built
-// labels, the vertex id for labels read from a file) and hold (hub << 32) | dist
-// with hubs ascending. shared_sides: side 1 is side 0 (symmetric graph).
+// labels, the vertex id for labels read from a file); each side is a list of
+// segments with ascending, disjoint hub ranges, the same for both sides.
+// shared_sides: side 1 is side 0 (symmetric graph).
 struct LabelStore {
   int64_t n = 0;
   bool shared_sides = false;
   int device = 0;
-  DBuf<uint64_t> off[2];
-  GrowBuf<uint64_t> ent[2];  // grows in place as batches are merged
+  std::vector<std::unique_ptr<Segment>> seg[2];
+  DBuf<SegView> dview[2];  // device copies of the segment views
   uint64_t total[2] = {0, 0};
-  const uint64_t* d_off(int s) const { return off[shared_sides ? 0 : s].p; }
-  const uint64_t* d_ent(int s) const { return ent[shared_sides ? 0 : s].p; }
+  int nsides() const { return shared_sides ? 1 : 2; }
+  const std::vector<std::unique_ptr<Segment>>& side(int s) const { return seg[shared_sides ? 0 : s]; }
+  const SegView* d_views(int s) const { return dview[shared_sides ? 0 : s].p; }
+  uint32_t nseg() const { return (uint32_t)seg[0].size(); }
   uint64_t side_total(int s) const { return total[shared_sides ? 0 : s]; }
+  uint64_t bytes() const;  // device bytes of entries and offsets
   int init(int64_t n, bool shared, int dev);
+  int sync_views();
+  // appends one batch's segments (s1 ignored with shared sides)
+  int append(std::unique_ptr<Segment> s0, std::unique_ptr<Segment> s1);
+  int merge_range(size_t a, size_t b, hipStream_t st);
+  int tidy(size_t budget, hipStream_t st);
+  int consolidate(hipStream_t st);
+  int build_ranges(const uint32_t* row_of, hipStream_t st);
   int download(int side, std::vector<uint64_t>& off, std::vector<uint64_t>& ent) const;
+  int upload(int side, const std::vector<uint64_t>& off, const std::vector<uint64_t>& ent);
 };
 
 // The graph resident on the device, kept in the graph handle across builds:

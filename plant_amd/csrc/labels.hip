@@ -1,11 +1,15 @@
-// Label-store kernels: appending a batch of labels to the per-vertex rows,
-// in-batch cleaning, and the merge-join distance query.
+// Label-store kernels: building a batch's segment (count, scan, scatter, per-row
+// sort), in-batch cleaning and compaction, merging segments, digests, and the
+// merge-join distance query over segmented sides.
 //
-// The store keeps, per side, one row per vertex (rank ids) in CSR form:
-// off[n+1] and ent[T] with ent = (hub << 32) | dist, rows hub-ascending. A batch
-// covers a contiguous rank range above every hub already stored, so appending
-// the batch's sorted segment after each row keeps rows sorted, as the
-// superstep append of shared_memory/akiba.hpp:197-208 does.
+// A batch covers a contiguous hub-rank range above every hub already stored, as
+// a superstep of shared_memory/akiba.hpp:152-208 does, so its labels form one
+// new segment: each row is the batch's part of a vertex's label, hub-ascending.
+// The segment is built in O(batch): the tree kernel writes each label's row
+// and (hub, dist); cnt[row] counts them, a scan gives the segment's row
+// offsets, a scatter places every label in its row, and a per-row sort puts
+// the row in hub order (labels of one row arrive from different trees in any
+// order). No row of the existing store moves.
 #include <hipcub/hipcub.hpp>
 
 #include "dev.hpp"
@@ -15,25 +19,421 @@ namespace {
 
 constexpr uint32_t kInfDist = 2147483647u;
 
-// Lower bound of (v << 32) in the sorted batch keys, one thread per vertex.
-__global__ void seg_offsets_kernel(const uint64_t* keys, uint64_t m, uint32_t n, uint64_t* seg) {
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v <= n; v += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t target = v << 32;
-    uint64_t lo = 0, hi = m;
-    while (lo < hi) {
-      const uint64_t mid = (lo + hi) >> 1;
-      if (keys[mid] < target) lo = mid + 1;
-      else hi = mid;
+unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap) {
+  uint64_t b = (items + per_block - 1) / per_block;
+  if (b == 0) b = 1;
+  return (unsigned)(b < cap ? b : cap);
+}
+
+// ---- count, scan, scatter ------------------------------------------------------
+
+__global__ void count_rows_kernel(const uint32_t* row, uint64_t m, uint32_t* cnt) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[row[k]], 1u);
+}
+
+// Exclusive scan in tiles of kScanTile items: a block per tile, each thread
+// owning kScanPer consecutive items.
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 8;
+constexpr uint64_t kScanTile = (uint64_t)kScanThreads * kScanPer;
+
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t x, uint32_t lane) {
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+// Block-wide exclusive scan of one value per thread; *total = the block sum.
+__device__ __forceinline__ uint64_t block_excl_scan(uint64_t x, uint64_t* s_w, uint64_t* total) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t incl = wave_incl_scan(x, lane);
+  if (lane == 63) s_w[wave] = incl;
+  __syncthreads();
+  uint64_t before = 0, sum = 0;
+  for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+    if (w < wave) before += s_w[w];
+    sum += s_w[w];
+  }
+  __syncthreads();
+  *total = sum;
+  return before + incl - x;
+}
+
+template <class T>
+__global__ __launch_bounds__(kScanThreads) void tile_sums_kernel(const T* in, uint64_t count, uint64_t* sums) {
+  __shared__ uint64_t s_w[kScanThreads / 64];
+  const uint64_t base = blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanPer;
+  uint64_t x = 0;
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i)
+    if (base + i < count) x += (uint64_t)in[base + i];
+  uint64_t total = 0;
+  (void)block_excl_scan(x, s_w, &total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// out[i] = prefix[tile] + the exclusive scan inside the tile; the block of the
+// last tile also writes out[count], the total.
+template <class T>
+__global__ __launch_bounds__(kScanThreads) void tile_scan_kernel(const T* in, uint64_t count, const uint64_t* prefix,
+                                                                 uint64_t* out) {
+  __shared__ uint64_t s_w[kScanThreads / 64];
+  const uint64_t base = blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanPer;
+  uint64_t v[kScanPer];
+  uint64_t x = 0;
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    v[i] = base + i < count ? (uint64_t)in[base + i] : 0;
+    x += v[i];
+  }
+  uint64_t total = 0;
+  uint64_t run = block_excl_scan(x, s_w, &total) + (prefix ? prefix[blockIdx.x] : 0);
+#pragma unroll
+  for (int i = 0; i < kScanPer; ++i) {
+    if (base + i < count) out[base + i] = run;
+    run += v[i];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[count] = (prefix ? prefix[blockIdx.x] : 0) + total;
+}
+
+template <class T>
+hipError_t scan_any(const T* in, uint64_t count, uint64_t* out, uint64_t* tmp, hipStream_t s) {
+  if (count == 0) return launch_fill_u64(out, 0, 1, s);
+  const uint64_t tiles = (count + kScanTile - 1) / kScanTile;
+  if (tiles == 1) {
+    hipLaunchKernelGGL((tile_scan_kernel<T>), dim3(1), dim3(kScanThreads), 0, s, in, count, nullptr, out);
+    return hipGetLastError();
+  }
+  uint64_t* sums = tmp;
+  uint64_t* prefix = tmp + tiles;
+  hipLaunchKernelGGL((tile_sums_kernel<T>), dim3((unsigned)tiles), dim3(kScanThreads), 0, s, in, count, sums);
+  hipError_t e = scan_any<uint64_t>(sums, tiles, prefix, tmp + 2 * tiles + 1, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((tile_scan_kernel<T>), dim3((unsigned)tiles), dim3(kScanThreads), 0, s, in, count, prefix, out);
+  return hipGetLastError();
+}
+
+__global__ void scatter_labels_kernel(const uint32_t* row, const uint64_t* hd, uint64_t m, unsigned long long* cur,
+                                      uint32_t fmt, uint32_t h0, uint32_t* key, uint8_t* lo, uint64_t* ent) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t e = hd[k];
+    const uint64_t pos = atomicAdd(&cur[row[k]], 1ull);
+    if (fmt == SEG_F8) {
+      ent[pos] = e;
+    } else {
+      const uint32_t h = (uint32_t)(e >> 32), d = (uint32_t)e;
+      key[pos] = ((h - h0) << 8) | (d >> 8);
+      lo[pos] = (uint8_t)(d & 0xFFu);
     }
-    seg[v] = lo;
   }
 }
 
-__global__ void max_row_kernel(const uint64_t* off, uint32_t n, unsigned long long* out) {
+// ---- per-row sort ----------------------------------------------------------------
+// The sort key of an entry is a 64-bit composite with the hub in its high bits
+// (hubs are distinct within a row): the entry itself in SEG_F8, key << 8 | lo
+// in SEG_F5.
+
+__device__ __forceinline__ uint64_t comp_get(uint32_t fmt, const uint32_t* key, const uint8_t* lo, const uint64_t* ent,
+                                             uint64_t k) {
+  return fmt == SEG_F8 ? ent[k] : (((uint64_t)key[k] << 8) | lo[k]);
+}
+__device__ __forceinline__ void comp_put(uint32_t fmt, uint32_t* key, uint8_t* lo, uint64_t* ent, uint64_t k,
+                                         uint64_t x) {
+  if (fmt == SEG_F8) {
+    ent[k] = x;
+  } else {
+    key[k] = (uint32_t)(x >> 8);
+    lo[k] = (uint8_t)(x & 0xFFu);
+  }
+}
+
+constexpr uint64_t kSmallRow = 16;  // rows up to this length: insertion sort by one thread
+constexpr uint32_t kLdsSort = 4096;  // rows up to this length: bitonic sort in LDS
+
+// One thread per row: short rows sorted in place, longer rows listed in
+// longq[1..] (longq[0] counts them; zeroed first).
+__global__ void sort_small_rows_kernel(const uint64_t* off, uint32_t rows, uint32_t fmt, uint32_t* key, uint8_t* lo,
+                                       uint64_t* ent, uint32_t* longq) {
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < rows; r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t b = off[r], len = off[r + 1] - b;
+    if (len <= 1) continue;
+    if (len > kSmallRow) {
+      longq[1 + atomicAdd(&longq[0], 1u)] = (uint32_t)r;
+      continue;
+    }
+    for (uint64_t i = 1; i < len; ++i) {
+      const uint64_t x = comp_get(fmt, key, lo, ent, b + i);
+      uint64_t j = i;
+      while (j > 0) {
+        const uint64_t y = comp_get(fmt, key, lo, ent, b + j - 1);
+        if (y <= x) break;
+        comp_put(fmt, key, lo, ent, b + j, y);
+        --j;
+      }
+      if (j != i) comp_put(fmt, key, lo, ent, b + j, x);
+    }
+  }
+}
+
+// Bitonic network in the form where every comparison puts the smaller value at
+// the lower index: for each block size k the first step compares i with its
+// mirror i ^ (k - 1) inside the block, the later steps i with i ^ j. Positions
+// at or past len hold +infinity, which never moves down, so a row of any length
+// sorts in place within a power-of-two span P.
+// Pair t of a step: its lower index i and partner.
+__device__ __forceinline__ void bitonic_pair(uint32_t t, uint32_t k, uint32_t j, uint32_t& i, uint32_t& part) {
+  i = 2 * j * (t / j) + (t % j);
+  part = (j == k / 2) ? (i ^ (k - 1)) : (i + j);
+}
+
+__global__ __launch_bounds__(256) void sort_long_rows_kernel(const uint64_t* off, uint32_t fmt, uint32_t* key,
+                                                             uint8_t* lo, uint64_t* ent, const uint32_t* longq) {
+  __shared__ uint64_t s[kLdsSort];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t count = longq[0];
+  for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {
+    const uint32_t r = longq[1 + q];
+    const uint64_t b = off[r];
+    const uint32_t len = (uint32_t)(off[r + 1] - b);
+    uint32_t P = 1;
+    while (P < len) P <<= 1;
+    if (len <= kLdsSort) {
+      for (uint32_t t = tid; t < P; t += 256) s[t] = t < len ? comp_get(fmt, key, lo, ent, b + t) : ~0ull;
+      __syncthreads();
+      for (uint32_t k = 2; k <= P; k <<= 1)
+        for (uint32_t j = k / 2; j > 0; j >>= 1) {
+          for (uint32_t t = tid; t < P / 2; t += 256) {
+            uint32_t i, part;
+            bitonic_pair(t, k, j, i, part);
+            const uint64_t a = s[i], c = s[part];
+            if (a > c) {
+              s[i] = c;
+              s[part] = a;
+            }
+          }
+          __syncthreads();
+        }
+      for (uint32_t t = tid; t < len; t += 256) comp_put(fmt, key, lo, ent, b + t, s[t]);
+    } else {
+      // in global memory (rare: a batch gives one row more than kLdsSort labels)
+      for (uint32_t k = 2; k <= P; k <<= 1)
+        for (uint32_t j = k / 2; j > 0; j >>= 1) {
+          for (uint32_t t = tid; t < P / 2; t += 256) {
+            uint32_t i, part;
+            bitonic_pair(t, k, j, i, part);
+            if (part >= len) continue;  // +infinity there: nothing moves
+            const uint64_t a = comp_get(fmt, key, lo, ent, b + i), c = comp_get(fmt, key, lo, ent, b + part);
+            if (a > c) {
+              comp_put(fmt, key, lo, ent, b + i, c);
+              comp_put(fmt, key, lo, ent, b + part, a);
+            }
+          }
+          __syncthreads();
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- in-batch cleaning -----------------------------------------------------------
+// clean_cover (shared_memory/labeling.hpp:78-88) over the batch labels. One
+// wavefront per row v of this side: v's row (hub-ascending) is staged in LDS;
+// for each candidate (h, d) of it the lanes walk row h of the other side's
+// batch labels 64 entries at a time and look each hub x < h up among v's
+// entries before the candidate (those are exactly v's batch hubs below h); the
+// candidate is covered when some x gives L(v)[x] + L(h)[x] <= d.
+constexpr int kCleanWaves = 4;
+constexpr uint32_t kCleanTile = 1024;
+
+__device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* a, uint32_t len, uint32_t x) {
+  uint32_t lo = 0, hi = len;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void clean_kernel(SegView seg, SegView oseg, uint32_t rows, uint8_t* keep,
+                                                    unsigned long long* ctr) {
+  __shared__ uint32_t s_h[kCleanWaves][kCleanTile];
+  __shared__ uint32_t s_d[kCleanWaves][kCleanTile];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  unsigned long long reads = 0;
+  for (uint64_t v = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; v < rows; v += nw) {
+    const uint64_t rb = seg.off[v], re = seg.off[v + 1];
+    const uint32_t len = (uint32_t)(re - rb);
+    if (len == 0) continue;
+    const bool staged = len <= kCleanTile;
+    if (staged) {
+      for (uint32_t j = lane; j < len; j += 64) {
+        uint32_t h, d;
+        seg_entry(seg, rb + j, h, d);
+        s_h[wave][j] = h;
+        s_d[wave][j] = d;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t c0 = 0; c0 < len; c0 += 64) {
+      // each lane fetches one candidate and the bounds of its other row
+      uint32_t ch = 0, cd = 0;
+      uint64_t ob = 0, oe = 0;
+      if (c0 + lane < len) {
+        if (staged) {
+          ch = s_h[wave][c0 + lane];
+          cd = s_d[wave][c0 + lane];
+        } else {
+          seg_entry(seg, rb + c0 + lane, ch, cd);
+        }
+        ob = oseg.off[ch];
+        oe = oseg.off[ch + 1];
+      }
+      uint64_t kept = 0;  // keep bits of these candidates
+      const uint32_t cn = min(64u, len - c0);
+      for (uint32_t i = 0; i < cn; ++i) {
+        const uint32_t c = c0 + i;
+        const uint32_t h = __shfl(ch, i), d = __shfl(cd, i);
+        const uint64_t b = __shfl(ob, i), e = __shfl(oe, i);
+        bool covered = false;
+        for (uint64_t base = b; base < e; base += 64) {
+          const uint64_t j = base + lane;
+          bool hit = false, past = true;
+          if (j < e) {
+            uint32_t x, dx;
+            seg_entry(oseg, j, x, dx);
+            past = x >= h;
+            if (!past) {
+              uint32_t at, hv, dv;
+              if (staged) {
+                at = lds_lower_bound(s_h[wave], c, x);
+                hv = at < c ? s_h[wave][at] : 0xFFFFFFFFu;
+                dv = at < c ? s_d[wave][at] : 0;
+              } else {
+                uint32_t lo = 0, hi = c;
+                while (lo < hi) {
+                  const uint32_t mid = (lo + hi) >> 1;
+                  uint32_t mh, md;
+                  seg_entry(seg, rb + mid, mh, md);
+                  if (mh < x) lo = mid + 1;
+                  else hi = mid;
+                }
+                at = lo;
+                hv = 0xFFFFFFFFu;
+                dv = 0;
+                if (at < c) seg_entry(seg, rb + at, hv, dv);
+              }
+              hit = hv == x && dv + dx <= d;
+            }
+          }
+          reads += (j < e) ? 1 : 0;
+          if (__any(hit)) {
+            covered = true;
+            break;
+          }
+          if (__all(past)) break;
+        }
+        if (!covered) kept |= 1ull << i;
+      }
+      if (lane < cn) keep[rb + c0 + lane] = (uint8_t)((kept >> lane) & 1ull);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int o = 32; o > 0; o >>= 1) reads += __shfl_xor(reads, o);
+  if (lane == 0 && reads) atomicAdd(&ctr[CTR_PRUNE_ENT], reads);
+}
+
+// ---- compaction, merge -----------------------------------------------------------
+
+__global__ __launch_bounds__(256) void count_kept_kernel(const uint64_t* off, const uint8_t* keep, uint32_t rows,
+                                                         uint32_t* cnt) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t r = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; r < rows; r += nw) {
+    uint32_t c = 0;
+    for (uint64_t k = off[r] + lane; k < off[r + 1]; k += 64) c += keep[k];
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (lane == 0) cnt[r] = c;
+  }
+}
+
+__device__ __forceinline__ void put_entry(uint32_t fmt, uint32_t h0, uint32_t* key, uint8_t* lo, uint64_t* ent,
+                                          uint64_t k, uint32_t h, uint32_t d) {
+  if (fmt == SEG_F8) {
+    ent[k] = ((uint64_t)h << 32) | d;
+  } else {
+    key[k] = ((h - h0) << 8) | (d >> 8);
+    lo[k] = (uint8_t)(d & 0xFFu);
+  }
+}
+
+__global__ __launch_bounds__(256) void copy_kept_kernel(SegView seg, const uint8_t* keep, uint32_t rows,
+                                                        const uint64_t* noff, uint32_t* key, uint8_t* lo,
+                                                        uint64_t* ent) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t below = (1ull << lane) - 1;
+  for (uint64_t r = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; r < rows; r += nw) {
+    uint64_t pos = noff[r];
+    const uint64_t e = seg.off[r + 1];
+    for (uint64_t base = seg.off[r]; base < e; base += 64) {
+      const uint64_t k = base + lane;
+      const bool kp = k < e && keep[k];
+      const uint64_t mask = __ballot(kp);
+      if (kp) {
+        const uint64_t at = pos + __popcll(mask & below);
+        if (seg.fmt == SEG_F8) {
+          ent[at] = seg.ent[k];
+        } else {
+          key[at] = seg.key[k];
+          lo[at] = seg.lo[k];
+        }
+      }
+      pos += __popcll(mask);
+    }
+  }
+}
+
+__global__ void row_total_kernel(const SegView* views, uint32_t k, uint32_t rows, uint32_t* cnt) {
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < rows; r += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t t = 0;
+    for (uint32_t s = 0; s < k; ++s) t += views[s].off[r + 1] - views[s].off[r];
+    cnt[r] = (uint32_t)t;
+  }
+}
+
+__global__ __launch_bounds__(256) void concat_rows_kernel(const SegView* views, uint32_t k, uint32_t rows,
+                                                          const uint64_t* noff, uint32_t fmt, uint32_t h0,
+                                                          uint32_t* key, uint8_t* lo, uint64_t* ent) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t r = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; r < rows; r += nw) {
+    uint64_t pos = noff[r];
+    for (uint32_t s = 0; s < k; ++s) {
+      const SegView v = views[s];
+      const uint64_t b = v.off[r], e = v.off[r + 1];
+      for (uint64_t j = b + lane; j < e; j += 64) {
+        uint32_t h, d;
+        seg_entry(v, j, h, d);
+        put_entry(fmt, h0, key, lo, ent, pos + (j - b), h, d);
+      }
+      pos += e - b;
+    }
+  }
+}
+
+__global__ void max_row_kernel(const SegView* views, uint32_t k, uint32_t rows, unsigned long long* out) {
   unsigned long long mx = 0;
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
-    const unsigned long long l = off[v + 1] - off[v];
-    mx = l > mx ? l : mx;
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < rows; r += (uint64_t)gridDim.x * blockDim.x) {
+    unsigned long long t = 0;
+    for (uint32_t s = 0; s < k; ++s) t += views[s].off[r + 1] - views[s].off[r];
+    mx = t > mx ? t : mx;
   }
   for (int o = 32; o > 0; o >>= 1) {
     const unsigned long long other = __shfl_xor(mx, o);
@@ -51,39 +451,15 @@ __global__ void switch_test_kernel(const uint32_t* tree_size, const uint32_t* la
   }
 }
 
-__global__ void merge_offsets_kernel(const uint64_t* old_off, const uint64_t* excl, uint32_t n, uint64_t* new_off) {
-  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v <= n; v += (uint64_t)gridDim.x * blockDim.x)
-    new_off[v] = old_off[v] + excl[v];
-}
+// ---- digest, ranges, decode ----------------------------------------------------
 
-// In-place merge of a batch into the store, one block of vertices [a, b) at a
-// time from the last block down: the block's old rows were first copied to tmp
-// (they start at old_off[a]); row v goes to new_off[v] >= old_off[v] followed by
-// the batch's entries of v. Later blocks are already in place and earlier rows
-// lie below new_off[a], so no unread entry is overwritten.
-__global__ __launch_bounds__(256) void place_rows_kernel(const uint64_t* old_off, const uint64_t* tmp, uint64_t tmp_base,
-                                                         const uint64_t* new_off, const uint64_t* excl,
-                                                         const uint64_t* bat, uint32_t a, uint32_t b, uint64_t* ent) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t w0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t v = a + w0; v < b; v += nw) {
-    const uint64_t ob = old_off[v] - tmp_base, len = old_off[v + 1] - old_off[v], nb = new_off[v];
-    for (uint64_t j = lane; j < len; j += 64) ent[nb + j] = tmp[ob + j];
-    const uint64_t bb = excl[v], be = excl[v + 1];
-    for (uint64_t j = lane; j < be - bb; j += 64) ent[nb + len + j] = bat[bb + j];
-  }
-}
-
-// Order-independent digest of a label side: the sum over all entries (hub, d) of
-// vertex v's row of a 64-bit mix of (v, hub, d) (the splitmix64 finalizer).
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
 }
-__global__ __launch_bounds__(256) void checksum_kernel(const uint64_t* off, const uint64_t* ent, const uint32_t* row_of,
-                                                       uint32_t n, unsigned long long* out) {
+__global__ __launch_bounds__(256) void checksum_kernel(SegView seg, const uint32_t* row_of, uint32_t n,
+                                                       unsigned long long* out) {
   unsigned long long acc = 0;
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t w0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
@@ -91,76 +467,16 @@ __global__ __launch_bounds__(256) void checksum_kernel(const uint64_t* off, cons
   for (uint64_t v = w0; v < n; v += nw) {
     const uint32_t r = row_of[v];
     const uint64_t key = (uint64_t)v * 0x9E3779B97F4A7C15ull;
-    for (uint64_t k = off[r] + lane; k < off[r + 1]; k += 64) acc += mix64(ent[k] ^ key);
+    for (uint64_t k = seg.off[r] + lane; k < seg.off[r + 1]; k += 64) {
+      uint32_t h, d;
+      seg_entry(seg, k, h, d);
+      acc += mix64((((uint64_t)h << 32) | d) ^ key);
+    }
   }
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
   if (lane == 0 && acc) atomicAdd(out, acc);
 }
 
-__global__ void copy_u64_kernel(const uint64_t* src, uint64_t count, uint64_t* dst) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
-}
-
-// clean_cover (shared_memory/labeling.hpp:78-88) of candidate k = (v, h, d):
-// merge v's batch labels below h with the opposite-side batch labels of h below
-// h; covered if a common hub x gives L(v)[x] + L(h)[x] <= d.
-__global__ void clean_kernel(const uint64_t* key, const uint64_t* ent, uint64_t m, const uint64_t* seg_off,
-                             const uint64_t* oent, const uint64_t* oseg_off, uint8_t* keep, unsigned long long* ctr) {
-  unsigned long long reads = 0;
-  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t v = (uint32_t)(key[k] >> 32);
-    const uint64_t e = ent[k];
-    const uint32_t h = (uint32_t)(e >> 32);
-    const uint32_t d = (uint32_t)e;
-    uint64_t i = seg_off[v], ie = k;
-    uint64_t j = oseg_off[h], je = oseg_off[h + 1];
-    bool cov = false;
-    while (i < ie && j < je) {
-      const uint64_t a = ent[i], b = oent[j];
-      const uint32_t ha = (uint32_t)(a >> 32), hb = (uint32_t)(b >> 32);
-      ++reads;
-      if (ha >= h || hb >= h) break;
-      if (ha == hb) {
-        if ((uint32_t)a + (uint32_t)b <= d) {
-          cov = true;
-          break;
-        }
-        ++i;
-        ++j;
-      } else if (ha < hb) {
-        ++i;
-      } else {
-        ++j;
-      }
-    }
-    keep[k] = cov ? 0 : 1;
-  }
-  if (reads) atomicAdd(&ctr[CTR_PRUNE_ENT], reads);
-}
-
-// Query: one wavefront per pair. The side-1 row of u is staged into LDS (in
-// tiles of kQT entries), each lane takes entries of the side-0 row of v and
-// binary-searches their hub among the staged hubs; the answer is the minimum
-// sum over common hubs, INT_MAX when there is none (labeling.hpp:56-65).
-// Every lane issues all its loads of a tile (kQPer entries) and of v's chunk
-// before it uses any of them, so a pair costs a few memory latencies, not one
-// per 64 entries, and its kQPer searches advance in lock step (their LDS reads
-// overlap).
-constexpr int kQWaves = 4;
-constexpr int kQT = 1024;
-constexpr int kQPer = kQT / 64;
-
-__device__ __forceinline__ void load_chunk(const uint64_t* ent, uint64_t base, uint64_t end, uint32_t lane,
-                                           uint64_t (&e)[kQPer]) {
-#pragma unroll
-  for (int i = 0; i < kQPer; ++i) {
-    const uint64_t j = base + lane + 64u * i;
-    e[i] = j < end ? ent[j] : ~0ull;  // hub all ones: past every staged hub
-  }
-}
-
-// rng[2 v], rng[2 v + 1] = the bounds of vertex v's row (row row_of[v]).
 __global__ void row_ranges_kernel(const uint64_t* off, const uint32_t* row_of, uint32_t n, uint64_t* rng) {
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
     const uint32_t r = row_of[v];
@@ -168,79 +484,140 @@ __global__ void row_ranges_kernel(const uint64_t* off, const uint32_t* row_of, u
   }
 }
 
-// rng1 / rng0: per vertex id the (begin, end) of its side-1 / side-0 row, so a
-// pair reaches its rows with one 16-byte load per endpoint.
-__global__ __launch_bounds__(256) void query_kernel(const uint64_t* rng1, const uint64_t* ent1, const uint64_t* rng0,
-                                                    const uint64_t* ent0, const uint32_t* pairs, uint32_t n,
-                                                    uint64_t q, int32_t* out) {
+__global__ void decode_kernel(SegView seg, uint64_t count, uint64_t* out) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < count; k += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t h, d;
+    seg_entry(seg, k, h, d);
+    out[k] = ((uint64_t)h << 32) | d;
+  }
+}
+
+__global__ void copy_u64_kernel(const uint64_t* src, uint64_t count, uint64_t* dst) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// ---- query ---------------------------------------------------------------------
+// One wavefront per pair. Lane s fetches the bounds of u's side-1 row and v's
+// side-0 row in segment s, so the rows of every segment are located in one
+// round trip; then, per segment where both rows are non-empty, u's row is staged
+// into LDS (in tiles of kQT entries), each lane takes entries of v's row and
+// binary-searches their hub among the staged hubs; the answer is the minimum sum
+// over common hubs, INT_MAX when there is none (labeling.hpp:56-65). Every lane
+// issues all its loads of a tile (kQPer entries) and of v's chunk before it uses
+// any of them, and its kQPer searches advance in lock step (their LDS reads
+// overlap).
+constexpr int kQWaves = 4;
+constexpr int kQT = 1024;
+constexpr int kQPer = kQT / 64;
+
+// Entries [base + lane + 64 i] of a segment as (hub << 32) | dist; past the end
+// the hub is all ones (past every staged hub).
+__device__ __forceinline__ void load_chunk(const SegView& s, uint64_t base, uint64_t end, uint32_t lane,
+                                           uint64_t (&e)[kQPer]) {
+  if (s.fmt == SEG_F8) {
+#pragma unroll
+    for (int i = 0; i < kQPer; ++i) {
+      const uint64_t j = base + lane + 64u * i;
+      e[i] = j < end ? s.ent[j] : ~0ull;
+    }
+  } else {
+    uint32_t x[kQPer];
+    uint8_t l[kQPer];
+#pragma unroll
+    for (int i = 0; i < kQPer; ++i) {
+      const uint64_t j = base + lane + 64u * i;
+      x[i] = j < end ? s.key[j] : 0xFFFFFFFFu;
+      l[i] = j < end ? s.lo[j] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kQPer; ++i) {
+      const uint64_t j = base + lane + 64u * i;
+      e[i] = j < end ? (((uint64_t)(s.h0 + (x[i] >> 8)) << 32) | ((x[i] & 0xFFu) << 8) | l[i]) : ~0ull;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void query_kernel(const SegView* v1, uint32_t k1, const SegView* v0, uint32_t k0,
+                                                    const uint32_t* pairs, uint32_t n, uint64_t q, int32_t* out) {
   __shared__ uint32_t s_h[kQWaves][kQT];
   __shared__ uint32_t s_d[kQWaves][kQT];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint32_t ks = k1 < k0 ? k1 : k0;  // the two sides have the same segments
   for (uint64_t k = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; k < q; k += nw) {
     const uint32_t pu = pairs[2 * k], pv = pairs[2 * k + 1];
     if (pu >= n || pv >= n) {
       if (lane == 0) out[k] = -1;
       continue;
     }
-    const ulonglong2 ru = reinterpret_cast<const ulonglong2*>(rng1)[pu];
-    const ulonglong2 rv = reinterpret_cast<const ulonglong2*>(rng0)[pv];
-    const uint64_t ab = ru.x, ae = ru.y, bb = rv.x, be = rv.y;
+    ulonglong2 ru = make_ulonglong2(0, 0), rv = make_ulonglong2(0, 0);
+    if (lane < ks) {
+      ru = reinterpret_cast<const ulonglong2*>(v1[lane].rng)[pu];
+      rv = reinterpret_cast<const ulonglong2*>(v0[lane].rng)[pv];
+    }
+    uint64_t segs = __ballot(ru.x < ru.y && rv.x < rv.y);
     uint32_t best = kInfDist;
-    for (uint64_t t0 = ab; t0 < ae; t0 += kQT) {
-      const uint32_t tl = (uint32_t)((ae - t0) < kQT ? (ae - t0) : kQT);
-      uint64_t eu[kQPer], ev[kQPer];
-      load_chunk(ent1, t0, ae, lane, eu);
-      load_chunk(ent0, bb, be, lane, ev);
-#pragma unroll
-      for (int i = 0; i < kQPer; ++i) {
-        const uint32_t j = lane + 64u * i;
-        if (j < tl) {
-          s_h[wave][j] = (uint32_t)(eu[i] >> 32);
-          s_d[wave][j] = (uint32_t)eu[i];
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const uint32_t lo_h = s_h[wave][0], hi_h = s_h[wave][tl - 1];
-      for (uint64_t c0 = bb;;) {
-        // kQPer branch-free lower-bound searches in lock step: every step is one
-        // LDS read per search, all independent, so their latencies overlap
-        uint32_t base[kQPer];
-        bool past = true;
+    while (segs) {
+      const uint32_t sg = (uint32_t)__ffsll((long long)segs) - 1;
+      segs &= segs - 1;
+      const uint64_t ab = __shfl(ru.x, sg), ae = __shfl(ru.y, sg), bb = __shfl(rv.x, sg), be = __shfl(rv.y, sg);
+      const SegView S1 = v1[sg], S0 = v0[sg];
+      for (uint64_t t0 = ab; t0 < ae; t0 += kQT) {
+        const uint32_t tl = (uint32_t)((ae - t0) < kQT ? (ae - t0) : kQT);
+        uint64_t eu[kQPer], ev[kQPer];
+        load_chunk(S1, t0, ae, lane, eu);
+        load_chunk(S0, bb, be, lane, ev);
 #pragma unroll
         for (int i = 0; i < kQPer; ++i) {
-          base[i] = 0;
-          past = past && (uint32_t)(ev[i] >> 32) > hi_h;
+          const uint32_t j = lane + 64u * i;
+          if (j < tl) {
+            s_h[wave][j] = (uint32_t)(eu[i] >> 32);
+            s_d[wave][j] = (uint32_t)eu[i];
+          }
         }
-        for (uint32_t len = tl; len > 1;) {
-          const uint32_t half = len >> 1;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t lo_h = s_h[wave][0], hi_h = s_h[wave][tl - 1];
+        for (uint64_t c0 = bb;;) {
+          // kQPer branch-free lower-bound searches in lock step: every step is one
+          // LDS read per search, all independent, so their latencies overlap
+          uint32_t base[kQPer];
+          bool past = true;
+#pragma unroll
+          for (int i = 0; i < kQPer; ++i) {
+            base[i] = 0;
+            past = past && (uint32_t)(ev[i] >> 32) > hi_h;
+          }
+          for (uint32_t len = tl; len > 1;) {
+            const uint32_t half = len >> 1;
+#pragma unroll
+            for (int i = 0; i < kQPer; ++i) {
+              const uint32_t h = (uint32_t)(ev[i] >> 32);
+              base[i] = s_h[wave][base[i] + half] < h ? base[i] + half : base[i];
+            }
+            len -= half;
+          }
 #pragma unroll
           for (int i = 0; i < kQPer; ++i) {
             const uint32_t h = (uint32_t)(ev[i] >> 32);
-            base[i] = s_h[wave][base[i] + half] < h ? base[i] + half : base[i];
+            // base = last index with hub < h (or 0); the match, if any, is there or next
+            const uint32_t at = s_h[wave][base[i]] < h ? base[i] + 1 : base[i];
+            if (h >= lo_h && h <= hi_h && at < tl && s_h[wave][at] == h) {
+              const uint32_t sum = s_d[wave][at] + (uint32_t)ev[i];
+              best = sum < best ? sum : best;
+            }
           }
-          len -= half;
+          // the side-0 row is hub-ascending: stop once every lane is past this tile
+          c0 += kQT;
+          if (__all(past) || c0 >= be) break;
+          load_chunk(S0, c0, be, lane, ev);
         }
-#pragma unroll
-        for (int i = 0; i < kQPer; ++i) {
-          const uint32_t h = (uint32_t)(ev[i] >> 32);
-          // base = last index with hub < h (or 0); the match, if any, is there or next
-          const uint32_t at = s_h[wave][base[i]] < h ? base[i] + 1 : base[i];
-          if (h >= lo_h && h <= hi_h && at < tl && s_h[wave][at] == h) {
-            const uint32_t sum = s_d[wave][at] + (uint32_t)ev[i];
-            best = sum < best ? sum : best;
-          }
-        }
-        // the side-0 row is hub-ascending: stop once every lane is past this tile
-        c0 += kQT;
-        if (__all(past) || c0 >= be) break;
-        load_chunk(ent0, c0, be, lane, ev);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     // wave minimum
     for (int o = 32; o > 0; o >>= 1) {
@@ -251,22 +628,85 @@ __global__ __launch_bounds__(256) void query_kernel(const uint64_t* rng1, const 
   }
 }
 
-static unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap) {
-  uint64_t b = (items + per_block - 1) / per_block;
-  if (b == 0) b = 1;
-  return (unsigned)(b < cap ? b : cap);
-}
-
 }  // namespace
 
-hipError_t launch_seg_offsets(const uint64_t* keys, uint64_t m, uint32_t n, uint64_t* seg, hipStream_t s) {
-  hipLaunchKernelGGL(seg_offsets_kernel, dim3(grid_for((uint64_t)n + 1, 256, 16384)), dim3(256), 0, s, keys, m, n,
-                     seg);
+// ---- launches ------------------------------------------------------------------
+
+hipError_t launch_count_rows(const uint32_t* row, uint64_t m, uint32_t* cnt, hipStream_t s) {
+  if (!m) return hipSuccess;
+  hipLaunchKernelGGL(count_rows_kernel, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, row, m, cnt);
   return hipGetLastError();
 }
 
-hipError_t launch_max_row(const uint64_t* off, uint32_t n, unsigned long long* out, hipStream_t s) {
-  hipLaunchKernelGGL(max_row_kernel, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, off, n, out);
+size_t scan_tmp_words(uint64_t count) {
+  const uint64_t tiles = (count + kScanTile - 1) / kScanTile;
+  if (tiles <= 1) return 1;
+  return (size_t)(2 * tiles + 1) + scan_tmp_words(tiles);
+}
+
+hipError_t launch_scan_u32(const uint32_t* in, uint64_t count, uint64_t* out, uint64_t* tmp, hipStream_t s) {
+  return scan_any<uint32_t>(in, count, out, tmp, s);
+}
+
+hipError_t launch_scatter_labels(const uint32_t* row, const uint64_t* hd, uint64_t m, uint64_t* cur, uint32_t fmt,
+                                 uint32_t h0, uint32_t* key, uint8_t* lo, uint64_t* ent, hipStream_t s) {
+  if (!m) return hipSuccess;
+  hipLaunchKernelGGL(scatter_labels_kernel, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, row, hd, m,
+                     (unsigned long long*)cur, fmt, h0, key, lo, ent);
+  return hipGetLastError();
+}
+
+hipError_t launch_sort_rows(const uint64_t* off, uint32_t rows, uint32_t fmt, uint32_t* key, uint8_t* lo,
+                            uint64_t* ent, uint32_t* longq, hipStream_t s) {
+  if (!rows) return hipSuccess;
+  hipError_t e = launch_fill_u32(longq, 0, 1, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(sort_small_rows_kernel, dim3(grid_for(rows, 256, 16384)), dim3(256), 0, s, off, rows, fmt, key,
+                     lo, ent, longq);
+  hipLaunchKernelGGL(sort_long_rows_kernel, dim3(4096), dim3(256), 0, s, off, fmt, key, lo, ent, longq);
+  return hipGetLastError();
+}
+
+hipError_t launch_clean(const SegView& seg, const SegView& oseg, uint32_t rows, uint8_t* keep,
+                        unsigned long long* ctr, hipStream_t s) {
+  if (!rows) return hipSuccess;
+  hipLaunchKernelGGL(clean_kernel, dim3(grid_for((uint64_t)rows * 64, 256, 16384)), dim3(256), 0, s, seg, oseg, rows,
+                     keep, ctr);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_kept(const uint64_t* off, const uint8_t* keep, uint32_t rows, uint32_t* cnt, hipStream_t s) {
+  if (!rows) return hipSuccess;
+  hipLaunchKernelGGL(count_kept_kernel, dim3(grid_for((uint64_t)rows * 64, 256, 16384)), dim3(256), 0, s, off, keep,
+                     rows, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_kept(const SegView& seg, const uint8_t* keep, uint32_t rows, const uint64_t* noff,
+                            uint32_t* key, uint8_t* lo, uint64_t* ent, hipStream_t s) {
+  if (!rows) return hipSuccess;
+  hipLaunchKernelGGL(copy_kept_kernel, dim3(grid_for((uint64_t)rows * 64, 256, 16384)), dim3(256), 0, s, seg, keep,
+                     rows, noff, key, lo, ent);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_total(const SegView* views, uint32_t k, uint32_t rows, uint32_t* cnt, hipStream_t s) {
+  if (!rows) return hipSuccess;
+  hipLaunchKernelGGL(row_total_kernel, dim3(grid_for(rows, 256, 16384)), dim3(256), 0, s, views, k, rows, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_concat_rows(const SegView* views, uint32_t k, uint32_t rows, const uint64_t* noff, uint32_t fmt,
+                              uint32_t h0, uint32_t* key, uint8_t* lo, uint64_t* ent, hipStream_t s) {
+  if (!rows) return hipSuccess;
+  hipLaunchKernelGGL(concat_rows_kernel, dim3(grid_for((uint64_t)rows * 64, 256, 16384)), dim3(256), 0, s, views, k,
+                     rows, noff, fmt, h0, key, lo, ent);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_row(const SegView* views, uint32_t k, uint32_t rows, unsigned long long* out, hipStream_t s) {
+  if (!rows || !k) return hipSuccess;
+  hipLaunchKernelGGL(max_row_kernel, dim3(grid_for(rows, 256, 4096)), dim3(256), 0, s, views, k, rows, out);
   return hipGetLastError();
 }
 
@@ -278,50 +718,11 @@ hipError_t launch_switch_test(const uint32_t* tree_size, const uint32_t* label_c
   return hipGetLastError();
 }
 
-hipError_t launch_merge_offsets(const uint64_t* old_off, const uint64_t* excl, uint32_t n, uint64_t* new_off,
-                                hipStream_t s) {
-  hipLaunchKernelGGL(merge_offsets_kernel, dim3(grid_for((uint64_t)n + 1, 256, 16384)), dim3(256), 0, s, old_off,
-                     excl, n, new_off);
-  return hipGetLastError();
-}
-
-
-hipError_t launch_place_rows(const uint64_t* old_off, const uint64_t* tmp, uint64_t tmp_base, const uint64_t* new_off,
-                             const uint64_t* excl, const uint64_t* bat, uint32_t a, uint32_t b, uint64_t* ent,
-                             hipStream_t s) {
-  if (b <= a) return hipSuccess;
-  hipLaunchKernelGGL(place_rows_kernel, dim3(grid_for((uint64_t)(b - a) * 64, 256, 65536)), dim3(256), 0, s, old_off,
-                     tmp, tmp_base, new_off, excl, bat, a, b, ent);
-  return hipGetLastError();
-}
-
-hipError_t launch_checksum(const uint64_t* off, const uint64_t* ent, const uint32_t* row_of, uint32_t n,
-                           unsigned long long* out, hipStream_t s) {
-  hipLaunchKernelGGL(checksum_kernel, dim3(grid_for((uint64_t)n * 64, 256, 16384)), dim3(256), 0, s, off, ent, row_of,
-                     n, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_copy_u64(const uint64_t* src, uint64_t count, uint64_t* dst, hipStream_t s) {
-  if (!count) return hipSuccess;
-  hipLaunchKernelGGL(copy_u64_kernel, dim3(grid_for(count, 256, 16384)), dim3(256), 0, s, src, count, dst);
-  return hipGetLastError();
-}
-
-hipError_t launch_clean(const uint64_t* key, const uint64_t* ent, uint64_t m, const uint64_t* seg_off,
-                        const uint64_t* oent, const uint64_t* oseg_off, uint8_t* keep, unsigned long long* ctr,
-                        hipStream_t s) {
-  if (!m) return hipSuccess;
-  hipLaunchKernelGGL(clean_kernel, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, key, ent, m, seg_off, oent,
-                     oseg_off, keep, ctr);
-  return hipGetLastError();
-}
-
-hipError_t launch_query(const uint64_t* rng1, const uint64_t* ent1, const uint64_t* rng0, const uint64_t* ent0,
-                        const uint32_t* pairs, uint32_t n, uint64_t q, int32_t* out, hipStream_t s) {
-  if (!q) return hipSuccess;
-  hipLaunchKernelGGL(query_kernel, dim3(grid_for(q * 64, 256, 8192)), dim3(256), 0, s, rng1, ent1, rng0, ent0, pairs, n,
-                     q, out);
+hipError_t launch_checksum(const SegView& seg, const uint32_t* row_of, uint32_t n, unsigned long long* out,
+                           hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(checksum_kernel, dim3(grid_for((uint64_t)n * 64, 256, 16384)), dim3(256), 0, s, seg, row_of, n,
+                     out);
   return hipGetLastError();
 }
 
@@ -331,21 +732,31 @@ hipError_t launch_row_ranges(const uint64_t* off, const uint32_t* row_of, uint32
   return hipGetLastError();
 }
 
-// ---- library primitives (rocPRIM through hipCUB) ------------------------------
+hipError_t launch_decode(const SegView& seg, uint64_t count, uint64_t* out, hipStream_t s) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(decode_kernel, dim3(grid_for(count, 256, 16384)), dim3(256), 0, s, seg, count, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_query(const SegView* v1, uint32_t k1, const SegView* v0, uint32_t k0, const uint32_t* pairs,
+                        uint32_t n, uint64_t q, int32_t* out, hipStream_t s) {
+  if (!q) return hipSuccess;
+  hipLaunchKernelGGL(query_kernel, dim3(grid_for(q * 64, 256, 8192)), dim3(256), 0, s, v1, k1, v0, k0, pairs, n, q,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_u64(const uint64_t* src, uint64_t count, uint64_t* dst, hipStream_t s) {
+  if (!count) return hipSuccess;
+  hipLaunchKernelGGL(copy_u64_kernel, dim3(grid_for(count, 256, 16384)), dim3(256), 0, s, src, count, dst);
+  return hipGetLastError();
+}
+
+// ---- library primitive (rocPRIM through hipCUB), for the device degree order -------
 
 hipError_t sort_pairs_u64(void* tmp, size_t& tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint64_t* vin,
                           uint64_t* vout, uint64_t m, int end_bit, hipStream_t s) {
   return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kin, kout, vin, vout, (int)m, 0, end_bit, s);
-}
-
-hipError_t exclusive_scan_u32_u64(void* tmp, size_t& tmp_bytes, const uint32_t* in, uint64_t* out, uint64_t m,
-                                  hipStream_t s) {
-  return hipcub::DeviceScan::ExclusiveScan(tmp, tmp_bytes, in, out, hipcub::Sum(), (uint64_t)0, (int)m, s);
-}
-
-hipError_t select_flagged_u64(void* tmp, size_t& tmp_bytes, const uint64_t* in, const uint8_t* flags, uint64_t* out,
-                              uint64_t* nsel, uint64_t m, hipStream_t s) {
-  return hipcub::DeviceSelect::Flagged(tmp, tmp_bytes, in, flags, out, nsel, (int)m, s);
 }
 
 }  // namespace chl

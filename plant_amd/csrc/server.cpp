@@ -112,10 +112,9 @@ struct Pool {
 
 struct chl_server {
   const chl_labels* labels = nullptr;
-  const uint64_t* rng1 = nullptr;  // per vertex id: (begin, end) of its side-1 row
-  const uint64_t* ent1 = nullptr;
-  const uint64_t* rng0 = nullptr;
-  const uint64_t* ent0 = nullptr;
+  const SegView* v1 = nullptr;  // device views of the side-1 / side-0 segments
+  const SegView* v0 = nullptr;
+  uint32_t k1 = 0, k0 = 0;
   uint32_t n = 0;
   int device = 0;
   chl_server_opts o;
@@ -151,8 +150,8 @@ struct chl_server {
 
 namespace chl {
 // Device views of a label handle (defined in capi_device.cpp).
-void labels_views(const chl_labels* l, const uint64_t** rng1, const uint64_t** ent1, const uint64_t** rng0,
-                  const uint64_t** ent0, uint32_t* n, int* device);
+void labels_views(const chl_labels* l, const SegView** v1, uint32_t* k1, const SegView** v0, uint32_t* k0,
+                  uint32_t* n, int* device);
 }
 
 int chl_server::launch(Slot& s) {
@@ -165,7 +164,7 @@ int chl_server::launch(Slot& s) {
     std::lock_guard<std::mutex> lk(mu);
     for (Request* r : s.group) arena_pool.release(r->arena_off, (uint64_t)r->q);
   }
-  CHL_HIP(launch_query(rng1, ent1, rng0, ent0, s.d_pairs, n, s.total, s.d_out, s.stream));
+  CHL_HIP(launch_query(v1, k1, v0, k0, s.d_pairs, n, s.total, s.d_out, s.stream));
   CHL_HIP(hipEventRecord(s.done, s.stream));
   return CHL_OK;
 }
@@ -268,7 +267,7 @@ int chl_server_create(const chl_labels* l, const chl_server_opts* opts, chl_serv
   std::unique_ptr<chl_server> s(new chl_server());
   s->labels = l;
   s->o = o;
-  labels_views(l, &s->rng1, &s->ent1, &s->rng0, &s->ent0, &s->n, &s->device);
+  labels_views(l, &s->v1, &s->k1, &s->v0, &s->k0, &s->n, &s->device);
   CHL_HIP(hipSetDevice(s->device));
   s->pool.init((uint64_t)o.max_inflight_pairs);
   // queue arena: twice the pool (the back-pressure limit) plus one request;
@@ -295,7 +294,7 @@ int chl_server_create(const chl_labels* l, const chl_server_opts* opts, chl_serv
   // stream's first use (device queue set-up takes milliseconds)
   for (auto& sl : s->slots) {
     sl.h_pairs[0] = sl.h_pairs[1] = 0;
-    CHL_HIP(launch_query(s->rng1, s->ent1, s->rng0, s->ent0, sl.d_pairs, s->n, 1, sl.d_out, sl.stream));
+    CHL_HIP(launch_query(s->v1, s->k1, s->v0, s->k0, sl.d_pairs, s->n, 1, sl.d_out, sl.stream));
     CHL_HIP(hipStreamSynchronize(sl.stream));
   }
   chl_server* raw = s.release();

@@ -149,7 +149,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
   __shared__ uint32_t s_val[PRUNE ? kLdsHash : 1];
   __shared__ uint32_t s_ri, s_fsize, s_nsize, s_nvis, s_below, s_clean, s_minnext, s_emit, s_dsize, s_dmin;
   __shared__ unsigned long long s_base;
-  __shared__ unsigned long long s_ctr[4];
+  __shared__ unsigned long long s_ctr[6];
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63;
@@ -160,8 +160,12 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
   uint32_t* vis = p.vis + slot * n;
   uint64_t* D = p.qd + slot * n;  // deferred pile: (vertex << 32) | stamp at deferral
 
-  unsigned long long c_proc = 0, c_arcs = 0, c_prune = 0, c_rounds = 0;
-  if (tid < 4) s_ctr[tid] = 0;
+  unsigned long long c_proc = 0, c_arcs = 0, c_prune = 0, c_rounds = 0, c_settled = 0, c_sarcs = 0;
+  uint32_t c_maxd = 0;
+  // this root's share, added to the launch counters only when its labels fit
+  // (a spilled root runs again in a later launch and is counted there)
+  unsigned long long r_arcs = 0, r_prune = 0, r_settled = 0, r_sarcs = 0;
+  if (tid < 6) s_ctr[tid] = 0;
 
   for (;;) {
     if (tid == 0) {
@@ -190,8 +194,10 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
     uint32_t* tval = s_val;
     uint32_t tshift = 32 - kLdsHashBits, tmask = kLdsHash - 1;
     if (PRUNE) {
-      const uint64_t rb = p.root_off[rr], re = p.root_off[rr + 1];
-      if (2 * (re - rb) > kLdsHash) {
+      // the root's row over all segments of the root side
+      uint64_t rlen = 0;
+      for (uint32_t sg = 0; sg < p.root_nseg; ++sg) rlen += p.root_view[sg].off[rr + 1] - p.root_view[sg].off[rr];
+      if (2 * rlen > kLdsHash) {
         tkey = p.gkey + slot * p.gcap;
         tval = p.gval + slot * p.gcap;
         tmask = p.gcap - 1;
@@ -199,9 +205,14 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
       }
       for (uint32_t i = tid; i <= tmask; i += kThreads) tkey[i] = kEmpty;
       __syncthreads();
-      for (uint64_t k = rb + tid; k < re; k += kThreads) {
-        uint64_t e = p.root_ent[k];
-        table_insert(tkey, tval, tshift, tmask, (uint32_t)(e >> 32), (uint32_t)e);
+      for (uint32_t sg = 0; sg < p.root_nseg; ++sg) {
+        const SegView rv = p.root_view[sg];
+        const uint64_t rb = rv.off[rr], re = rv.off[rr + 1];
+        for (uint64_t k = rb + tid; k < re; k += kThreads) {
+          uint32_t h, d;
+          seg_entry(rv, k, h, d);
+          table_insert(tkey, tval, tshift, tmask, h, d);
+        }
       }
     }
     if (tid == 0) {
@@ -218,6 +229,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
       s_dmin = kEmpty;
     }
     uint32_t B = p.delta;
+    uint32_t bfin = kEmpty;  // the bucket bound where the search stopped (all ones: it ran out)
     uint32_t round = 1;
     __syncthreads();
 
@@ -229,7 +241,10 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
       const bool adv = s_below == 0;
       if (adv) {
         // bucket [.., B) is final; every queued vertex is at >= B
-        if (FLAGS && s_clean == 0) break;
+        if (FLAGS && s_clean == 0) {
+          bfin = B;
+          break;
+        }
         B = min(s_minnext, s_dmin) + p.delta;
       }
       const uint32_t dscan = adv ? dsize : 0u;
@@ -304,7 +319,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
           }
           if (!FLAGS && rank_of(cell, u) < skip) continue;
           ++c_proc;
-          c_arcs += ae - ab;
+          r_arcs += ae - ab;
           for (uint32_t a0 = ab; a0 < ae; a0 += kArcStep) {
             uint64_t av[kArcStep];
             ulonglong2 seen[kArcStep];
@@ -338,29 +353,35 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
           if (ru < skip || lo == 0) continue;  // lo == 0: pruned at this distance already
           if (lane == 0) ++c_proc;
           bool pruned = false;
-          const uint64_t rb = p.scan_off[ru], re = p.scan_off[ru + 1];
-          uint64_t base = rb;
-          for (; base < re; base += 64) {
-            const uint64_t kk = base + lane;
-            bool hit = false;
-            if (kk < re) {
-              const uint64_t e = p.scan_ent[kk];
-              const uint32_t rd = table_find(tkey, tval, tshift, tmask, (uint32_t)(e >> 32));
-              hit = rd != kEmpty && rd + (uint32_t)e <= d;
+          uint64_t scanned = 0;
+          for (uint32_t sg = 0; sg < p.scan_nseg && !pruned; ++sg) {
+            const SegView sv = p.scan_view[sg];
+            const uint64_t rb = sv.off[ru], re = sv.off[ru + 1];
+            uint64_t base = rb;
+            for (; base < re; base += 64) {
+              const uint64_t kk = base + lane;
+              bool hit = false;
+              if (kk < re) {
+                uint32_t h, ed;
+                seg_entry(sv, kk, h, ed);
+                const uint32_t rd = table_find(tkey, tval, tshift, tmask, h);
+                hit = rd != kEmpty && rd + ed <= d;
+              }
+              if (__any(hit)) {
+                pruned = true;
+                break;
+              }
             }
-            if (__any(hit)) {
-              pruned = true;
-              break;
-            }
+            scanned += (pruned ? min(base + 64, re) : re) - rb;
           }
-          if (lane == 0) c_prune += (pruned ? base + 64 : re) - rb;
+          if (lane == 0) r_prune += scanned;
           if (pruned) {
             if (lane == 0)
               atomicMin((unsigned long long*)&cell[2 * (size_t)u], (unsigned long long)((uint64_t)d << 32));
             continue;
           }
           const uint32_t ab = p.off[u], ae = p.off[u + 1];
-          if (lane == 0) c_arcs += ae - ab;
+          if (lane == 0) r_arcs += ae - ab;
           for (uint32_t a = ab + lane; a < ae; a += 64)
             relax<FLAGS>(cell, N, vis, &s_nsize, &s_nvis, nstamp, p.arc[a], d, lo, B, bk);
         }
@@ -415,10 +436,18 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
       const uint32_t v = vis[k];
       const ulonglong2 c = load_cell(cell, v);
       const uint32_t rv = (uint32_t)(c.y >> 32);
-      if ((uint32_t)c.x == rr && rv >= rr) {
+      const uint32_t dv = (uint32_t)(c.x >> 32), lv = (uint32_t)c.x;
+      if (lv == rr && rv >= rr) {
         const uint32_t j = atomicAdd(&s_emit, 1u);
         F[j] = rv;
-        N[j] = (uint32_t)(c.x >> 32);
+        N[j] = dv;
+        c_maxd = max(c_maxd, dv);
+      }
+      if (dv < bfin) {
+        // settled: processed at its final distance; expanded unless pruned
+        // (low 0) or ranked above a GLL root
+        ++r_settled;
+        if ((!PRUNE || lv != 0) && (FLAGS || rv >= skip)) r_sarcs += p.off[v + 1] - p.off[v];
       }
       *reinterpret_cast<ulonglong2*>(cell + 2 * (size_t)v) = make_ulonglong2(kStateInf, (uint64_t)rv << 32);
     }
@@ -444,26 +473,39 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
     const unsigned long long obase = s_base;
     if (obase != ~0ull)
       for (uint32_t k = tid; k < emit; k += kThreads) {
-        p.out_key[obase + k] = ((uint64_t)F[k] << 32) | rr;
-        p.out_ent[obase + k] = ((uint64_t)rr << 32) | N[k];
+        p.out_row[obase + k] = F[k];
+        p.out_hd[obase + k] = ((uint64_t)rr << 32) | N[k];
       }
     // tree size = processed entries of this root
     if (c_proc) atomicAdd(&s_ctr[0], c_proc);
+    if (obase != ~0ull) {
+      c_arcs += r_arcs;
+      c_prune += r_prune;
+      c_settled += r_settled;
+      c_sarcs += r_sarcs;
+    }
+    r_arcs = r_prune = r_settled = r_sarcs = 0;
     __syncthreads();
     if (tid == 0) {
       p.tree_size[ri] = (uint32_t)s_ctr[0];
-      atomicAdd(&s_ctr[1], s_ctr[0]);
+      if (obase != ~0ull) s_ctr[1] += s_ctr[0];
       s_ctr[0] = 0;
     }
     c_proc = 0;
   }
   if (c_arcs) atomicAdd(&s_ctr[2], c_arcs);
   if (c_prune) atomicAdd(&s_ctr[3], c_prune);
+  if (c_settled) atomicAdd(&s_ctr[4], c_settled);
+  if (c_sarcs) atomicAdd(&s_ctr[5], c_sarcs);
+  for (int o = 32; o > 0; o >>= 1) c_maxd = max(c_maxd, (uint32_t)__shfl_xor((int)c_maxd, o));
+  if (lane == 0 && c_maxd) atomicMax(&p.ctr[CTR_MAXD], (unsigned long long)c_maxd);
   __syncthreads();
   if (tid == 0) {
     if (s_ctr[1]) atomicAdd(&p.ctr[CTR_PROCESSED], s_ctr[1]);
     if (s_ctr[2]) atomicAdd(&p.ctr[CTR_ARCS], s_ctr[2]);
     if (s_ctr[3]) atomicAdd(&p.ctr[CTR_PRUNE_ENT], s_ctr[3]);
+    if (s_ctr[4]) atomicAdd(&p.ctr[CTR_SETTLED], s_ctr[4]);
+    if (s_ctr[5]) atomicAdd(&p.ctr[CTR_SETTLED_ARCS], s_ctr[5]);
     if (c_rounds) atomicAdd(&p.ctr[CTR_ROUNDS], c_rounds);
   }
 }

@@ -23,6 +23,14 @@ typedef void* hipStream_t;
 typedef void* hipMemGenericAllocationHandle_t;
 inline const char* hipGetErrorString(hipError_t) { return ""; }
 inline hipError_t hipGetLastError() { return 0; }
+inline hipError_t hipMalloc(void** p, size_t bytes) {
+  *p = malloc(bytes);
+  return *p ? 0 : 2;
+}
+inline hipError_t hipFree(void* p) {
+  free(p);
+  return 0;
+}
 struct dim3 {
   unsigned x, y, z;
   dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
@@ -113,6 +121,7 @@ T __shfl_xor(T x, int m) {
 }
 inline int __clz(int x) { return x ? __builtin_clz((unsigned)x) : 32; }
 template <class T> T min(T a, T b) { return a < b ? a : b; }
+template <class T> T max(T a, T b) { return a < b ? b : a; }
 template <class T, class U> T atomicAdd(T* p, U v) {
   emu_maybe_yield();
   T o = *p;
@@ -135,6 +144,12 @@ template <class T, class U> T atomicMin(T* p, U v) {
   emu_maybe_yield();
   T o = *p;
   if ((T)v < o) *p = (T)v;
+  return o;
+}
+template <class T, class U> T atomicMax(T* p, U v) {
+  emu_maybe_yield();
+  T o = *p;
+  if (o < (T)v) *p = (T)v;
   return o;
 }
 #define __hip_atomic_load(p, o, s) (*(p))

@@ -3,7 +3,8 @@
 // every root of one launch. Input and output are little-endian binary files:
 //   in:  u32 n, m, R, delta, threads, slots, flags; u32 off[n + 1]; u64 arc[m];
 //        u32 rank[n]; u32 roots[R] (vertex ids)
-//   out: u64 labels; (u64 key, u64 ent) per label; u32 1 when every search
+//   out: u64 labels; (u64 key = row << 32 | hub, u64 ent = hub << 32 | dist)
+//        per label; u32 1 when every search
 //        cell came back to its unreached state (rank kept), else 0
 #include "../../plant_amd/csrc/tree.hip"
 
@@ -37,7 +38,8 @@ int main(int argc, char** argv) {
   }
   std::vector<uint32_t> qa(cells), qb(cells), vis(cells), spill(R), ts(R), lc(R), gk(1), gv(1);
   const uint64_t cap = (uint64_t)n * R + 1;
-  std::vector<uint64_t> ok(cap), oe(cap);
+  std::vector<uint32_t> orow(cap);
+  std::vector<uint64_t> ohd(cap);
   std::vector<unsigned long long> ctr(CTR_COUNT, 0);
   ctr[CTR_SPILL_START] = ~0ull;
   TreeParams p{};
@@ -56,8 +58,8 @@ int main(int argc, char** argv) {
   p.gkey = gk.data();
   p.gval = gv.data();
   p.gcap = 1;
-  p.out_key = ok.data();
-  p.out_ent = oe.data();
+  p.out_row = orow.data();
+  p.out_hd = ohd.data();
   p.out_cap = cap;
   p.spill = spill.data();
   p.tree_size = ts.data();
@@ -74,8 +76,9 @@ int main(int argc, char** argv) {
   const uint64_t L = ctr[CTR_LABELS];
   fwrite(&L, 8, 1, o);
   for (uint64_t k = 0; k < L; ++k) {
-    fwrite(&ok[k], 8, 1, o);
-    fwrite(&oe[k], 8, 1, o);
+    const uint64_t key = ((uint64_t)orow[k] << 32) | (ohd[k] >> 32);
+    fwrite(&key, 8, 1, o);
+    fwrite(&ohd[k], 8, 1, o);
   }
   fwrite(&restored, 4, 1, o);
   fclose(o);
