@@ -133,7 +133,9 @@ typedef struct chl_build_opts {
   int64_t out_capacity; /* initial batch output buffer in labels (0 = automatic; it grows as needed) */
   const float* vertex_score; /* optional per-vertex importance score (e.g. sampled betweenness), a
                                 feature of the learned batching predictor; NULL = not used */
-  int reserved[8];
+  int keep_segments;    /* 1: keep the label store's segments as the build left them (no final merge);
+                           0: merge them into one where device memory allows */
+  int reserved[7];
 } chl_build_opts;
 
 typedef struct chl_build_stats {
@@ -213,6 +215,13 @@ int chl_labels_copy(const chl_labels* l, int side, uint64_t* off, uint32_t* hub,
 int chl_labels_checksum(const chl_labels* l, int side, uint64_t* sum);
 /* len[v] = entries in vertex v's row of one side (no entry is copied). */
 int chl_labels_row_lengths(const chl_labels* l, int side, uint32_t* len);
+/* The segments of one side (no reference counterpart: the store's layout). A
+ * side is a list of segments with ascending, disjoint hub-rank ranges
+ * [h0, h1); each holds its entries in 5 bytes (fmt 5: hub - h0 < 2^24, dist <=
+ * 65535) or 8 bytes (fmt 8). Writes up to cap segments; returns the count, or a
+ * negative CHL_E_*. */
+int chl_labels_segments(const chl_labels* l, int side, int64_t cap, uint32_t* h0, uint32_t* h1, uint32_t* fmt,
+                        uint64_t* entries);
 void chl_labels_free(chl_labels* l);
 
 /* ---- queries ----------------------------------------------------------------

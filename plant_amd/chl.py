@@ -37,7 +37,7 @@ class BuildOpts(C.Structure):
     _fields_ = [("mode", C.c_int), ("batching", C.c_int), ("autotune", C.c_int), ("batch_roots", C.c_int64),
                 ("delta", C.c_int32), ("slots", C.c_int32), ("granularity", C.c_int), ("switch_ratio", C.c_double),
                 ("plant_prune", C.c_int), ("device", C.c_int), ("verbose", C.c_int), ("out_capacity", C.c_int64),
-                ("vertex_score", C.c_void_p), ("reserved", C.c_int * 8)]
+                ("vertex_score", C.c_void_p), ("keep_segments", C.c_int), ("reserved", C.c_int * 7)]
 
 
 class BuildStats(C.Structure):
@@ -108,6 +108,7 @@ def lib():
             "chl_labels_avg": ([vp], C.c_double),
             "chl_labels_copy": ([vp, C.c_int, vp, vp, vp], C.c_int),
             "chl_labels_row_lengths": ([vp, C.c_int, vp], C.c_int),
+            "chl_labels_segments": ([vp, C.c_int, C.c_int64, vp, vp, vp, vp], C.c_int),
             "chl_labels_checksum": ([vp, C.c_int, C.POINTER(u64)], C.c_int),
             "chl_labels_free": ([vp], None),
             "chl_query": ([vp, vp, i64, vp], C.c_int),
@@ -359,6 +360,16 @@ class Labels:
         out = np.zeros(self.n, np.uint32)
         _check(lib().chl_labels_row_lengths(self._h, side, _ptr(out)))
         return out
+
+    def segments(self, side=0):
+        """The store's segments of one side: list of (h0, h1, fmt, entries)."""
+        k = lib().chl_labels_segments(self._h, side, 0, None, None, None, None)
+        if k < 0:
+            _check(k)
+        h0, h1, fmt = (np.zeros(max(k, 1), np.uint32) for _ in range(3))
+        cnt = np.zeros(max(k, 1), np.uint64)
+        _check(min(0, lib().chl_labels_segments(self._h, side, k, _ptr(h0), _ptr(h1), _ptr(fmt), _ptr(cnt))))
+        return [(int(h0[i]), int(h1[i]), int(fmt[i]), int(cnt[i])) for i in range(k)]
 
     def query(self, pairs):
         """d(u -> v) for each row (u, v) of pairs; INFTY when no hub is shared."""

@@ -15,6 +15,7 @@ using namespace chl;
 
 struct chl_labels {
   LabelStore store;
+  QueryView qv;                  // what queries read (set once the range tables exist)
   DBuf<uint32_t> rank_of;        // vertex id -> row id
   std::vector<uint32_t> row_of;  // host copy of rank_of
   // query staging for host-pointer calls
@@ -36,7 +37,10 @@ int make_handle(chl_labels* l, const std::vector<uint32_t>& row_of) {
   if (!row_of.empty())
     CHL_HIP(hipMemcpy(l->rank_of.p, row_of.data(), sizeof(uint32_t) * row_of.size(), hipMemcpyHostToDevice));
   CHL_HIP(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking));
-  return l->store.build_ranges(l->rank_of.p, l->stream);
+  int rc = l->store.build_ranges(l->rank_of.p, l->stream);
+  if (rc) return rc;
+  l->qv = l->store.query_view();
+  return CHL_OK;
 }
 
 // Host rows in vertex order from the device store.
@@ -88,11 +92,8 @@ int from_host_labels(const Labels& H, int device, chl_labels** out) {
 }  // namespace
 
 namespace chl {
-void labels_views(const chl_labels* l, const SegView** v1, uint32_t* k1, const SegView** v0, uint32_t* k0,
-                  uint32_t* n, int* device) {
-  *v1 = l->store.d_views(1);
-  *v0 = l->store.d_views(0);
-  *k1 = *k0 = l->store.nseg();
+void labels_views(const chl_labels* l, QueryView* qv, uint32_t* n, int* device) {
+  *qv = l->qv;
   *n = (uint32_t)l->store.n;
   *device = l->store.device;
 }
@@ -344,6 +345,19 @@ int chl_labels_row_lengths(const chl_labels* l, int side, uint32_t* len) {
   return CHL_OK;
 }
 
+int chl_labels_segments(const chl_labels* l, int side, int64_t cap, uint32_t* h0, uint32_t* h1, uint32_t* fmt,
+                        uint64_t* entries) {
+  if (!l || side < 0 || side > 1 || cap < 0) return set_error(CHL_E_INVAL, "bad arguments");
+  const auto& S = l->store.side(side);
+  for (size_t i = 0; i < S.size() && (int64_t)i < cap; ++i) {
+    if (h0) h0[i] = S[i]->h0;
+    if (h1) h1[i] = S[i]->h1;
+    if (fmt) fmt[i] = S[i]->fmt;
+    if (entries) entries[i] = S[i]->total;
+  }
+  return (int)S.size();
+}
+
 int chl_labels_checksum(const chl_labels* l, int side, uint64_t* sum) {
   if (!l || side < 0 || side > 1 || !sum) return set_error(CHL_E_INVAL, "bad arguments");
   CHL_HIP(hipSetDevice(l->store.device));
@@ -397,8 +411,7 @@ int chl_query_device(const chl_labels* l, const uint32_t* d_pairs, int64_t q, in
   if (!l || q < 0 || (q > 0 && (!d_pairs || !d_out))) return set_error(CHL_E_INVAL, "bad arguments");
   if (q == 0) return CHL_OK;
   CHL_HIP(hipSetDevice(l->store.device));
-  CHL_HIP(launch_query(l->store.d_views(1), l->store.nseg(), l->store.d_views(0), l->store.nseg(), d_pairs,
-                       (uint32_t)l->store.n, (uint64_t)q, d_out, (hipStream_t)stream));
+  CHL_HIP(launch_query(l->qv, d_pairs, (uint32_t)l->store.n, (uint64_t)q, d_out, (hipStream_t)stream));
   return CHL_OK;
 }
 
@@ -412,8 +425,7 @@ int chl_query(const chl_labels* lc, const uint32_t* pairs, int64_t q, int32_t* o
   CHL_HIP(l->qpairs.reserve(2 * q));
   CHL_HIP(l->qout.reserve(q));
   CHL_HIP(hipMemcpyAsync(l->qpairs.p, pairs, sizeof(uint32_t) * 2 * q, hipMemcpyHostToDevice, l->stream));
-  CHL_HIP(launch_query(l->store.d_views(1), l->store.nseg(), l->store.d_views(0), l->store.nseg(), l->qpairs.p,
-                       (uint32_t)l->store.n, (uint64_t)q, l->qout.p, l->stream));
+  CHL_HIP(launch_query(l->qv, l->qpairs.p, (uint32_t)l->store.n, (uint64_t)q, l->qout.p, l->stream));
   CHL_HIP(hipMemcpyAsync(out, l->qout.p, sizeof(int32_t) * q, hipMemcpyDeviceToHost, l->stream));
   CHL_HIP(hipStreamSynchronize(l->stream));
   return CHL_OK;

@@ -125,11 +125,11 @@ enum : uint32_t { SEG_F5 = 5, SEG_F8 = 8 };
 constexpr uint32_t kSegMax = 48;  // segments per side at most (more are merged first)
 constexpr uint32_t kF5MaxDist = 65535;
 constexpr uint32_t kF5MaxWidth = 1u << 24;
+constexpr uint64_t kF5Pad = 4;
 
 struct SegView {
   const uint64_t* off;  // row offsets, n + 1, segment-relative
-  const uint64_t* rng;  // per vertex id (begin, end) of its row; null where not built
-  const uint32_t* key;  // SEG_F5
+  const uint32_t* key;  // SEG_F5 (allocated with kF5Pad entries to spare, for 4-entry group loads)
   const uint8_t* lo;    // SEG_F5
   const uint64_t* ent;  // SEG_F8
   uint32_t h0;
@@ -344,15 +344,35 @@ hipError_t launch_switch_test(const uint32_t* tree_size, const uint32_t* label_c
 // independent digest; summed over the segments it is the side's digest.
 hipError_t launch_checksum(const SegView& seg, const uint32_t* row_of, uint32_t n, unsigned long long* out,
                            hipStream_t s);
-// rng[2 v], rng[2 v + 1] = off[row_of[v]], off[row_of[v] + 1] for v in [0, n).
-hipError_t launch_row_ranges(const uint64_t* off, const uint32_t* row_of, uint32_t n, uint64_t* rng, hipStream_t s);
+// rng[2 (v k + s)], rng[2 (v k + s) + 1] = the bounds of vertex v's row (row
+// row_of[v]) in segment s, for v in [0, n) and s < k (views: device array).
+hipError_t launch_row_ranges(const SegView* views, uint32_t k, const uint32_t* row_of, uint32_t n, uint64_t* rng,
+                             hipStream_t s);
 // (hub << 32) | dist of every entry of a segment, in entry order.
 hipError_t launch_decode(const SegView& seg, uint64_t count, uint64_t* out, hipStream_t s);
+// What a query reads of a label handle, passed by value as a kernel argument
+// (so segment fields are uniform scalar loads): per side the vertex-major range
+// table of launch_row_ranges and every segment's entry arrays; both sides have
+// nseg segments with the same hub ranges. fmts: SEG_F5 or SEG_F8 when every
+// segment of both sides has that format, else 0.
+constexpr uint32_t kQuerySeg = 16;  // segments per side a query takes (a handle merges more)
+struct QuerySide {
+  const uint64_t* rng;
+  const uint32_t* key[kQuerySeg];
+  const uint8_t* lo[kQuerySeg];
+  const uint64_t* ent[kQuerySeg];
+  uint32_t h0[kQuerySeg];
+  uint32_t fmt[kQuerySeg];
+};
+struct QueryView {
+  QuerySide s1, s0;
+  uint32_t nseg;
+  uint32_t fmts;
+};
 // Label merge-join query (shared_memory/labeling.hpp:56-65) over segmented
-// sides. v1 / v0: device arrays of the side-1 / side-0 views (rng set), k1 / k0
-// segments; pairs hold vertex ids; an id >= n gives -1.
-hipError_t launch_query(const SegView* v1, uint32_t k1, const SegView* v0, uint32_t k0, const uint32_t* pairs,
-                        uint32_t n, uint64_t q, int32_t* out, hipStream_t s);
+// sides; pairs hold vertex ids; an id >= n gives -1.
+hipError_t launch_query(const QueryView& v, const uint32_t* pairs, uint32_t n, uint64_t q, int32_t* out,
+                        hipStream_t s);
 hipError_t launch_copy_u64(const uint64_t* src, uint64_t count, uint64_t* dst, hipStream_t s);
 
 // Library primitive (hipCUB) used by the device degree order: radix sort of

@@ -919,8 +919,10 @@ struct Builder {
       }
       dev_cache_flush();
       CHL_HIP(hipEventRecord(ev[6], stream));
-      rc = L.consolidate(stream);
-      if (rc) return rc;
+      if (!o.keep_segments) {
+        rc = L.consolidate(stream);
+        if (rc) return rc;
+      }
       CHL_HIP(hipEventRecord(ev[7], stream));
       rc = elapsed(ev[6], ev[7], ct);
       if (rc) return rc;

@@ -10,7 +10,7 @@ namespace chl {
 
 // One segment of a label side: the labels of hub ranks [h0, h1) as a CSR over
 // row ids (off, rows + 1 offsets), entries in SEG_F5 (key, lo) or SEG_F8 (ent)
-// form; rng: per vertex id (begin, end) of its row, built for queries.
+// form.
 struct Segment {
   uint32_t h0 = 0, h1 = 0;
   uint32_t fmt = SEG_F8;
@@ -19,7 +19,6 @@ struct Segment {
   BigBuf<uint32_t> key;
   BigBuf<uint8_t> lo;
   BigBuf<uint64_t> ent;
-  BigBuf<uint64_t> rng;
   SegView view() const;
   uint64_t entry_bytes() const;
   int alloc(uint64_t rows, uint64_t count, uint32_t fmt);
@@ -35,6 +34,7 @@ struct LabelStore {
   int device = 0;
   std::vector<std::unique_ptr<Segment>> seg[2];
   DBuf<SegView> dview[2];  // device copies of the segment views
+  BigBuf<uint64_t> rng[2];  // per side, built for queries: vertex-major row bounds in every segment
   uint64_t total[2] = {0, 0};
   int nsides() const { return shared_sides ? 1 : 2; }
   const std::vector<std::unique_ptr<Segment>>& side(int s) const { return seg[shared_sides ? 0 : s]; }
@@ -49,7 +49,9 @@ struct LabelStore {
   int merge_range(size_t a, size_t b, hipStream_t st);
   int tidy(size_t budget, hipStream_t st);
   int consolidate(hipStream_t st);
+  // merges down to kQuerySeg segments, then builds the range tables of queries
   int build_ranges(const uint32_t* row_of, hipStream_t st);
+  QueryView query_view() const;
   int download(int side, std::vector<uint64_t>& off, std::vector<uint64_t>& ent) const;
   int upload(int side, const std::vector<uint64_t>& off, const std::vector<uint64_t>& ent);
 };

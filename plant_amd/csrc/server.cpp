@@ -112,9 +112,7 @@ struct Pool {
 
 struct chl_server {
   const chl_labels* labels = nullptr;
-  const SegView* v1 = nullptr;  // device views of the side-1 / side-0 segments
-  const SegView* v0 = nullptr;
-  uint32_t k1 = 0, k0 = 0;
+  QueryView qv;  // the label handle's segments and range tables
   uint32_t n = 0;
   int device = 0;
   chl_server_opts o;
@@ -150,8 +148,7 @@ struct chl_server {
 
 namespace chl {
 // Device views of a label handle (defined in capi_device.cpp).
-void labels_views(const chl_labels* l, const SegView** v1, uint32_t* k1, const SegView** v0, uint32_t* k0,
-                  uint32_t* n, int* device);
+void labels_views(const chl_labels* l, QueryView* qv, uint32_t* n, int* device);
 }
 
 int chl_server::launch(Slot& s) {
@@ -164,7 +161,7 @@ int chl_server::launch(Slot& s) {
     std::lock_guard<std::mutex> lk(mu);
     for (Request* r : s.group) arena_pool.release(r->arena_off, (uint64_t)r->q);
   }
-  CHL_HIP(launch_query(v1, k1, v0, k0, s.d_pairs, n, s.total, s.d_out, s.stream));
+  CHL_HIP(launch_query(qv, s.d_pairs, n, s.total, s.d_out, s.stream));
   CHL_HIP(hipEventRecord(s.done, s.stream));
   return CHL_OK;
 }
@@ -267,7 +264,7 @@ int chl_server_create(const chl_labels* l, const chl_server_opts* opts, chl_serv
   std::unique_ptr<chl_server> s(new chl_server());
   s->labels = l;
   s->o = o;
-  labels_views(l, &s->v1, &s->k1, &s->v0, &s->k0, &s->n, &s->device);
+  labels_views(l, &s->qv, &s->n, &s->device);
   CHL_HIP(hipSetDevice(s->device));
   s->pool.init((uint64_t)o.max_inflight_pairs);
   // queue arena: twice the pool (the back-pressure limit) plus one request;
@@ -294,7 +291,7 @@ int chl_server_create(const chl_labels* l, const chl_server_opts* opts, chl_serv
   // stream's first use (device queue set-up takes milliseconds)
   for (auto& sl : s->slots) {
     sl.h_pairs[0] = sl.h_pairs[1] = 0;
-    CHL_HIP(launch_query(s->v1, s->k1, s->v0, s->k0, sl.d_pairs, s->n, 1, sl.d_out, sl.stream));
+    CHL_HIP(launch_query(s->qv, sl.d_pairs, s->n, 1, sl.d_out, sl.stream));
     CHL_HIP(hipStreamSynchronize(sl.stream));
   }
   chl_server* raw = s.release();

@@ -24,7 +24,6 @@ SegView Segment::view() const {
   SegView v;
   memset(&v, 0, sizeof(v));
   v.off = off.p;
-  v.rng = rng.p;
   v.key = key.p;
   v.lo = lo.p;
   v.ent = ent.p;
@@ -42,8 +41,8 @@ int Segment::alloc(uint64_t rows, uint64_t count, uint32_t f) {
   if (fmt == SEG_F8) {
     CHL_HIP(ent.alloc(count));
   } else {
-    CHL_HIP(key.alloc(count));
-    CHL_HIP(lo.alloc(count));
+    CHL_HIP(key.alloc(count + kF5Pad));
+    CHL_HIP(lo.alloc(count + kF5Pad));
   }
   return CHL_OK;
 }
@@ -202,14 +201,51 @@ int LabelStore::consolidate(hipStream_t st) {
 }
 
 int LabelStore::build_ranges(const uint32_t* row_of, hipStream_t st) {
-  const uint32_t nn = (uint32_t)n;
-  for (int s = 0; s < nsides(); ++s)
-    for (auto& g : seg[s]) {
-      CHL_HIP(g->rng.alloc(2 * (size_t)std::max<uint32_t>(nn, 1)));
-      CHL_HIP(launch_row_ranges(g->off.p, row_of, nn, g->rng.p, st));
+  while (nseg() > kQuerySeg) {
+    // a kept store of many segments: merge the smallest neighbouring pair
+    size_t best = 0;
+    uint64_t bb = ~0ull;
+    for (size_t i = 0; i + 1 < nseg(); ++i) {
+      const uint64_t c = seg_bytes(*this, i) + seg_bytes(*this, i + 1);
+      if (c < bb) {
+        bb = c;
+        best = i;
+      }
     }
+    int rc = merge_range(best, best + 2, st);
+    if (rc) return rc;
+  }
+  const uint32_t nn = (uint32_t)n, k = nseg();
+  for (int s = 0; s < nsides(); ++s) {
+    CHL_HIP(rng[s].alloc(2 * (size_t)std::max<uint32_t>(nn, 1) * std::max<uint32_t>(k, 1)));
+    if (k) CHL_HIP(launch_row_ranges(dview[s].p, k, row_of, nn, rng[s].p, st));
+  }
   CHL_HIP(hipStreamSynchronize(st));
-  return sync_views();
+  return CHL_OK;
+}
+
+QueryView LabelStore::query_view() const {
+  QueryView q;
+  memset(&q, 0, sizeof(q));
+  q.nseg = nseg();
+  bool all5 = true, all8 = true;
+  for (int side = 0; side < 2; ++side) {
+    QuerySide& Q = side ? q.s1 : q.s0;
+    const int s = shared_sides ? 0 : side;
+    Q.rng = rng[s].p;
+    for (uint32_t i = 0; i < q.nseg && i < kQuerySeg; ++i) {
+      const Segment& g = *seg[s][i];
+      Q.key[i] = g.key.p;
+      Q.lo[i] = g.lo.p;
+      Q.ent[i] = g.ent.p;
+      Q.h0[i] = g.h0;
+      Q.fmt[i] = g.fmt;
+      all5 = all5 && g.fmt == SEG_F5;
+      all8 = all8 && g.fmt == SEG_F8;
+    }
+  }
+  q.fmts = all5 ? SEG_F5 : all8 ? SEG_F8 : 0;
+  return q;
 }
 
 // Host rows in row-id order, entries (hub << 32) | dist.

@@ -1,11 +1,13 @@
 """Parity at the configurations' own sizes: the reference's label file for the
-100x100 grid (byte-identical, every mode); at 1M vertices, the device digest of
-the label set under different batch schedules, query distances against the
-oracle's plain Dijkstra on sampled pairs, and the labels of whole hub-rank
-windows bit for bit against the oracle's PLaNT trees of those ranks (independent
-trees, so a window's oracle labels are exactly the canonical labels whose hub lies
-in it). The digest itself is checked against a host computation over the oracle's
-labels."""
+100x100 grid (byte-identical, every mode); at 1M vertices, the whole label set of
+both sides against the reference's own output for the same graph and order (entry
+counts and digests of its label file, tests/golden/fullsize.json) under both
+batching policies, query distances against the oracle's plain Dijkstra on sampled
+pairs, and the labels of whole hub-rank windows bit for bit against the oracle's
+PLaNT trees of those ranks (independent trees, so a window's oracle labels are
+exactly the canonical labels whose hub lies in it). The digest itself is checked
+against a host computation over the oracle's labels, and label_digest (which made
+the fixtures) against the same host computation in tests/test_oracle.py."""
 import hashlib
 import json
 import os
@@ -17,6 +19,23 @@ pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 MANIFEST = json.load(open(os.path.join(GOLD, "manifest.json")))
+# per-side entry counts and digests of the reference's own label files for the
+# full-size inputs (tests/golden/make_fullsize.py: shared_memory/hhl run on the
+# same graph and order files, its -l output streamed through label_digest)
+FULLSIZE = json.load(open(os.path.join(GOLD, "fullsize.json")))
+
+
+def whole_set_check(chl, L, name, g, order, tmp_path):
+    """Every label of both sides against the reference's own output, by count
+    and by the order-independent digest of chl_labels_checksum."""
+    f = FULLSIZE[name]
+    op = tmp_path / f"{name}.order"
+    chl.order_write(str(op), order)
+    assert hashlib.sha256(op.read_bytes()).hexdigest() == f["order_sha256"]
+    assert g.n == f["n"]
+    for side in (0, 1):
+        assert L.total(side) == f["count"][side], (name, side)
+        assert L.checksum(side) == int(f["digest"][side], 16), (name, side)
 
 
 def host_digest(off, hub, dist):
@@ -104,14 +123,14 @@ def test_c1_grid100_label_file_is_the_reference_file(chl, orc, tmp_path):
     assert np.array_equal(L.query(pairs), D[pairs[:, 0], pairs[:, 1]])
 
 
-def test_c2_road_1M_schedules_agree_and_match_dijkstra(chl, orc):
+def test_c2_road_1M_whole_set_and_dijkstra(chl, orc, tmp_path):
     # C2 at its own size: 1M points, 3-NN, 256-root sampled-betweenness order
     g = chl.Graph.knn(1_000_000, 3, seed=1)
     order = g.order_betweenness(256, seed=7)
     learned = chl.build(g, order, batching=chl.BATCH_LEARNED)
+    whole_set_check(chl, learned, "c2", g, order, tmp_path)
     static = chl.build(g, order, batching=chl.BATCH_STATIC)
-    assert learned.stats.labels == static.stats.labels
-    assert learned.checksum(0) == static.checksum(0)
+    whole_set_check(chl, static, "c2", g, order, tmp_path)
     oracle_dijkstra_check(orc, g, learned)
     del static
     n = g.n
@@ -119,14 +138,14 @@ def test_c2_road_1M_schedules_agree_and_match_dijkstra(chl, orc):
 
 
 @pytest.mark.parametrize("wmax", [1, 16])
-def test_c3_rmat20_hybrid_schedules_agree_and_match_dijkstra(chl, orc, wmax):
+def test_c3_rmat20_hybrid_whole_set_and_dijkstra(chl, orc, wmax, tmp_path):
     # C3 at its own size: R-MAT scale 20, edge factor 3, degree order, hybrid
     g = chl.Graph.rmat(20, 3 << 20, wmax=wmax, seed=1)
     order = g.order_degree(seed=2)
     learned = chl.build(g, order, mode=chl.MODE_HYBRID, batching=chl.BATCH_LEARNED)
+    whole_set_check(chl, learned, "c3u" if wmax == 1 else "c3w", g, order, tmp_path)
     static = chl.build(g, order, mode=chl.MODE_HYBRID, batching=chl.BATCH_STATIC, switch_ratio=16.0)
-    assert learned.stats.labels == static.stats.labels
-    assert learned.checksum(0) == static.checksum(0)
+    whole_set_check(chl, static, "c3u" if wmax == 1 else "c3w", g, order, tmp_path)
     oracle_dijkstra_check(orc, g, learned)
     # windows on both sides of the switch to pruned trees
     n, sr = g.n, max(int(learned.stats.switch_rank), 0)

@@ -9,6 +9,7 @@
 * planted mistakes in labels are caught by the same checks.
 """
 import hashlib
+import subprocess
 import json
 import os
 
@@ -189,3 +190,66 @@ def test_planted_mistakes_are_caught(chl, orc):
     o0b[v + 1:] -= 1
     Lb = orc.OLabels.from_csr(g.n, (o0b, h0[keep], d0[keep]), s1)
     assert not np.array_equal(Lb.query(pairs), D)
+
+
+def parse_label_file(path):
+    """(off, hub, dist) per side of a label file (labeling.hpp:167-183 layout)."""
+    lines = open(path).read().split("\n")
+    n = int(lines[0])
+    out = []
+    for side in (0, 1):
+        off, hub, dist = [0], [], []
+        for v in range(n):
+            t = list(map(int, lines[1 + 2 * v + side].split()))
+            hub += t[1::2][: t[0]]
+            dist += t[2::2][: t[0]]
+            off.append(off[-1] + t[0])
+        out.append((np.array(off), np.array(hub, np.uint32), np.array(dist, np.int32)))
+    return out
+
+
+@pytest.fixture(scope="module")
+def label_digest(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("digest") / "label_digest")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(GOLD, "label_digest.cpp")], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("name", SMALL)
+def test_label_digest_tool_matches_host_digest(label_digest, name, tmp_path):
+    """tests/golden/label_digest (which turns the reference's full-size label files
+    into fixtures) against the numpy restatement of chl_labels_checksum, and a
+    planted one-unit change moves its digest."""
+    from test_gpu_fullsize import host_digest
+    path = os.path.join(GOLD, f"{name}.labels")
+    got = json.loads(subprocess.run([label_digest, path], capture_output=True, text=True, check=True).stdout)
+    sides = parse_label_file(path)
+    assert got["count"] == [int(s[0][-1]) for s in sides]
+    assert [int(x, 16) for x in got["digest"]] == [host_digest(*s) for s in sides]
+    text = open(path).read().split("\n")
+    t = text[1].split()
+    t[2] = str(int(t[2]) + 1)  # distance of the first entry of vertex 0, side 0
+    text[1] = " ".join(t)
+    planted = tmp_path / "planted.labels"
+    planted.write_text("\n".join(text))
+    again = json.loads(subprocess.run([label_digest, str(planted)], capture_output=True, text=True,
+                                      check=True).stdout)
+    assert again["digest"][0] != got["digest"][0] and again["digest"][1] == got["digest"][1]
+
+
+@pytest.mark.parametrize("name", ["c3u", "c3w", "c2"])
+def test_fullsize_fixture_inputs_are_the_generators(chl, name, tmp_path):
+    """The inputs behind tests/golden/fullsize.json are what this project's
+    generators give for the stated arguments (graph and order file sha256)."""
+    f = json.load(open(os.path.join(GOLD, "fullsize.json")))[name]
+    if name == "c2":
+        g = chl.Graph.knn(1_000_000, 3, seed=1)
+        order = g.order_betweenness(256, seed=7)
+    else:
+        g = chl.Graph.rmat(20, 3 << 20, wmax=1 if name == "c3u" else 16, seed=1)
+        order = g.order_degree(seed=2)
+    gp, op = tmp_path / "g.gr", tmp_path / "g.order"
+    g.write(str(gp))
+    chl.order_write(str(op), order)
+    assert hashlib.sha256(gp.read_bytes()).hexdigest() == f["graph_sha256"]
+    assert hashlib.sha256(op.read_bytes()).hexdigest() == f["order_sha256"]
