@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <queue>
+#include <thread>
 
 #include "../../include/chl.h"
 #include "host.hpp"
@@ -165,6 +166,38 @@ void sample_roots(int64_t n, int64_t roots, uint64_t seed, std::vector<uint32_t>
   }
 }
 
+// The trees run on a copy of the graph renumbered in breadth-first order from a
+// vertex of maximum degree (over both arc ranges), so that a search front
+// touches few cache lines; the parent rule still compares original ids. With
+// arc lengths below kDialMax the queue is a circular array of one bucket per
+// distance (Dial's algorithm: every vertex is popped at its final distance in
+// O(arcs + max distance) per tree); longer arcs use a binary heap.
+namespace {
+constexpr int32_t kDialMax = 1 << 16;
+
+struct LocalCsr {
+  std::vector<uint64_t> off;
+  std::vector<uint32_t> head;
+  std::vector<int32_t> len;
+};
+
+void local_csr(const Graph& g, int range, const std::vector<uint32_t>& inv, const std::vector<uint32_t>& pos,
+               LocalCsr& c) {
+  const int64_t n = g.n;
+  c.off.assign(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) c.off[i + 1] = c.off[i] + (g.off[range][inv[i] + 1] - g.off[range][inv[i]]);
+  c.head.resize(c.off[n]);
+  c.len.resize(c.off[n]);
+  parallel_for(n, host_threads(), 4096, [&](int64_t i) {
+    uint64_t o = c.off[i];
+    for (uint64_t a = g.off[range][inv[i]]; a < g.off[range][inv[i] + 1]; ++a, ++o) {
+      c.head[o] = pos[g.head[range][a]];
+      c.len[o] = g.len[range][a];
+    }
+  });
+}
+}  // namespace
+
 void order_betweenness(const Graph& g, int64_t roots, uint64_t seed, int threads, std::vector<uint32_t>& order,
                        std::vector<uint64_t>* score_out) {
   const int64_t n = g.n;
@@ -172,56 +205,136 @@ void order_betweenness(const Graph& g, int64_t roots, uint64_t seed, int threads
   std::vector<uint32_t> rs;
   sample_roots(n, roots, seed, rs);
   if (threads <= 0) threads = host_threads();
+  // breadth-first renumbering (inv: new id -> vertex, pos: vertex -> new id)
+  std::vector<uint32_t> inv, pos(n, kNone);
+  inv.reserve(n);
+  {
+    int64_t best = 0;
+    uint64_t bdeg = 0;
+    for (int64_t v = 0; v < n; ++v) {
+      const uint64_t d = (g.off[1][v + 1] - g.off[1][v]) + (g.off[0][v + 1] - g.off[0][v]);
+      if (d > bdeg) {
+        bdeg = d;
+        best = v;
+      }
+    }
+    for (int64_t k = -1; k < n; ++k) {
+      const uint32_t s = (uint32_t)(k < 0 ? best : k);
+      if (pos[s] != kNone) continue;
+      size_t head = inv.size();
+      pos[s] = (uint32_t)inv.size();
+      inv.push_back(s);
+      while (head < inv.size()) {
+        const uint32_t u = inv[head++];
+        for (int rg = 1; rg >= 0; --rg)
+          for (uint64_t a = g.off[rg][u]; a < g.off[rg][u + 1]; ++a) {
+            const uint32_t v = g.head[rg][a];
+            if (pos[v] == kNone) {
+              pos[v] = (uint32_t)inv.size();
+              inv.push_back(v);
+            }
+          }
+      }
+    }
+  }
+  LocalCsr out, in;
+  local_csr(g, 1, inv, pos, out);
+  local_csr(g, 0, inv, pos, in);
+  int32_t maxw = 0;
+  for (int32_t w : out.len) maxw = std::max(maxw, w);
+  const bool dial = maxw < kDialMax;
+  const size_t nb = dial ? (size_t)maxw + 1 : 1;
+
   int T = (int)std::min<int64_t>(threads, std::max<int64_t>(1, roots));
-  std::vector<std::vector<uint64_t>> part(T, std::vector<uint64_t>(n, 0));
+  std::vector<std::vector<uint64_t>> part(T);
   std::atomic_int64_t next(0);
   std::vector<std::thread> ts;
   for (int t = 0; t < T; ++t)
     ts.emplace_back([&, t]() {
+      part[t].assign(n, 0);
       std::vector<int32_t> dist(n, kInf);
       std::vector<uint32_t> sub(n, 0), settled;
       settled.reserve(n);
+      std::vector<std::vector<uint32_t>> bucket(nb);
       typedef std::pair<int32_t, uint32_t> QE;
       while (true) {
-        int64_t k = next.fetch_add(1);
+        const int64_t k = next.fetch_add(1);
         if (k >= roots) break;
-        uint32_t src = rs[k];
-        std::priority_queue<QE, std::vector<QE>, std::greater<QE>> pq;
+        const uint32_t src = pos[rs[k]];
         settled.clear();
         dist[src] = 0;
-        pq.push({0, src});
-        while (!pq.empty()) {
-          QE q = pq.top();
-          pq.pop();
-          if (q.first != dist[q.second]) continue;
-          uint32_t u = q.second;
-          settled.push_back(u);
-          for (uint64_t a = g.off[1][u]; a < g.off[1][u + 1]; ++a) {
-            int32_t nd = q.first + g.len[1][a];
-            uint32_t h = g.head[1][a];
-            if (nd < dist[h]) { dist[h] = nd; pq.push({nd, h}); }
+        if (dial) {
+          // every queued distance lies in [cur, cur + maxw]: bucket d % nb
+          size_t queued = 1;
+          bucket[0].push_back(src);
+          for (int64_t cur = 0; queued; ++cur) {
+            std::vector<uint32_t>& B = bucket[(size_t)(cur % (int64_t)nb)];
+            for (size_t i = 0; i < B.size(); ++i) {  // B may grow only through zero-length arcs, which are absent
+              const uint32_t u = B[i];
+              --queued;
+              if (dist[u] != cur) continue;
+              settled.push_back(u);
+              for (uint64_t a = out.off[u]; a < out.off[u + 1]; ++a) {
+                const int32_t nd = (int32_t)cur + out.len[a];
+                const uint32_t h = out.head[a];
+                if (nd < dist[h]) {
+                  dist[h] = nd;
+                  bucket[(size_t)(nd % (int64_t)nb)].push_back(h);
+                  ++queued;
+                }
+              }
+            }
+            B.clear();
+          }
+        } else {
+          std::priority_queue<QE, std::vector<QE>, std::greater<QE>> pq;
+          pq.push({0, src});
+          while (!pq.empty()) {
+            QE q = pq.top();
+            pq.pop();
+            if (q.first != dist[q.second]) continue;
+            const uint32_t u = q.second;
+            settled.push_back(u);
+            for (uint64_t a = out.off[u]; a < out.off[u + 1]; ++a) {
+              const int32_t nd = q.first + out.len[a];
+              const uint32_t h = out.head[a];
+              if (nd < dist[h]) {
+                dist[h] = nd;
+                pq.push({nd, h});
+              }
+            }
           }
         }
-        // descendants accumulate child -> parent in reverse settle order
+        // descendants accumulate child -> parent in reverse settle order; the
+        // parent of v is its shortest-path predecessor of smallest original id
         for (uint32_t v : settled) sub[v] = 1;
         for (int64_t idx = (int64_t)settled.size() - 1; idx >= 1; --idx) {
-          uint32_t v = settled[idx];
-          uint32_t p = kNone;
-          for (uint64_t a = g.off[0][v]; a < g.off[0][v + 1]; ++a) {
-            uint32_t u = g.head[0][a];
-            if (dist[u] != kInf && (int64_t)dist[u] + g.len[0][a] == dist[v] && u < p) p = u;
+          const uint32_t v = settled[idx];
+          uint32_t p = kNone, pid = kNone;
+          for (uint64_t a = in.off[v]; a < in.off[v + 1]; ++a) {
+            const uint32_t u = in.head[a];
+            if (dist[u] != kInf && (int64_t)dist[u] + in.len[a] == dist[v] && inv[u] < pid) {
+              pid = inv[u];
+              p = u;
+            }
           }
           sub[p] += sub[v];
-          part[t][v] += sub[v] - 1;
+          part[t][inv[v]] += sub[v] - 1;
         }
-        part[t][src] += sub[src] - 1;
-        for (uint32_t v : settled) { dist[v] = kInf; sub[v] = 0; }
+        part[t][inv[src]] += sub[src] - 1;
+        for (uint32_t v : settled) {
+          dist[v] = kInf;
+          sub[v] = 0;
+        }
       }
     });
   for (auto& th : ts) th.join();
   std::vector<uint64_t> score(n, 0);
-  for (int t = 0; t < T; ++t)
-    for (int64_t v = 0; v < n; ++v) score[v] += part[t][v];
+  parallel_for(n, host_threads(), 1 << 16, [&](int64_t v) {
+    uint64_t s = 0;
+    for (int t = 0; t < T; ++t) s += part[t][v];
+    score[v] = s;
+  });
   order.resize(n);
   for (int64_t v = 0; v < n; ++v) order[v] = (uint32_t)v;
   std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
