@@ -233,6 +233,16 @@ int chl_query(const chl_labels* l, const uint32_t* pairs, int64_t q, int32_t* ou
  * returns once enqueued. A pair with an id >= n gives -1 (chl_query rejects it
  * with CHL_E_RANGE instead). */
 int chl_query_device(const chl_labels* l, const uint32_t* d_pairs, int64_t q, int32_t* d_out, void* stream);
+/* chl_query without any device allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Pairs and distances are
+ * staged in a fixed device array of the library (12 bytes per pair, one array
+ * per device, held until the process ends) instead of buffers the handle
+ * allocates; at most chl_query_noalloc_max() pairs per call (CHL_E_RANGE
+ * beyond). Calls that overlap on one device share that array and race: the
+ * caller serialises them. Same results and errors as chl_query otherwise. */
+int chl_query_noalloc(const chl_labels* l, const uint32_t* pairs, int64_t q, int32_t* out);
+int64_t chl_query_noalloc_max(void);
 
 /* Query sources and results of distributed_memory/query.hpp:18-51:
  * chl_query_generate = generate_query: 2q values rand() % range from the C

@@ -112,6 +112,8 @@ def lib():
             "chl_labels_checksum": ([vp, C.c_int, C.POINTER(u64)], C.c_int),
             "chl_labels_free": ([vp], None),
             "chl_query": ([vp, vp, i64, vp], C.c_int),
+            "chl_query_noalloc": ([vp, vp, i64, vp], C.c_int),
+            "chl_query_noalloc_max": ([], i64),
             "chl_query_device": ([vp, vp, i64, vp, vp], C.c_int),
             "chl_query_generate": ([i64, C.c_uint32, vp], C.c_int),
             "chl_gen_local_pairs": ([vp, i64, C.c_int, u64, vp], C.c_int),
@@ -376,6 +378,14 @@ class Labels:
         pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
         out = np.zeros(pairs.shape[0], np.int32)
         _check(lib().chl_query(self._h, _ptr(pairs), pairs.shape[0], _ptr(out)))
+        return out
+
+    def query_noalloc(self, pairs):
+        """query() through chl_query_noalloc: no device allocation, staging in the
+        library's fixed device array (at most chl_query_noalloc_max() pairs)."""
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        out = np.zeros(pairs.shape[0], np.int32)
+        _check(lib().chl_query_noalloc(self._h, _ptr(pairs), pairs.shape[0], _ptr(out)))
         return out
 
 

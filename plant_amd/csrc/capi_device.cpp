@@ -97,6 +97,7 @@ void labels_views(const chl_labels* l, QueryView* qv, uint32_t* n, int* device) 
   *n = (uint32_t)l->store.n;
   *device = l->store.device;
 }
+hipStream_t labels_stream(const chl_labels* l) { return l->stream; }
 }  // namespace chl
 
 extern "C" {
