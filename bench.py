@@ -67,47 +67,23 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-class SOpts(C.Structure):
-    _fields_ = [("streams", C.c_int), ("max_inflight_pairs", C.c_int64), ("max_batch_pairs", C.c_int64),
-                ("reserved", C.c_int * 8)]
-
-
-class SStats(C.Structure):
-    _fields_ = [("completed", C.c_int64), ("pairs", C.c_int64), ("rejected", C.c_int64),
-                ("p50_us", C.c_double), ("p99_us", C.c_double), ("mean_us", C.c_double),
-                ("max_us", C.c_double), ("qps", C.c_double), ("p90_us", C.c_double), ("p999_us", C.c_double),
-                ("reserved", C.c_double * 6)]
-
-
-def server_lib(chl):
-    lib = chl.lib()
-    vp = C.c_void_p
-    lib.chl_server_opts_default.argtypes = [C.POINTER(SOpts)]
-    lib.chl_server_create.argtypes = [vp, C.POINTER(SOpts), C.POINTER(vp)]
-    lib.chl_server_replay.argtypes = [vp, vp, vp, vp, C.c_int64, vp]
-    lib.chl_server_stats_get.argtypes = [vp, C.POINTER(SStats)]
-    lib.chl_server_stats_reset.argtypes = [vp]
-    lib.chl_server_free.argtypes = [vp]
-    return lib
-
-
-def replay(lib, srv, pairs, sizes, arrival_us):
+def replay(srv, L, pairs, sizes, arrival_us, rng, check=65536):
     """Open-loop replay of a request trace from the library's native submitter;
-    returns the served rate and latency percentiles."""
+    returns the served rate and latency percentiles (latency counts from each
+    request's scheduled arrival), the refused and late submissions, and checks a
+    seeded sample of the served distances against chl_query on the same pairs."""
     out = np.empty(pairs.shape[0], np.int32)
     out.fill(-1)  # fault the pages in before the timed replay
-    sizes = np.ascontiguousarray(sizes, np.int64)
-    arrival_us = np.ascontiguousarray(arrival_us, np.float64)
-    lib.chl_server_stats_reset(srv)
-    rc = lib.chl_server_replay(srv, pairs.ctypes.data, sizes.ctypes.data, arrival_us.ctypes.data, sizes.size,
-                               out.ctypes.data)
-    if rc != 0:
-        raise RuntimeError(lib.chl_last_error().decode())
-    st = SStats()
-    lib.chl_server_stats_get(srv, C.byref(st))
+    srv.reset()
+    srv.replay(pairs, sizes, arrival_us, out)
+    st = srv.stats()
+    pick = rng.choice(pairs.shape[0], min(check, pairs.shape[0]), replace=False)
+    if not np.array_equal(out[pick], L.query(pairs[pick])):
+        raise RuntimeError("served distances differ from chl_query on the same pairs")
     return {"requests": int(st.completed), "pairs": int(st.pairs), "served_queries_per_s": st.qps,
             "p50_us": st.p50_us, "p90_us": st.p90_us, "p99_us": st.p99_us, "p999_us": st.p999_us,
-            "max_us": st.max_us}
+            "max_us": st.max_us, "rejected": int(st.rejected), "late_submissions": int(st.late),
+            "submit_lag_p99_us": st.lag_p99_us, "checked_pairs": int(pick.size)}
 
 
 def query_bench(chl, L, g, seconds=0.2, seed=11, serve=True):
@@ -134,30 +110,23 @@ def query_bench(chl, L, g, seconds=0.2, seed=11, serve=True):
 
 def serving(chl, L, g, rng, seconds):
     n = g.n
-    lib = server_lib(chl)
-    o = SOpts()
-    lib.chl_server_opts_default(C.byref(o))
-    o.streams = 4  # HIP's default hardware queues per process; 8 streams gave a worse p99 at 2.5e8 pairs/s
-    o.max_inflight_pairs = 1 << 21
-    o.max_batch_pairs = 1 << 16
-    srv = C.c_void_p()
-    if lib.chl_server_create(L._h, C.byref(o), C.byref(srv)) != 0:
-        raise RuntimeError(lib.chl_last_error().decode())
+    # 4 streams: HIP's default hardware queues per process; 8 streams gave a worse p99 at 2.5e8 pairs/s
+    srv = chl.Server(L, streams=4, max_inflight_pairs=1 << 21, max_batch_pairs=1 << 16)
     res = {"fixed_4096": []}
     # warm-up: a third of a second of 4096-pair requests at 2e8 pairs/s, so the
     # measured replays see a device and host already at their serving clocks
     nreq = int(2e8 * 0.33 / 4096)
-    replay(lib, srv, rng.integers(0, n, (nreq * 4096, 2), dtype=np.uint32), np.full(nreq, 4096),
-           np.cumsum(rng.exponential(4096.0 / 2e8 * 1e6, nreq)))
+    replay(srv, L, rng.integers(0, n, (nreq * 4096, 2), dtype=np.uint32), np.full(nreq, 4096),
+           np.cumsum(rng.exponential(4096.0 / 2e8 * 1e6, nreq)), rng)
     for rate in (1.0e8, 2.0e8, 2.2e8, 2.5e8, 3.0e8):
         nreq = int(rate * seconds / 4096)
         arrivals = np.cumsum(rng.exponential(4096.0 / rate * 1e6, nreq))
         pairs = rng.integers(0, n, (nreq * 4096, 2), dtype=np.uint32)
-        r = replay(lib, srv, pairs, np.full(nreq, 4096), arrivals)
+        r = replay(srv, L, pairs, np.full(nreq, 4096), arrivals, rng)
         r["offered_queries_per_s"] = rate
         res["fixed_4096"].append(r)
         log(f"serving 4096-pair requests offered {rate:.2e}/s: served {r['served_queries_per_s']:.3e}/s "
-            f"p50 {r['p50_us']:.0f} us p99 {r['p99_us']:.0f} us")
+            f"p50 {r['p50_us']:.0f} us p99 {r['p99_us']:.0f} us rejected {r['rejected']} late {r['late_submissions']}")
     ok = [r for r in res["fixed_4096"] if r["p99_us"] < 1000.0 and
           r["served_queries_per_s"] >= 0.95 * r["offered_queries_per_s"]]
     best = max(ok, key=lambda r: r["served_queries_per_s"]) if ok else None
@@ -175,54 +144,67 @@ def serving(chl, L, g, rng, seconds):
     pick = rng.random(total) < 0.2
     pairs[pick] = local[rng.integers(0, local.shape[0], int(pick.sum()))]
     arrivals = np.cumsum(rng.exponential(mean_size / rate * 1e6, nreq))
-    r = replay(lib, srv, pairs, sizes, arrivals)
+    r = replay(srv, L, pairs, sizes, arrivals, rng)
     r.update({"offered_queries_per_s": rate, "sizes": "log-uniform [1, 65536]", "local_fraction": 0.2,
               "local_hops": 50})
     res["mix"] = r
     log(f"serving mixed requests offered {rate:.2e}/s: served {r['served_queries_per_s']:.3e}/s "
-        f"p50 {r['p50_us']:.0f} us p99 {r['p99_us']:.0f} us")
-    lib.chl_server_free(srv)
+        f"p50 {r['p50_us']:.0f} us p99 {r['p99_us']:.0f} us rejected {r['rejected']} late {r['late_submissions']}")
+    srv.close()
     return res
+
+
+def hip_check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with HIP error {rc}")
 
 
 def query_kernel_rate(chl, L, pairs, reps=5):
     """The merge-join kernel alone on device-resident pairs (HIP events around
     chl_query_device on one stream), with its algorithmic bytes: per pair the
-    pair, the result, the two row ranges (begin, end) and both label rows."""
+    pair, the result, the two row ranges (begin, end) and both label rows (5 or
+    8 bytes per entry, as stored). The distances of the last call are read back
+    and compared with chl_query on the same pairs."""
     hip = C.CDLL("libamdhip64.so")
     vp = C.c_void_p
     q = pairs.shape[0]
     dp, do = vp(), vp()
-    hip.hipMalloc(C.byref(dp), C.c_size_t(pairs.nbytes))
-    hip.hipMalloc(C.byref(do), C.c_size_t(4 * q))
-    hip.hipMemcpy(dp, pairs.ctypes.data_as(vp), C.c_size_t(pairs.nbytes), 1)  # host to device
+    hip_check(hip.hipMalloc(C.byref(dp), C.c_size_t(pairs.nbytes)), "hipMalloc")
+    hip_check(hip.hipMalloc(C.byref(do), C.c_size_t(4 * q)), "hipMalloc")
+    hip_check(hip.hipMemcpy(dp, pairs.ctypes.data_as(vp), C.c_size_t(pairs.nbytes), 1), "hipMemcpy")  # to device
     stream = vp()
-    hip.hipStreamCreate(C.byref(stream))
+    hip_check(hip.hipStreamCreate(C.byref(stream)), "hipStreamCreate")
     e0, e1 = vp(), vp()
-    hip.hipEventCreate(C.byref(e0))
-    hip.hipEventCreate(C.byref(e1))
+    hip_check(hip.hipEventCreate(C.byref(e0)), "hipEventCreate")
+    hip_check(hip.hipEventCreate(C.byref(e1)), "hipEventCreate")
     lib = chl.lib()
-    lib.chl_query_device(L._h, dp, q, do, stream)  # warm-up
-    hip.hipStreamSynchronize(stream)
-    hip.hipEventRecord(e0, stream)
+    chl._check(lib.chl_query_device(L._h, dp, q, do, stream))  # warm-up
+    hip_check(hip.hipStreamSynchronize(stream), "hipStreamSynchronize")
+    hip_check(hip.hipEventRecord(e0, stream), "hipEventRecord")
     for _ in range(reps):
-        lib.chl_query_device(L._h, dp, q, do, stream)
-    hip.hipEventRecord(e1, stream)
-    hip.hipEventSynchronize(e1)
+        chl._check(lib.chl_query_device(L._h, dp, q, do, stream))
+    hip_check(hip.hipEventRecord(e1, stream), "hipEventRecord")
+    hip_check(hip.hipEventSynchronize(e1), "hipEventSynchronize")
     ms = C.c_float()
-    hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+    hip_check(hip.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+    got = np.empty(q, np.int32)
+    hip_check(hip.hipMemcpy(got.ctypes.data_as(vp), do, C.c_size_t(4 * q), 2), "hipMemcpy")  # to host
     for h in (dp, do):
         hip.hipFree(h)
     hip.hipEventDestroy(e0)
     hip.hipEventDestroy(e1)
     hip.hipStreamDestroy(stream)
+    if not np.array_equal(got, L.query(pairs)):
+        raise RuntimeError("chl_query_device distances differ from chl_query")
     sec = ms.value / 1e3 / reps
     len1, len0 = L.row_lengths(1), L.row_lengths(0)
     rows = len1[pairs[:, 0]].astype(np.float64) + len0[pairs[:, 1]].astype(np.float64)
-    algo = float(q) * (8 + 4 + 32) + 8.0 * rows.sum()
+    segs = L.segments(0)
+    eb = 8.0 if any(f == 8 for _, _, f, _ in segs + L.segments(1)) else 5.0
+    algo = float(q) * (8 + 4 + 16 * 2 * len(segs)) + eb * rows.sum()
     gbs = algo / sec / 1e9
     return {"kernel_1M_pairs_queries_per_s": q / sec, "kernel_ms": sec * 1e3, "kernel_algorithmic_GBps": gbs,
-            "kernel_hbm_frac": gbs / HBM_PEAK_GBS}
+            "kernel_hbm_frac": gbs / HBM_PEAK_GBS, "entry_bytes": eb, "segments": len(segs)}
 
 
 def copy_calibration(nbytes=1 << 30):

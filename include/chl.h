@@ -290,7 +290,9 @@ typedef struct chl_server_stats {
   double p50_us, p99_us, mean_us, max_us;
   double qps;              /* pairs per second since creation / last reset */
   double p90_us, p999_us;  /* more latency percentiles */
-  double reserved[6];
+  int64_t late;            /* replayed requests submitted more than 100 us after their scheduled arrival */
+  double lag_p99_us, lag_max_us; /* replayed requests: submission minus scheduled arrival */
+  double reserved[3];
 } chl_server_stats;
 
 void chl_server_opts_default(chl_server_opts* o);
@@ -304,10 +306,13 @@ int chl_server_drain(chl_server* s);
 int chl_server_stats_get(chl_server* s, chl_server_stats* st);
 void chl_server_stats_reset(chl_server* s);
 /* Replays a request trace from a native thread, open loop: request i (sizes[i]
- * pairs, taken in order from `pairs`) is submitted arrival_us[i] microseconds
- * after the call starts, whatever is still in flight (a request refused by
- * admission is retried, and its wait counts in its latency); then drains. The
- * results land in `out` in the same order; latencies in the server statistics. */
+ * pairs, taken in order from `pairs`) is due arrival_us[i] microseconds after
+ * the call starts and is submitted then, whatever is still in flight (a request
+ * refused by admission is retried every 10 us; each refusal counts in
+ * `rejected`). Latency is measured from the due time, so a late submission and
+ * the wait behind refusals are part of it. Every admitted request has ended
+ * when the call returns, also on an error. The results land in `out` in the
+ * same order; latencies in the server statistics. */
 int chl_server_replay(chl_server* s, const uint32_t* pairs, const int64_t* sizes, const double* arrival_us,
                       int64_t count, int32_t* out);
 void chl_server_free(chl_server* s);

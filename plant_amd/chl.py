@@ -40,6 +40,21 @@ class BuildOpts(C.Structure):
                 ("vertex_score", C.c_void_p), ("keep_segments", C.c_int), ("reserved", C.c_int * 7)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("streams", C.c_int), ("max_inflight_pairs", C.c_int64), ("max_batch_pairs", C.c_int64),
+                ("reserved", C.c_int * 8)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("pairs", C.c_int64), ("rejected", C.c_int64), ("p50_us", C.c_double),
+                ("p99_us", C.c_double), ("mean_us", C.c_double), ("max_us", C.c_double), ("qps", C.c_double),
+                ("p90_us", C.c_double), ("p999_us", C.c_double), ("late", C.c_int64), ("lag_p99_us", C.c_double),
+                ("lag_max_us", C.c_double), ("reserved", C.c_double * 3)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 class BuildStats(C.Structure):
     _fields_ = [("lt", C.c_double), ("st", C.c_double), ("ct", C.c_double), ("total_s", C.c_double),
                 ("avg_label", C.c_double), ("labels", C.c_int64), ("batches", C.c_int64),
@@ -113,6 +128,15 @@ def lib():
             "chl_labels_free": ([vp], None),
             "chl_query": ([vp, vp, i64, vp], C.c_int),
             "chl_query_noalloc": ([vp, vp, i64, vp], C.c_int),
+            "chl_server_opts_default": ([C.POINTER(ServerOpts)], None),
+            "chl_server_create": ([vp, C.POINTER(ServerOpts), pp], C.c_int),
+            "chl_server_submit": ([vp, vp, i64, vp, C.POINTER(C.c_uint64)], C.c_int),
+            "chl_server_poll": ([vp, u64, C.c_int], C.c_int),
+            "chl_server_drain": ([vp], C.c_int),
+            "chl_server_stats_get": ([vp, C.POINTER(ServerStats)], C.c_int),
+            "chl_server_stats_reset": ([vp], None),
+            "chl_server_replay": ([vp, vp, vp, vp, i64, vp], C.c_int),
+            "chl_server_free": ([vp], None),
             "chl_query_noalloc_max": ([], i64),
             "chl_query_device": ([vp, vp, i64, vp, vp], C.c_int),
             "chl_query_generate": ([i64, C.c_uint32, vp], C.c_int),
@@ -387,6 +411,63 @@ class Labels:
         out = np.zeros(pairs.shape[0], np.int32)
         _check(lib().chl_query_noalloc(self._h, _ptr(pairs), pairs.shape[0], _ptr(out)))
         return out
+
+
+class Server:
+    """Continuous-batching query server over a label handle (chl_server_*)."""
+
+    def __init__(self, labels, streams=None, max_inflight_pairs=None, max_batch_pairs=None):
+        o = ServerOpts()
+        lib().chl_server_opts_default(C.byref(o))
+        if streams is not None:
+            o.streams = streams
+        if max_inflight_pairs is not None:
+            o.max_inflight_pairs = max_inflight_pairs
+        if max_batch_pairs is not None:
+            o.max_batch_pairs = max_batch_pairs
+        self.opts = o
+        self._labels = labels  # kept alive while the server runs
+        h = C.c_void_p()
+        _check(lib().chl_server_create(labels._h, C.byref(o), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().chl_server_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, pairs, out):
+        """Queues pairs (contiguous uint32 rows); returns (rc, ticket). out must
+        stay alive until the ticket completes."""
+        t = C.c_uint64()
+        rc = lib().chl_server_submit(self._h, _ptr(pairs), pairs.shape[0], _ptr(out), C.byref(t))
+        return rc, t.value
+
+    def poll(self, ticket, wait=True):
+        return lib().chl_server_poll(self._h, ticket, 1 if wait else 0)
+
+    def drain(self):
+        _check(lib().chl_server_drain(self._h))
+
+    def replay(self, pairs, sizes, arrival_us, out):
+        pairs = np.ascontiguousarray(pairs, np.uint32)
+        sizes = np.ascontiguousarray(sizes, np.int64)
+        arrival_us = np.ascontiguousarray(arrival_us, np.float64)
+        _check(lib().chl_server_replay(self._h, _ptr(pairs), _ptr(sizes), _ptr(arrival_us), sizes.size, _ptr(out)))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().chl_server_stats_get(self._h, C.byref(st)))
+        return st
+
+    def reset(self):
+        lib().chl_server_stats_reset(self._h)
 
 
 def query_generate(q, vertex_range):

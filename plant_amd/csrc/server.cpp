@@ -41,6 +41,9 @@ using namespace chl;
 
 namespace {
 
+// A replayed request submitted this long after its scheduled arrival counts as late.
+constexpr double kLateUs = 100.0;
+
 double now_us() {
   return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -139,12 +142,35 @@ struct chl_server {
 
   // statistics since creation / last reset
   std::vector<double> lat_us;
-  int64_t completed = 0, pair_count = 0, rejected = 0;
+  std::vector<double> lag_us;  // replayed requests: submission minus scheduled arrival
+  int64_t completed = 0, pair_count = 0, rejected = 0, late = 0;
   double t_first = 0, t_last = 0;
 
   void loop();
   int launch(Slot& s);
+  // stops the worker (finishing what is queued and in flight) and frees every
+  // slot's stream, event and pinned staging; also after a failed creation
+  ~chl_server();
 };
+
+chl_server::~chl_server() {
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    stop = true;
+  }
+  cv.notify_all();
+  if (worker.joinable()) worker.join();
+  (void)hipSetDevice(device);
+  for (auto& sl : slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.h_pairs) (void)hipHostFree(sl.h_pairs);
+    if (sl.h_out) (void)hipHostFree(sl.h_out);
+  }
+  for (auto& kv : all) delete kv.second;
+  for (Request* r : spare) delete r;
+}
 
 namespace chl {
 // Device views of a label handle (defined in capi_device.cpp).
@@ -174,8 +200,19 @@ void chl_server::loop() {
     bool progress = false;
     // admission: a free slot takes the queued requests, in arrival order, that
     // fit its staging and one free range of the in-flight budget
+    if (device_error) {
+      // nothing more is admitted: queued requests end with the error
+      std::lock_guard<std::mutex> lk(mu);
+      for (Request* r : pending) {
+        arena_pool.release(r->arena_off, (uint64_t)r->q);
+        done[r->ticket] = true;
+      }
+      queued_pairs = 0;
+      if (!pending.empty()) progress = true;
+      pending.clear();
+    }
     for (auto& s : slots) {
-      if (s.busy) continue;
+      if (s.busy || device_error) continue;
       {
         std::lock_guard<std::mutex> lk(mu);
         if (pending.empty()) break;
@@ -203,8 +240,13 @@ void chl_server::loop() {
     // completion
     for (auto& s : slots) {
       if (!s.busy) continue;
-      const hipError_t e = device_error ? hipSuccess : hipEventQuery(s.done);
-      if (e == hipErrorNotReady) continue;
+      hipError_t e = hipEventQuery(s.done);
+      if (e == hipErrorNotReady) {
+        if (!device_error) continue;
+        // after an error elsewhere the slot is released only once its launch
+        // has ended: its kernel may still read and write the staging
+        e = hipStreamSynchronize(s.stream);
+      }
       const double t = now_us();
       uint64_t pos = 0;
       for (Request* r : s.group) {
@@ -300,12 +342,17 @@ int chl_server_create(const chl_labels* l, const chl_server_opts* opts, chl_serv
   return CHL_OK;
 }
 
-int chl_server_submit(chl_server* s, const uint32_t* pairs, int64_t q, int32_t* out, uint64_t* ticket) {
+}  // extern "C"
+
+namespace {
+// Queues a request whose latency is measured from t_arrival (now for a direct
+// submission, the scheduled arrival for a replayed one).
+int submit_at(chl_server* s, const uint32_t* pairs, int64_t q, int32_t* out, uint64_t* ticket, double t_arrival) {
   if (!s || !ticket || q < 0 || (q > 0 && (!pairs || !out))) return set_error(CHL_E_INVAL, "bad arguments");
   if (q > s->o.max_batch_pairs) return set_error(CHL_E_RANGE, "request of %lld pairs over max_batch_pairs", (long long)q);
   for (int64_t k = 0; k < 2 * q; ++k)
     if (pairs[k] >= s->n) return set_error(CHL_E_RANGE, "vertex %u out of range", pairs[k]);
-  const double t_submit = now_us();
+  const double t_submit = t_arrival;
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->device_error) return s->device_error;
   // queued pairs beyond twice the device pool are refused (back-pressure)
@@ -340,6 +387,13 @@ int chl_server_submit(chl_server* s, const uint32_t* pairs, int64_t q, int32_t* 
   s->cv.notify_all();
   return CHL_OK;
 }
+}  // namespace
+
+extern "C" {
+
+int chl_server_submit(chl_server* s, const uint32_t* pairs, int64_t q, int32_t* out, uint64_t* ticket) {
+  return submit_at(s, pairs, q, out, ticket, now_us());
+}
 
 int chl_server_poll(chl_server* s, uint64_t ticket, int wait) {
   if (!s) return set_error(CHL_E_INVAL, "null server");
@@ -358,7 +412,7 @@ int chl_server_poll(chl_server* s, uint64_t ticket, int wait) {
 int chl_server_drain(chl_server* s) {
   if (!s) return set_error(CHL_E_INVAL, "null server");
   std::unique_lock<std::mutex> lk(s->mu);
-  s->cv.wait(lk, [&] { return s->device_error || (s->pending.empty() && s->inflight == 0); });
+  s->cv.wait(lk, [&] { return s->pending.empty() && s->inflight == 0; });
   return s->device_error;
 }
 
@@ -369,6 +423,13 @@ int chl_server_stats_get(chl_server* s, chl_server_stats* st) {
   st->completed = s->completed;
   st->pairs = s->pair_count;
   st->rejected = s->rejected;
+  st->late = s->late;
+  if (!s->lag_us.empty()) {
+    std::vector<double> v = s->lag_us;
+    std::sort(v.begin(), v.end());
+    st->lag_p99_us = v[(size_t)std::min<double>(v.size() - 1, std::floor(0.99 * (v.size() - 1) + 0.5))];
+    st->lag_max_us = v.back();
+  }
   if (!s->lat_us.empty()) {
     std::vector<double> v = s->lat_us;
     std::sort(v.begin(), v.end());
@@ -396,6 +457,8 @@ void chl_server_stats_reset(chl_server* s) {
   s->completed = 0;
   s->pair_count = 0;
   s->rejected = 0;
+  s->late = 0;
+  s->lag_us.clear();
   s->t_first = 0;
   s->t_last = 0;
 }
@@ -410,54 +473,50 @@ int chl_server_replay(chl_server* s, const uint32_t* pairs, const int64_t* sizes
   (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);
   const double t0 = now_us();
   int64_t pos = 0;
-  for (int64_t i = 0; i < count; ++i) {
+  int rc = CHL_OK;
+  std::vector<double> lags;
+  lags.reserve((size_t)count);
+  int64_t late = 0;
+  for (int64_t i = 0; i < count && rc == CHL_OK; ++i) {
     // open loop: wait for the arrival time, not for earlier completions (sleep,
     // then yield for the last few microseconds)
-    for (double ahead; (ahead = arrival_us[i] - (now_us() - t0)) > 0;) {
+    const double due = t0 + arrival_us[i];
+    for (double ahead; (ahead = due - now_us()) > 0;) {
       if (ahead > 30) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(ahead - 20)));
       else std::this_thread::yield();
     }
     uint64_t t = 0;
     for (;;) {
-      const int rc = chl_server_submit(s, pairs + 2 * pos, sizes[i], out + pos, &t);
-      if (rc == CHL_OK) break;
-      if (rc != CHL_E_BUSY) {
-        (void)prctl(PR_SET_TIMERSLACK, slack, 0, 0, 0);
-        return rc;
-      }
-      // admission refused: the request waits (its latency counts it)
+      // latency counts from the scheduled arrival, so time spent behind a
+      // late submission or a refused admission is part of it
+      rc = submit_at(s, pairs + 2 * pos, sizes[i], out + pos, &t, due);
+      if (rc != CHL_E_BUSY) break;
       std::this_thread::sleep_for(std::chrono::microseconds(10));
     }
+    if (rc != CHL_OK) break;
+    const double lag = now_us() - due;
+    lags.push_back(lag);
+    if (lag > kLateUs) ++late;
     tickets.push_back(t);
     pos += sizes[i];
   }
   (void)prctl(PR_SET_TIMERSLACK, slack, 0, 0, 0);
-  int rc = chl_server_drain(s);
-  if (rc) return rc;
-  for (uint64_t t : tickets)
-    if (chl_server_poll(s, t, 1) < 0) return s->device_error ? (int)s->device_error : CHL_E_STATE;
-  return CHL_OK;
-}
-
-void chl_server_free(chl_server* s) {
-  if (!s) return;
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    s->stop = true;
+    s->lag_us.insert(s->lag_us.end(), lags.begin(), lags.end());
+    s->late += late;
   }
-  s->cv.notify_all();
-  if (s->worker.joinable()) s->worker.join();
-  (void)hipSetDevice(s->device);
-  for (auto& sl : s->slots) {
-    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-    if (sl.done) (void)hipEventDestroy(sl.done);
-    if (sl.stream) (void)hipStreamDestroy(sl.stream);
-    if (sl.h_pairs) (void)hipHostFree(sl.h_pairs);
-    if (sl.h_out) (void)hipHostFree(sl.h_out);
+  // every admitted request ends before the call returns, also on an error, so
+  // nothing writes into `out` afterwards
+  const int drc = chl_server_drain(s);
+  if (rc == CHL_OK) rc = drc;
+  for (uint64_t t : tickets) {
+    const int prc = chl_server_poll(s, t, 1);
+    if (prc < 0 && rc == CHL_OK) rc = prc;
   }
-  for (auto& kv : s->all) delete kv.second;
-  for (Request* r : s->spare) delete r;
-  delete s;
+  return rc;
 }
+
+void chl_server_free(chl_server* s) { delete s; }
 
 }  // extern "C"
