@@ -213,6 +213,9 @@ struct Builder {
   DBuf<uint64_t> scan_tmp, cur;
   SideBatch sb[2];
   static constexpr double kBatchBytes = 32.0;  // device bytes per batch label in flight
+  // mean processed vertices per root above which memory-limited slots run
+  // 1024-thread workgroups
+  static constexpr double kWideTree = 65536.0;
   uint64_t out_cap = 0;
   uint64_t next_need = 0;  // output estimate of the next batch (labels)
   uint64_t out_limit = ~0ull;  // output capacity the memory cap allows for the current batch
@@ -685,6 +688,13 @@ struct Builder {
     rc = alloc_slots(o.autotune && o.slots <= 0 ? 4 * base_slots : slots_for(base_slots, tpr));
     active = std::min<uint32_t>(slots, slots_for(base_slots, tpr));
     if (rc) return rc;
+    // When device memory allows fewer search slots than asked for (a large
+    // graph: every slot holds 36 bytes per vertex), the default granularity
+    // gives each slot a workgroup of 1024 threads while the trees are large, so
+    // the threads in flight stay near what the full slot count would have; once
+    // the mean tree of a batch is small the next batch uses 256 threads again.
+    const bool auto_wide = o.granularity == 0 && !o.autotune && slots < slots_for(base_slots, 256);
+    double last_tree_mean = -1.0;  // processed vertices per root in the last batch (-1: none yet)
     // Static schedule: a fixed width when batch_roots is given; otherwise widths
     // doubling from the slot count up to 65536 roots (later roots have small
     // trees, and fewer batches mean fewer appends to the store).
@@ -778,6 +788,11 @@ struct Builder {
       }
       const uint32_t R = (uint32_t)std::min<int64_t>(bw, n - r0);
       const uint32_t prefix = (uint32_t)r0;
+      if (auto_wide) {
+        tpr = (last_tree_mean < 0 || last_tree_mean > kWideTree) ? 1024u : 256u;
+        active = slots;
+        if (stats) stats->granularity_used = granularity_of(tpr);
+      }
       const double t_batch = lt + stt + ct;
       // hybrid under a policy: the policy picks each batch's tree kind until it
       // makes the pruned kind final (a pruned batch is exact as any other)
@@ -880,6 +895,11 @@ struct Builder {
                 (unsigned long long)L.total[0], L.nseg(), L.bytes() / 1e9);
       }
       ++stats_batches;
+      {
+        double sum = 0;
+        for (uint32_t k = 0; k < R; ++k) sum += hts[k];
+        last_tree_mean = R ? sum / R : 0.0;
+      }
       {
         // labels per root (candidates for pruned trees) of the batch's highest
         // eighth of ranks, forward side

@@ -319,9 +319,9 @@ void order_betweenness(const Graph& g, int64_t roots, uint64_t seed, int threads
             }
           }
           sub[p] += sub[v];
-          part[t][inv[v]] += sub[v] - 1;
+          part[t][v] += sub[v] - 1;
         }
-        part[t][inv[src]] += sub[src] - 1;
+        part[t][src] += sub[src] - 1;
         for (uint32_t v : settled) {
           dist[v] = kInf;
           sub[v] = 0;
@@ -329,11 +329,13 @@ void order_betweenness(const Graph& g, int64_t roots, uint64_t seed, int threads
       }
     });
   for (auto& th : ts) th.join();
+  // per-thread scores are kept by new id (written in search order, so the
+  // accumulation stays local); summed and mapped back to vertex ids here
   std::vector<uint64_t> score(n, 0);
-  parallel_for(n, host_threads(), 1 << 16, [&](int64_t v) {
+  parallel_for(n, host_threads(), 1 << 16, [&](int64_t i) {
     uint64_t s = 0;
-    for (int t = 0; t < T; ++t) s += part[t][v];
-    score[v] = s;
+    for (int t = 0; t < T; ++t) s += part[t][i];
+    score[inv[i]] = s;
   });
   order.resize(n);
   for (int64_t v = 0; v < n; ++v) order[v] = (uint32_t)v;

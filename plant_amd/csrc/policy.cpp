@@ -59,8 +59,10 @@ struct Learned {
   double mlp_ms = 0;
   std::mt19937_64 rng{12345};
   const BuildHooks* hooks = nullptr;  // the builder writes its slot count there
-  // labels one batch may write: an eighth of the free device memory at 32 bytes
-  // per label (output key and entry plus their sorted copies), at most 2^31 - 1
+  // labels one batch may write: the free device memory at 32 bytes per label
+  // (the tree output's 12, the batch segment's 5 to 8, cleaning flags and the
+  // compacted copy), at most 2^31 - 1; next_width keeps a 2x margin on the
+  // predicted total
   double label_budget = 268435456.0;
 
   // Hybrid switch point. PLaNT work per root grows with its tree/label ratio,
@@ -255,7 +257,7 @@ struct Learned {
   }
 
   void update_budget(size_t freeb) {
-    label_budget = std::min<double>(2147483647.0, std::max<double>(1 << 22, (double)freeb / 8.0 / 32.0));
+    label_budget = std::min<double>(2147483647.0, std::max<double>(1 << 22, (double)freeb / 32.0));
   }
 
   uint32_t next_width(int64_t r0, uint32_t static_width, uint32_t max_width) {
