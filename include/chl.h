@@ -97,8 +97,7 @@ int chl_order_betweenness_scored(const chl_graph* g, int64_t roots, uint64_t see
 /* The sampled-betweenness order computed on a gfx950 device: the same order
  * and scores as chl_order_betweenness_scored (score may be NULL). device_ms, if
  * not NULL, receives the device time (trees, scores and sort). No counterpart in
- * the reference, which reads the order from a file (shared_memory/hhl.cpp:59-62).
- * Currently switched off: returns CHL_E_STATE. */
+ * the reference, which reads the order from a file (shared_memory/hhl.cpp:59-62). */
 int chl_order_betweenness_device(const chl_graph* g, int64_t roots, uint64_t seed, int device, uint32_t* order,
                                  float* score, double* device_ms);
 /* Degree order, distributed_memory/degree.cpp:62-67 when reference_ties=1. */

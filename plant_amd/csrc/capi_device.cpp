@@ -375,17 +375,9 @@ int chl_labels_checksum(const chl_labels* l, int side, uint64_t* sum) {
 
 void chl_labels_free(chl_labels* l) { delete l; }
 
-// The device order (order.hip) stays off until the device fault seen on its
-// first run has a cause; its kernel passes a CPU emulation of one workgroup.
-static constexpr bool kDeviceOrderEnabled = false;
-
 int chl_order_betweenness_device(const chl_graph* g, int64_t roots, uint64_t seed, int device, uint32_t* order,
                                  float* score, double* device_ms) {
   if (!g || !order || roots <= 0) return set_error(CHL_E_INVAL, "bad arguments");
-  if (!kDeviceOrderEnabled)
-    return set_error(CHL_E_STATE,
-                     "the device betweenness order is switched off: its first run on the GPU ended with a device "
-                     "fault whose cause is not yet found; use chl_order_betweenness[_scored]");
   chl_graph* gm = const_cast<chl_graph*>(g);  // the device copy is a cache inside the handle
   if (!gm->dev) gm->dev.reset(new DeviceGraph());
   std::vector<uint32_t> o;

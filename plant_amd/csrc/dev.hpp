@@ -153,6 +153,9 @@ __device__ __forceinline__ void seg_entry(const SegView& s, uint64_t k, uint32_t
 enum : uint32_t {
   TREE_FLAGS = 1u,  // propagate the most important rank on the path (PLaNT, distributed_memory/akiba.hpp:71,89)
   TREE_PRUNE = 2u,  // prune by the completed rank-prefix labels (shared_memory/akiba.hpp:94-110)
+  TREE_KEEP = 4u,   // alone: a plain search per workgroup (root = workgroup index, R <= slots) whose
+                    // final distances stay in the slot's cells and whose reached vertices stay in
+                    // its vis list (touched[b] of them); no labels, no reset
 };
 
 // Packed per-vertex search state: (dist << 32) | low, where low is the smallest
@@ -229,29 +232,28 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s)
 // dst[idx[j]] = src[j] for j in [0, m)
 hipError_t launch_scatter_u32(const uint32_t* src, const uint32_t* idx, uint32_t m, uint32_t* dst, hipStream_t s);
 
-// Sampled-betweenness order (order.hip). Graph in device ids; per-slot scratch
-// sized n: cell 2 n words (distance, stamp/parent; unreached = (INF, 0)), acc n
-// words (children << 32 | subtree; rest state 1), three u32 lists.
-enum { BC_NEXT = 0, BC_REACHED = 1, BC_ARCS = 2, BC_ROUNDS = 3, BC_COUNT = 4 };
-struct BcParams {
-  const uint32_t* off;   // out-arcs, arc = (head << 32) | length
-  const uint64_t* arc;
-  const uint32_t* ioff;  // in-arcs, arc = (tail << 32) | length
-  const uint64_t* iarc;
-  const uint32_t* inv;   // device id -> vertex id (the parent tie rule)
+// Sampled-betweenness order (order.hip) over the trees the tree kernel left in
+// S slots (TREE_KEEP): slot s holds the search of roots[s]. Per slot sized n:
+// cell 2 n words (the tree kernel's), vis (reached vertices, touched[s] of
+// them), parent, acc ((children << 32) | subtree), leaf (nleaf[s] of them,
+// zeroed first). score: per device id, summed over the roots.
+struct BcTrees {
+  const uint64_t* cell;
+  const uint32_t* vis;
+  const uint32_t* touched;
   const uint32_t* roots;  // device ids
-  uint32_t R;
-  uint32_t n;
-  uint32_t delta;
-  uint32_t* cell;
+  uint32_t* parent;
   unsigned long long* acc;
-  uint32_t* qa;
-  uint32_t* qb;
-  uint32_t* vis;
-  unsigned long long* score;  // per device id, summed over roots
-  unsigned long long* ctr;    // BC_*
+  uint32_t* leaf;
+  uint32_t* nleaf;
+  const uint32_t* ioff;   // in-arcs, arc = (tail << 32) | length
+  const uint64_t* iarc;
+  const uint32_t* inv;    // device id -> vertex id (the parent tie rule)
+  unsigned long long* score;
+  uint32_t n, S;
 };
-hipError_t launch_betweenness(const BcParams& p, uint32_t slots, hipStream_t s);
+// The four passes over the S trees; max_touched bounds touched[] (grid width).
+hipError_t launch_bc_trees(const BcTrees& t, unsigned max_touched, hipStream_t s);
 hipError_t launch_score_keys(const unsigned long long* score, const uint32_t* inv, uint32_t n, uint64_t* key,
                              uint64_t* val, hipStream_t s);
 // Degree order keys over vertex ids in [0, n); off1 / off0 are the out / in arc

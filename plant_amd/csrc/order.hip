@@ -1,197 +1,110 @@
-// Sampled-betweenness vertex order on the device. The reference takes the order
-// as an input file (shared_memory/hhl.cpp:59-62, made by an external tool,
-// README.md:24); this computes the same order as the host restatement
-// (gen.cpp order_betweenness): from each of R seeded roots a shortest-path tree
-// whose parent of v is the smallest-id u with dist(u) + len(u, v) == dist(v);
-// a vertex scores its proper descendants summed over the trees; the order is
-// score descending, then vertex id ascending.
+// Vertex orders on the device.
 //
-// One workgroup owns one root at a time (a device cursor hands out roots) and a
-// slot of scratch sized n. Per root:
-//  1. distances by bucketed label-correcting Dijkstra (delta stepping), a
-//     32-bit atomicMin per relaxation and a round stamp that queues a vertex
-//     once per round;
-//  2. every reached vertex v != root scans its in-arcs for the parent and adds
-//     one child to the parent's accumulator word (children << 32 | subtree);
-//  3. leaves walk up: each child adds (subtree - 2^32) to its parent's word in
-//     one 64-bit atomic, so the child that brings the children count to zero
-//     learns the parent's final subtree size from the returned word and
-//     continues upward. Every non-leaf is finished by exactly one child, which
-//     adds subtree - 1 to the vertex's score. No fence orders the counts against
-//     the sizes: they are the same word.
-//  4. the touched cells are restored to their unreached state.
+// Sampled betweenness. The reference takes the order as an input file
+// (shared_memory/hhl.cpp:59-62, made by an external tool, README.md:24); this
+// computes the same order as the host restatement (gen.cpp order_betweenness):
+// from each of R seeded roots a shortest-path tree whose parent of v is the
+// smallest-id u with dist(u) + len(u, v) == dist(v); a vertex scores its proper
+// descendants summed over the trees; the order is score descending, then vertex
+// id ascending. Per group of roots, one per search slot:
+//  1. the tree kernel (tree.hip, TREE_KEEP) runs each root's search in its own
+//     slot and leaves the final distances in the slot's cells and the reached
+//     vertices in its vis list;
+//  2. bc_init: every reached vertex's accumulator word (children << 32 |
+//     subtree) is set to (0, 1);
+//  3. bc_parent: every reached vertex other than the root scans its in-arcs for
+//     its parent and adds one child to the parent's word;
+//  4. bc_leaves: the reached vertices without children are listed (a pass of
+//     its own, so no vertex is taken for a leaf once its children have come);
+//  5. bc_walk: each leaf walks up: a child adds (subtree - 2^32) to its
+//     parent's word in one 64-bit atomic, so the child that brings the count to
+//     zero reads the parent's final subtree size from the returned word, adds
+//     subtree - 1 to the parent's score and continues upward; the others stop.
+//     Counts and sizes share one word, so no fence orders them.
+// Each kernel runs over (reached vertex, slot) on a two-dimensional grid.
+#include <algorithm>
+
 #include "dev.hpp"
 
 namespace chl {
 namespace {
 
-constexpr int kBcThreads = 256;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-// Distances read after other lanes' atomics: loads at device scope.
-__device__ __forceinline__ uint32_t load_dist(const uint32_t* cell, uint32_t v) {
-  return __hip_atomic_load(cell + 2 * (size_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ uint32_t tree_dist(const uint64_t* cell, uint32_t v) {
+  return (uint32_t)(cell[2 * (size_t)v] >> 32);
 }
 
-__global__ __launch_bounds__(kBcThreads) void betweenness_kernel(BcParams p) {
-  __shared__ uint32_t s_ri, s_fsize, s_nsize, s_nvis, s_below, s_minnext;
-  __shared__ unsigned long long s_arcs;
+__global__ void bc_init_kernel(BcTrees t) {
+  const uint32_t s = blockIdx.y;
+  const size_t n = t.n;
+  const uint32_t cnt = t.touched[s];
+  const uint32_t* vis = t.vis + s * n;
+  unsigned long long* acc = t.acc + s * n;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) acc[vis[k]] = 1;
+}
 
-  const uint32_t tid = threadIdx.x;
-  const uint32_t lane = tid & 63;
-  const size_t n = p.n;
-  const size_t slot = blockIdx.x;
-  uint32_t* cell = p.cell + slot * 2 * n;  // [2 v] distance, [2 v + 1] round stamp, then the parent
-  unsigned long long* acc = p.acc + slot * n;
-  uint32_t* vis = p.vis + slot * n;
+__global__ void bc_parent_kernel(BcTrees t) {
+  const uint32_t s = blockIdx.y;
+  const size_t n = t.n;
+  const uint32_t cnt = t.touched[s], root = t.roots[s];
+  const uint32_t* vis = t.vis + s * n;
+  const uint64_t* cell = t.cell + 2 * s * n;
+  uint32_t* parent = t.parent + s * n;
+  unsigned long long* acc = t.acc + s * n;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+    const uint32_t v = vis[k];
+    if (v == root) {
+      parent[v] = kNone;
+      continue;
+    }
+    const uint32_t dv = tree_dist(cell, v);
+    uint32_t best = kNone, bp = kNone;
+    for (uint32_t a = t.ioff[v]; a < t.ioff[v + 1]; ++a) {
+      const uint64_t arc = t.iarc[a];
+      const uint32_t u = (uint32_t)(arc >> 32);
+      const uint32_t du = tree_dist(cell, u);
+      if (du != kNone && du + (uint32_t)arc == dv && t.inv[u] < best) {
+        best = t.inv[u];
+        bp = u;
+      }
+    }
+    parent[v] = bp;
+    if (bp != kNone) atomicAdd(&acc[bp], 1ull << 32);
+  }
+}
 
-  for (;;) {
-    if (tid == 0) {
-      const unsigned long long r = atomicAdd(&p.ctr[BC_NEXT], 1ull);
-      s_ri = r < p.R ? (uint32_t)r : kNone;
-      s_arcs = 0;
-    }
-    __syncthreads();
-    const uint32_t ri = s_ri;
-    if (ri == kNone) break;
-    const uint32_t root = p.roots[ri];
-    uint32_t* F = p.qa + slot * n;
-    uint32_t* N = p.qb + slot * n;
-    if (tid == 0) {
-      cell[2 * (size_t)root] = 0;
-      cell[2 * (size_t)root + 1] = 1;
-      F[0] = root;
-      vis[0] = root;
-      s_fsize = 1;
-      s_nvis = 1;
-      s_below = 1;
-      s_minnext = 0;
-    }
-    uint32_t B = p.delta;
-    uint32_t round = 1;
-    unsigned long long arcs = 0, rounds = 0;
-    __syncthreads();
+__global__ void bc_leaves_kernel(BcTrees t) {
+  const uint32_t s = blockIdx.y;
+  const size_t n = t.n;
+  const uint32_t cnt = t.touched[s];
+  const uint32_t* vis = t.vis + s * n;
+  const unsigned long long* acc = t.acc + s * n;
+  uint32_t* leaf = t.leaf + s * n;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+    const uint32_t v = vis[k];
+    if ((acc[v] >> 32) == 0) leaf[atomicAdd(&t.nleaf[s], 1u)] = v;
+  }
+}
 
-    // 1. distances
-    for (;;) {
-      const uint32_t fsize = s_fsize;
-      if (fsize == 0) break;
-      if (s_below == 0) B = s_minnext + p.delta;  // every queued distance is >= B
-      __syncthreads();
-      if (tid == 0) {
-        s_nsize = 0;
-        s_below = 0;
-        s_minnext = kNone;
-      }
-      __syncthreads();
-      const uint32_t nstamp = round + 1;
-      uint32_t below = 0, mn = kNone;
-      for (uint32_t k = tid; k < fsize; k += kBcThreads) {
-        const uint32_t u = F[k];
-        const uint32_t d = load_dist(cell, u);
-        if (d >= B) {
-          mn = min(mn, d);
-          if (atomicExch(&cell[2 * (size_t)u + 1], nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = u;
-          continue;
-        }
-        const uint32_t ab = p.off[u], ae = p.off[u + 1];
-        arcs += ae - ab;
-        for (uint32_t a = ab; a < ae; ++a) {
-          const uint64_t arc = p.arc[a];
-          const uint32_t v = (uint32_t)(arc >> 32);
-          const uint32_t nd = d + (uint32_t)arc;
-          uint32_t* dv = &cell[2 * (size_t)v];
-          if (nd < *dv) {
-            const uint32_t old = atomicMin(dv, nd);
-            if (nd < old) {
-              below |= nd < B;
-              mn = min(mn, nd);
-              if (old == kNone) vis[atomicAdd(&s_nvis, 1u)] = v;
-              if (atomicExch(&cell[2 * (size_t)v + 1], nstamp) != nstamp) N[atomicAdd(&s_nsize, 1u)] = v;
-            }
-          }
-        }
-      }
-      // the next frontier's bucket bookkeeping: every state change of the round
-      // and every re-queued entry was noted, so both figures are exact
-      const bool any_below = __any(below != 0);
-      for (int off = 32; off > 0; off >>= 1) mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
-      if (lane == 0) {
-        if (any_below) s_below = 1;
-        if (mn != kNone) atomicMin(&s_minnext, mn);
-      }
-      __syncthreads();
-      if (tid == 0) s_fsize = s_nsize;
-      uint32_t* t = F;
-      F = N;
-      N = t;
-      ++round;
-      ++rounds;
-      __syncthreads();
-    }
-
-    // 2. parents (smallest vertex id among the shortest-path predecessors) and
-    // children counts
-    const uint32_t nvis = s_nvis;
-    for (uint32_t k = tid; k < nvis; k += kBcThreads) {
-      const uint32_t v = vis[k];
-      if (v == root) continue;
-      const uint32_t dv = load_dist(cell, v);
-      uint32_t best = kNone, bp = kNone;
-      const uint32_t ab = p.ioff[v], ae = p.ioff[v + 1];
-      for (uint32_t a = ab; a < ae; ++a) {
-        const uint64_t arc = p.iarc[a];
-        const uint32_t u = (uint32_t)(arc >> 32);
-        const uint32_t du = load_dist(cell, u);
-        if (du != kNone && du + (uint32_t)arc == dv) {
-          const uint32_t iu = p.inv[u];
-          if (iu < best) {
-            best = iu;
-            bp = u;
-          }
-        }
-      }
-      cell[2 * (size_t)v + 1] = bp;
-      if (bp != kNone) atomicAdd(&acc[bp], 1ull << 32);
-    }
-    __syncthreads();
-    // leaves (s_fsize is 0 since the search ended)
-    for (uint32_t k = tid; k < nvis; k += kBcThreads) {
-      const uint32_t v = vis[k];
-      if ((__hip_atomic_load(&acc[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32) == 0)
-        F[atomicAdd(&s_fsize, 1u)] = v;
-    }
-    __syncthreads();
-    // 3. subtree sizes, leaves upward
-    const uint32_t nleaf = s_fsize;
-    for (uint32_t k = tid; k < nleaf; k += kBcThreads) {
-      uint32_t v = F[k];
-      uint32_t s = 1;
-      while (v != root) {
-        const uint32_t pv = __hip_atomic_load(&cell[2 * (size_t)v + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (pv == kNone) break;
-        const unsigned long long old = atomicAdd(&acc[pv], (unsigned long long)s - (1ull << 32));
-        if ((old >> 32) != 1) break;  // a sibling is still to come
-        s += (uint32_t)old;
-        atomicAdd(&p.score[pv], (unsigned long long)(s - 1));
-        v = pv;
-      }
-    }
-    __syncthreads();
-    // 4. restore the slot
-    for (uint32_t k = tid; k < nvis; k += kBcThreads) {
-      const uint32_t v = vis[k];
-      cell[2 * (size_t)v] = kNone;
-      cell[2 * (size_t)v + 1] = 0;
-      acc[v] = 1;
-    }
-    for (int off = 32; off > 0; off >>= 1) arcs += (unsigned long long)__shfl_xor((long long)arcs, off);
-    if (lane == 0) atomicAdd(&s_arcs, arcs);
-    __syncthreads();
-    if (tid == 0) {
-      atomicAdd(&p.ctr[BC_REACHED], (unsigned long long)nvis);
-      atomicAdd(&p.ctr[BC_ARCS], s_arcs);
-      atomicAdd(&p.ctr[BC_ROUNDS], rounds);
+__global__ void bc_walk_kernel(BcTrees t) {
+  const uint32_t s = blockIdx.y;
+  const size_t n = t.n;
+  const uint32_t cnt = t.nleaf[s], root = t.roots[s];
+  const uint32_t* leaf = t.leaf + s * n;
+  const uint32_t* parent = t.parent + s * n;
+  unsigned long long* acc = t.acc + s * n;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+    uint32_t v = leaf[k];
+    uint32_t size = 1;
+    while (v != root) {
+      const uint32_t pv = parent[v];
+      if (pv == kNone) break;
+      const unsigned long long old = atomicAdd(&acc[pv], (unsigned long long)size - (1ull << 32));
+      if ((old >> 32) != 1) break;  // a sibling is still to come
+      size += (uint32_t)old;
+      atomicAdd(&t.score[pv], (unsigned long long)(size - 1));
+      v = pv;
     }
   }
 }
@@ -248,9 +161,13 @@ unsigned grid_for(size_t count) {
 
 }  // namespace
 
-hipError_t launch_betweenness(const BcParams& p, uint32_t slots, hipStream_t s) {
-  if (!p.R || !slots) return hipSuccess;
-  hipLaunchKernelGGL(betweenness_kernel, dim3(slots), dim3(kBcThreads), 0, s, p);
+hipError_t launch_bc_trees(const BcTrees& t, unsigned max_touched, hipStream_t s) {
+  if (!t.S) return hipSuccess;
+  const dim3 grid(std::min<unsigned>(1024, std::max<unsigned>(1, (max_touched + 255) / 256)), t.S), block(256);
+  hipLaunchKernelGGL(bc_init_kernel, grid, block, 0, s, t);
+  hipLaunchKernelGGL(bc_parent_kernel, grid, block, 0, s, t);
+  hipLaunchKernelGGL(bc_leaves_kernel, grid, block, 0, s, t);
+  hipLaunchKernelGGL(bc_walk_kernel, grid, block, 0, s, t);
   return hipGetLastError();
 }
 

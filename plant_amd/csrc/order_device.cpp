@@ -59,63 +59,97 @@ int order_impl(const Graph& g, DeviceGraph& dg, int device, int64_t roots, uint6
   CHL_HIP(hipEventCreate(&e0));
   CHL_HIP(hipEventCreate(&e1));
 
-  // slots: 28 bytes of scratch per vertex each, within half the free memory
+  // slots: 48 bytes of scratch per vertex each (the tree kernel's 16-byte cell,
+  // its three lists and pile; parent, accumulator and leaf list), within a
+  // third of the free memory
   size_t freeb = 0, totalb = 0;
   CHL_HIP(hipMemGetInfo(&freeb, &totalb));
-  const size_t per = (size_t)n * 28;
-  const size_t fit = std::max<size_t>(1, (freeb / 2) / per);
-  const uint32_t slots = (uint32_t)std::max<size_t>(1, std::min<size_t>({fit, (size_t)R, (size_t)1024}));
+  const size_t per = (size_t)n * 48;
+  const size_t fit = std::max<size_t>(1, (freeb / 3) / per);
+  const uint32_t slots = (uint32_t)std::max<size_t>(1, std::min<size_t>({fit, (size_t)R, (size_t)256}));
   const size_t cells = (size_t)slots * n;
-  *scratch = cells * 28;
-  DBuf<uint64_t> cell, acc, score_d, key, val, skey, sval;
-  DBuf<uint32_t> qa, qb, vis, inv, droots;
+  *scratch = cells * 48;
+  DBuf<uint64_t> cell, qd, acc, score_d, key, val, skey, sval;
+  DBuf<uint32_t> qa, qb, vis, parent, leaf, zero, inv, droots, touched, nleaf, tsize, lcount, spill;
   DBuf<unsigned long long> ctr;
   DBuf<uint8_t> tmp;
   StreamDrain drain{stream};
-  CHL_HIP(cell.reserve(cells));  // u32 pairs (distance, stamp)
+  CHL_HIP(cell.reserve(2 * cells));
+  CHL_HIP(qd.reserve(cells));
   CHL_HIP(acc.reserve(cells));
   CHL_HIP(qa.reserve(cells));
   CHL_HIP(qb.reserve(cells));
   CHL_HIP(vis.reserve(cells));
+  CHL_HIP(parent.reserve(cells));
+  CHL_HIP(leaf.reserve(cells));
+  CHL_HIP(zero.reserve(n));
   CHL_HIP(score_d.reserve(n));
   CHL_HIP(inv.reserve(n));
   CHL_HIP(droots.reserve(std::max<uint32_t>(R, 1)));
-  CHL_HIP(ctr.reserve(BC_COUNT));
+  for (auto* b : {&touched, &nleaf, &tsize, &lcount, &spill}) CHL_HIP(b->reserve(slots));
+  CHL_HIP(ctr.reserve(CTR_COUNT));
   CHL_HIP(hipMemcpyAsync(inv.p, dg.inv.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream));
   if (R) CHL_HIP(hipMemcpyAsync(droots.p, rs.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, stream));
-  CHL_HIP(launch_fill_u64(cell.p, 0x00000000FFFFFFFFull, cells, stream));  // distance INF, stamp 0
-  CHL_HIP(launch_fill_u64(acc.p, 1, cells, stream));
+  // every vertex has rank 0 here: the searches expand every vertex they reach
+  CHL_HIP(launch_fill_u32(zero.p, 0, n, stream));
+  CHL_HIP(launch_init_cells(cell.p, cells, zero.p, (uint32_t)n, stream));
   CHL_HIP(launch_fill_u64(score_d.p, 0, n, stream));
-  CHL_HIP(launch_fill_u64((uint64_t*)ctr.p, 0, BC_COUNT, stream));
 
-  BcParams p;
-  memset(&p, 0, sizeof(p));
   const int ir = dg.sym ? 1 : 0;
+  TreeParams p;
+  memset(&p, 0, sizeof(p));
   p.off = dg.off[1].p;
   p.arc = dg.arc[1].p;
-  p.ioff = dg.off[ir].p;
-  p.iarc = dg.arc[ir].p;
-  p.inv = inv.p;
-  p.roots = droots.p;
-  p.R = R;
+  p.rank = zero.p;
   p.n = (uint32_t)n;
   p.delta = delta;
-  p.cell = reinterpret_cast<uint32_t*>(cell.p);
-  p.acc = reinterpret_cast<unsigned long long*>(acc.p);
+  p.cell = cell.p;
   p.qa = qa.p;
   p.qb = qb.p;
   p.vis = vis.p;
-  p.score = reinterpret_cast<unsigned long long*>(score_d.p);
+  p.qd = qd.p;
+  p.spill = spill.p;
+  p.tree_size = tsize.p;
+  p.label_count = lcount.p;
+  p.touched = touched.p;
   p.ctr = ctr.p;
+  BcTrees t;
+  memset(&t, 0, sizeof(t));
+  t.cell = cell.p;
+  t.vis = vis.p;
+  t.touched = touched.p;
+  t.parent = parent.p;
+  t.acc = reinterpret_cast<unsigned long long*>(acc.p);
+  t.leaf = leaf.p;
+  t.nleaf = nleaf.p;
+  t.ioff = dg.off[ir].p;
+  t.iarc = dg.arc[ir].p;
+  t.inv = inv.p;
+  t.score = reinterpret_cast<unsigned long long*>(score_d.p);
+  t.n = (uint32_t)n;
+  // large graphs: a workgroup of 1024 threads per search
+  const int tpr = n >= (int64_t)1 << 20 ? 1024 : 256;
   CHL_HIP(hipEventRecord(e0, stream));
-  CHL_HIP(launch_betweenness(p, slots, stream));
+  for (uint32_t r0 = 0; r0 < R; r0 += slots) {
+    const uint32_t S = std::min<uint32_t>(slots, R - r0);
+    p.roots = droots.p + r0;
+    p.R = S;
+    t.roots = droots.p + r0;
+    t.S = S;
+    CHL_HIP(launch_fill_u64((uint64_t*)ctr.p, 0, CTR_COUNT, stream));
+    CHL_HIP(launch_fill_u32(nleaf.p, 0, S, stream));
+    CHL_HIP(launch_tree(p, TREE_KEEP, S, tpr, stream));
+    CHL_HIP(launch_bc_trees(t, (unsigned)n, stream));
+    // the slots back to their unreached state for the next group
+    CHL_HIP(launch_init_cells(cell.p, (size_t)S * n, zero.p, (uint32_t)n, stream));
+  }
 
   // order: stable ascending sort of ~score over ids in ascending order
   CHL_HIP(key.reserve(n));
   CHL_HIP(val.reserve(n));
   CHL_HIP(skey.reserve(n));
   CHL_HIP(sval.reserve(n));
-  CHL_HIP(launch_score_keys(p.score, inv.p, (uint32_t)n, key.p, val.p, stream));
+  CHL_HIP(launch_score_keys(t.score, inv.p, (uint32_t)n, key.p, val.p, stream));
   size_t tb = 0;
   CHL_HIP(sort_pairs_u64(nullptr, tb, key.p, skey.p, val.p, sval.p, (uint64_t)n, 64, stream));
   CHL_HIP(tmp.reserve(tb));

@@ -141,7 +141,12 @@ constexpr uint32_t kArcStep = 4;
 // root (more roots in flight, no cross-wave barrier cost for small trees); 1024:
 // sixteen wavefronts per root (a quarter of the trees in flight, so a quarter of
 // the per-tree state competes for each XCD's L2, for wide search fronts).
-template <bool FLAGS, bool PRUNE, int THREADS>
+//
+// KEEP (no FLAGS, no PRUNE): workgroup b searches root b only and leaves the
+// search in its slot (cells at their final distances, vis[0, touched[b]) the
+// reached vertices) for kernels that read the tree afterwards; nothing is
+// emitted and nothing is reset.
+template <bool FLAGS, bool PRUNE, int THREADS, bool KEEP = false>
 __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
   constexpr int kThreads = THREADS;
   constexpr int kWaves = THREADS / 64;
@@ -168,7 +173,8 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
   if (tid < 6) s_ctr[tid] = 0;
 
   for (;;) {
-    if (tid == 0) {
+    if (tid == 0 && KEEP) s_ri = blockIdx.x < p.R ? (uint32_t)blockIdx.x : kEmpty;
+    if (tid == 0 && !KEEP) {
       unsigned long long r = atomicAdd(&p.ctr[CTR_NEXT], 1ull);
       // once a root's labels did not fit, later roots are not started: they go
       // back to the host with the spilled ones (no tree is computed in vain)
@@ -424,6 +430,11 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
       __syncthreads();
     }
 
+    if (KEEP) {
+      if (tid == 0) p.touched[ri] = s_nvis;
+      break;
+    }
+
     // Emit (root rank, d) at every touched vertex whose final low is the root's
     // rank. One pass reads each touched cell once, keeps the label's (rank, d) in
     // the two frontier lists (free once the search has ended) and restores the
@@ -542,10 +553,11 @@ static unsigned fill_blocks(size_t count) {
 template <int T>
 static void launch_tree_t(const TreeParams& p, uint32_t flags, uint32_t slots, hipStream_t s) {
   dim3 grid(slots), block(T);
-  switch (flags & (TREE_FLAGS | TREE_PRUNE)) {
+  switch (flags & (TREE_FLAGS | TREE_PRUNE | TREE_KEEP)) {
     case TREE_FLAGS: hipLaunchKernelGGL((tree_kernel<true, false, T>), grid, block, 0, s, p); break;
     case TREE_FLAGS | TREE_PRUNE: hipLaunchKernelGGL((tree_kernel<true, true, T>), grid, block, 0, s, p); break;
     case TREE_PRUNE: hipLaunchKernelGGL((tree_kernel<false, true, T>), grid, block, 0, s, p); break;
+    case TREE_KEEP: hipLaunchKernelGGL((tree_kernel<false, false, T, true>), grid, block, 0, s, p); break;
     default: hipLaunchKernelGGL((tree_kernel<false, false, T>), grid, block, 0, s, p); break;
   }
 }

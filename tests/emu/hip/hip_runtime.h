@@ -167,12 +167,15 @@ inline void emu_lane_entry() {
 template <class... A, class... B>
 void hipLaunchKernelGGL(void (*k)(A...), dim3 g, dim3 b, int, hipStream_t, B... args) {
   gridDim.x = g.x;
+  gridDim.y = g.y;
   blockDim.x = b.x;
   std::mt19937_64 rng(emu_seed);
   emu_yield_state = emu_seed * 0x9E3779B97F4A7C15ull | 1;
-  for (unsigned bi = 0; bi < g.x; ++bi) {
+  for (unsigned bj = 0; bj < g.x * g.y; ++bj) {
+    const unsigned bi = bj % g.x;
     if (emu_only_block >= 0 && (int)bi != emu_only_block) continue;
     blockIdx.x = bi;
+    blockIdx.y = bj / g.x;
     EmuBlock blk;
     emu_blk = &blk;
     const unsigned T = b.x, W = (T + 63) / 64;

@@ -24,9 +24,6 @@ DEGREE_TOOL = os.path.join(os.path.dirname(GOLD), "..", "plant_amd", "bin", "deg
 with open(os.path.join(GOLD, "manifest.json")) as _f:
     MANIFEST = json.load(_f)
 
-skip_bc = pytest.mark.skip(reason="device betweenness order switched off: its first GPU run ended with a device "
-                           "fault whose cause is not found (chl_order_betweenness_device returns CHL_E_STATE)")
-
 M64 = (1 << 64) - 1
 
 
@@ -113,28 +110,24 @@ def check_same(g, roots, seed):
     return ms
 
 
-@skip_bc
 def test_grid_c1_order(chl):
     # the C1 order: 100x100 grid, 64 roots
     g = chl.Graph.grid(100, 100, 1, 100, seed=1)
     check_same(g, 64, 1)
 
 
-@skip_bc
 @pytest.mark.parametrize("roots", [1, 17, 300])
 def test_knn(chl, roots):
     g = chl.Graph.knn(20000, 3, seed=4)
     check_same(g, roots, 9)
 
 
-@skip_bc
 def test_rmat_unit_weights_ties(chl):
     # unit weights: many shortest-path predecessors, so the id tie rule decides
     g = chl.Graph.rmat(13, 3 * 8192, seed=3)
     check_same(g, 128, 2)
 
 
-@skip_bc
 def test_directed_and_disconnected(chl):
     rng = np.random.default_rng(8)
     n, m = 700, 2500
@@ -154,7 +147,6 @@ def test_directed_and_disconnected(chl):
     assert list(o) == [0] and list(s) == [0]
 
 
-@skip_bc
 def test_c2_road_1M_order(chl):
     # the C2 order at its own size: 1M-point 3-NN graph, 256 roots
     g = chl.Graph.knn(1_000_000, 3, seed=1)

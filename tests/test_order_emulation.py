@@ -1,11 +1,12 @@
-"""The device sampled-betweenness kernel's own source (plant_amd/csrc/order.hip),
-compiled for the host with g++ and AddressSanitizer against
-tests/emu/hip/hip_runtime.h, run as the last of several slots (so every per-slot
-offset is the largest the launch uses) over scratch arrays sized exactly as
-plant_amd/csrc/order_device.cpp sizes them. With every vertex as a root the
+"""The device sampled-betweenness pipeline's own source (the tree kernel of
+plant_amd/csrc/tree.hip in TREE_KEEP mode and the four passes of
+plant_amd/csrc/order.hip), compiled for the host with g++ and AddressSanitizer
+against tests/emu/hip/hip_runtime.h, every workgroup of every launch run, over
+scratch arrays sized exactly as plant_amd/csrc/order_device.cpp sizes them,
+with several groups of roots per slot set. With every vertex as a root the
 summed scores do not depend on the sampled root set, so they must equal the host
 order's scores (chl_order_betweenness_scored, itself pinned against scipy in
-tests/test_host.py), and every scratch word must come back to its rest state.
+tests/test_host.py), and every search cell must come back to its rest state.
 
 Not a model of the GPU memory system: it checks indexing, bounds and the
 barrier and counting logic without a GPU."""
@@ -32,7 +33,7 @@ def emulator(tmp_path_factory):
     return exe
 
 
-def run_order_emulator(exe, tmp_path, g, slots, delta, seed=7):
+def run_order_emulator(exe, tmp_path, g, slots, delta, seed=7, threads=64):
     n = g.n
     off, head, ln = g.csr(1)
     ioff, ihead, iln = g.csr(0)
@@ -41,7 +42,7 @@ def run_order_emulator(exe, tmp_path, g, slots, delta, seed=7):
     roots = np.random.default_rng(seed).permutation(n).astype(np.uint32)  # every vertex
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(inp, "wb") as f:
-        f.write(np.array([n, arc.size, iarc.size, n, delta, slots], np.uint32).tobytes())
+        f.write(np.array([n, arc.size, iarc.size, n, delta, slots, threads], np.uint32).tobytes())
         f.write(off.astype(np.uint32).tobytes())
         f.write(arc.tobytes())
         f.write(ioff.astype(np.uint32).tobytes())
@@ -69,11 +70,11 @@ def cases(chl):
     yield "rmat_unit", chl.Graph.rmat(6, 3 * 64, seed=3)
 
 
-@pytest.mark.parametrize("slots", [1, 5])
-def test_betweenness_kernel_emulated_equals_host_scores(chl, emulator, tmp_path, slots):
+@pytest.mark.parametrize("slots,threads", [(7, 64), (5, 256)])
+def test_betweenness_pipeline_emulated_equals_host_scores(chl, emulator, tmp_path, slots, threads):
     for name, g in cases(chl):
         _, want = g.order_betweenness(g.n, seed=1, with_score=True)
         for delta in (2, 200):
-            score, restored = run_order_emulator(emulator, tmp_path, g, slots, delta)
+            score, restored = run_order_emulator(emulator, tmp_path, g, slots, delta, threads=threads)
             assert restored == 1, (name, slots, delta)
             assert np.array_equal(score.astype(np.float64), np.asarray(want, np.float64)), (name, slots, delta)
