@@ -214,6 +214,12 @@ int chl_labels_copy(const chl_labels* l, int side, uint64_t* off, uint32_t* hub,
 int chl_labels_checksum(const chl_labels* l, int side, uint64_t* sum);
 /* len[v] = entries in vertex v's row of one side (no entry is copied). */
 int chl_labels_row_lengths(const chl_labels* l, int side, uint32_t* len);
+/* The labels of one side whose hub lies in [hub_lo, hub_hi), rows by vertex
+ * id: off[n+1] always; hub[off[n]] and dist[off[n]] too when both are non-NULL
+ * (call first with NULL to learn off[n]). A window of a store too large to copy
+ * whole (no reference counterpart). */
+int chl_labels_copy_hubs(const chl_labels* l, int side, uint32_t hub_lo, uint32_t hub_hi, uint64_t* off,
+                         uint32_t* hub, int32_t* dist);
 /* The segments of one side (no reference counterpart: the store's layout). A
  * side is a list of segments with ascending, disjoint hub-rank ranges
  * [h0, h1); each holds its entries in 5 bytes (fmt 5: hub - h0 < 2^24, dist <=

@@ -124,6 +124,7 @@ def lib():
             "chl_labels_copy": ([vp, C.c_int, vp, vp, vp], C.c_int),
             "chl_labels_row_lengths": ([vp, C.c_int, vp], C.c_int),
             "chl_labels_segments": ([vp, C.c_int, C.c_int64, vp, vp, vp, vp], C.c_int),
+            "chl_labels_copy_hubs": ([vp, C.c_int, C.c_uint32, C.c_uint32, vp, vp, vp], C.c_int),
             "chl_labels_checksum": ([vp, C.c_int, C.POINTER(u64)], C.c_int),
             "chl_labels_free": ([vp], None),
             "chl_query": ([vp, vp, i64, vp], C.c_int),
@@ -386,6 +387,18 @@ class Labels:
         out = np.zeros(self.n, np.uint32)
         _check(lib().chl_labels_row_lengths(self._h, side, _ptr(out)))
         return out
+
+    def copy_hubs(self, side, hub_lo, hub_hi):
+        """(off, hub, dist) of the labels of one side with hub in [hub_lo, hub_hi),
+        rows by vertex id (a window of the store, copied from the device)."""
+        off = np.zeros(self.n + 1, np.uint64)
+        _check(lib().chl_labels_copy_hubs(self._h, side, hub_lo, hub_hi, _ptr(off), None, None))
+        total = int(off[-1])
+        hub = np.zeros(max(total, 1), np.uint32)
+        dist = np.zeros(max(total, 1), np.int32)
+        if total:
+            _check(lib().chl_labels_copy_hubs(self._h, side, hub_lo, hub_hi, _ptr(off), _ptr(hub), _ptr(dist)))
+        return off, hub[:total], dist[:total]
 
     def segments(self, side=0):
         """The store's segments of one side: list of (h0, h1, fmt, entries)."""

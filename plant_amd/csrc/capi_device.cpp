@@ -346,6 +346,37 @@ int chl_labels_row_lengths(const chl_labels* l, int side, uint32_t* len) {
   return CHL_OK;
 }
 
+int chl_labels_copy_hubs(const chl_labels* l, int side, uint32_t hub_lo, uint32_t hub_hi, uint64_t* off,
+                         uint32_t* hub, int32_t* dist) {
+  if (!l || side < 0 || side > 1 || !off || hub_hi < hub_lo) return set_error(CHL_E_INVAL, "bad arguments");
+  const int64_t n = l->store.n;
+  CHL_HIP(hipSetDevice(l->store.device));
+  DBuf<uint32_t> cnt;
+  DBuf<uint64_t> noff, tmp;
+  CHL_HIP(cnt.reserve(std::max<int64_t>(n, 1)));
+  CHL_HIP(noff.reserve(n + 1));
+  CHL_HIP(tmp.reserve(scan_tmp_words((uint64_t)n)));
+  const SegView* views = l->store.d_views(side);
+  const uint32_t k = l->store.nseg();
+  hipStream_t st = l->stream;
+  if (k == 0) CHL_HIP(launch_fill_u32(cnt.p, 0, (size_t)n, st));
+  CHL_HIP(launch_window(views, k, l->rank_of.p, (uint32_t)n, hub_lo, hub_hi, cnt.p, nullptr, nullptr, nullptr, st));
+  CHL_HIP(launch_scan_u32(cnt.p, (uint64_t)n, noff.p, tmp.p, st));
+  CHL_HIP(hipMemcpyAsync(off, noff.p, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, st));
+  CHL_HIP(hipStreamSynchronize(st));
+  const uint64_t total = off[n];
+  if (!hub || !dist || total == 0) return CHL_OK;  // sizes only
+  DBuf<uint32_t> dh;
+  DBuf<int32_t> dd;
+  CHL_HIP(dh.reserve(total));
+  CHL_HIP(dd.reserve(total));
+  CHL_HIP(launch_window(views, k, l->rank_of.p, (uint32_t)n, hub_lo, hub_hi, cnt.p, noff.p, dh.p, dd.p, st));
+  CHL_HIP(hipMemcpyAsync(hub, dh.p, sizeof(uint32_t) * total, hipMemcpyDeviceToHost, st));
+  CHL_HIP(hipMemcpyAsync(dist, dd.p, sizeof(int32_t) * total, hipMemcpyDeviceToHost, st));
+  CHL_HIP(hipStreamSynchronize(st));
+  return CHL_OK;
+}
+
 int chl_labels_segments(const chl_labels* l, int side, int64_t cap, uint32_t* h0, uint32_t* h1, uint32_t* fmt,
                         uint64_t* entries) {
   if (!l || side < 0 || side > 1 || cap < 0) return set_error(CHL_E_INVAL, "bad arguments");

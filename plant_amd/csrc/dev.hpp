@@ -350,6 +350,12 @@ hipError_t launch_checksum(const SegView& seg, const uint32_t* row_of, uint32_t 
 // row_of[v]) in segment s, for v in [0, n) and s < k (views: device array).
 hipError_t launch_row_ranges(const SegView* views, uint32_t k, const uint32_t* row_of, uint32_t n, uint64_t* rng,
                              hipStream_t s);
+// Labels with hub in [hlo, hhi) per vertex id (row row_of[v]) over k segments:
+// with noff null, cnt[v] = their count; with noff (the exclusive scan of cnt),
+// they are written to hub[] / dist[] from noff[v] on, hub-ascending.
+hipError_t launch_window(const SegView* views, uint32_t k, const uint32_t* row_of, uint32_t n, uint32_t hlo,
+                         uint32_t hhi, uint32_t* cnt, const uint64_t* noff, uint32_t* hub, int32_t* dist,
+                         hipStream_t s);
 // (hub << 32) | dist of every entry of a segment, in entry order.
 hipError_t launch_decode(const SegView& seg, uint64_t count, uint64_t* out, hipStream_t s);
 // What a query reads of a label handle, passed by value as a kernel argument

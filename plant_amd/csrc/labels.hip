@@ -489,6 +489,54 @@ __global__ void row_ranges_kernel(const SegView* views, uint32_t k, const uint32
   }
 }
 
+// Entries of a row segment with hub in [hlo, hhi): [lo, hi) by two binary
+// searches (the row is hub-ascending).
+__device__ __forceinline__ void row_window(const SegView& s, uint64_t b, uint64_t e, uint32_t hlo, uint32_t hhi,
+                                           uint64_t& lo, uint64_t& hi) {
+  uint64_t a = b, c = e;
+  while (a < c) {
+    const uint64_t m = (a + c) >> 1;
+    uint32_t h, d;
+    seg_entry(s, m, h, d);
+    if (h < hlo) a = m + 1;
+    else c = m;
+  }
+  lo = a;
+  c = e;
+  while (a < c) {
+    const uint64_t m = (a + c) >> 1;
+    uint32_t h, d;
+    seg_entry(s, m, h, d);
+    if (h < hhi) a = m + 1;
+    else c = m;
+  }
+  hi = a;
+}
+
+// cnt[v] = entries of vertex v's row (row row_of[v]) with hub in [hlo, hhi),
+// over the k segments; with noff set, those entries are written from noff[v].
+__global__ void window_kernel(const SegView* views, uint32_t k, const uint32_t* row_of, uint32_t n, uint32_t hlo,
+                              uint32_t hhi, uint32_t* cnt, const uint64_t* noff, uint32_t* hub, int32_t* dist) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = row_of[v];
+    uint64_t pos = noff ? noff[v] : 0, c = 0;
+    for (uint32_t sg = 0; sg < k; ++sg) {
+      const SegView s = views[sg];
+      uint64_t lo, hi;
+      row_window(s, s.off[r], s.off[r + 1], hlo, hhi, lo, hi);
+      if (noff)
+        for (uint64_t j = lo; j < hi; ++j, ++pos) {
+          uint32_t h, d;
+          seg_entry(s, j, h, d);
+          hub[pos] = h;
+          dist[pos] = (int32_t)d;
+        }
+      c += hi - lo;
+    }
+    if (!noff) cnt[v] = (uint32_t)c;
+  }
+}
+
 __global__ void decode_kernel(SegView seg, uint64_t count, uint64_t* out) {
   for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < count; k += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t h, d;
@@ -772,6 +820,15 @@ hipError_t launch_row_ranges(const SegView* views, uint32_t k, const uint32_t* r
   if (!n || !k) return hipSuccess;
   hipLaunchKernelGGL(row_ranges_kernel, dim3(grid_for((uint64_t)n * k, 256, 8192)), dim3(256), 0, s, views, k, row_of,
                      n, rng);
+  return hipGetLastError();
+}
+
+hipError_t launch_window(const SegView* views, uint32_t k, const uint32_t* row_of, uint32_t n, uint32_t hlo,
+                         uint32_t hhi, uint32_t* cnt, const uint64_t* noff, uint32_t* hub, int32_t* dist,
+                         hipStream_t s) {
+  if (!n || !k) return hipSuccess;
+  hipLaunchKernelGGL(window_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, views, k, row_of, n, hlo, hhi, cnt,
+                     noff, hub, dist);
   return hipGetLastError();
 }
 
