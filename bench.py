@@ -223,9 +223,14 @@ def copy_calibration(nbytes=1 << 30):
     log(f"copy calibration: {nbytes} bytes device to device")
 
 
-def cpu_baseline(g, order, seconds):
+def cpu_baseline(g, order, seconds, whole_build_ratio=True):
     """The oracle's GLL (run_paraPLL restated) on all host cores over the rank prefix
-    it completes within `seconds` of labeling+sorting+cleaning time."""
+    it completes within `seconds` of labeling+sorting+cleaning time. On the default
+    1M road-like workload the rate is scaled to a whole build by the whole/prefix
+    ratio measured on one host for the same graph and order
+    (oracle/c2_whole_build.json, written by oracle/calibrate_cpu.py: the whole
+    build's labels/s over the same bounded prefix's), since the whole build takes
+    minutes."""
     from oracle import oracle as orc
     threads = os.cpu_count() or 1
     src, dst, w = g.arc_list(1)
@@ -233,10 +238,96 @@ def cpu_baseline(g, order, seconds):
     L, s = orc.gll_bounded(og, order, threads, seconds)
     tsec = float(s[0] + s[1] + s[2])
     labels = L.total()
-    return {"value": labels / tsec, "unit": "labels/s", "cores": threads, "kind": "port",
-            "sample": f"oracle GLL (shared_memory/akiba.hpp:132-217 restated), {threads} threads, rank prefix "
-                      f"[0, {int(s[4])}) of the same graph and order: {labels} labels in {tsec:.2f} s "
-                      f"(lt+st+ct, {int(s[3])} supersteps)"}
+    prefix_rate = labels / tsec
+    sample = (f"oracle GLL (shared_memory/akiba.hpp:132-217 restated), {threads} threads, rank prefix "
+              f"[0, {int(s[4])}) of the same graph and order: {labels} labels in {tsec:.2f} s "
+              f"(lt+st+ct, {int(s[3])} supersteps)")
+    out = {"value": prefix_rate, "unit": "labels/s", "cores": threads, "kind": "port", "sample": sample,
+           "prefix_value": prefix_rate}
+    cal_path = os.path.join(HERE, "oracle", "c2_whole_build.json")
+    if whole_build_ratio and os.path.exists(cal_path):
+        cal = json.load(open(cal_path))
+        ratio = cal["whole_over_prefix_rate"]
+        out["value"] = prefix_rate * ratio
+        out["whole_over_prefix"] = ratio
+        out["whole_build_s_estimate"] = 1197955678 / out["value"]
+        out["sample"] = (sample + f"; scaled to a whole build by {ratio:.3f}, the whole/prefix labels/s ratio "
+                         f"measured on one {cal['threads']}-thread host (whole build {cal['whole']['seconds']:.1f} s, "
+                         f"oracle/c2_whole_build.json)")
+        ref = json.load(open(os.path.join(HERE, "tests", "golden", "fullsize.json")))["c2"]
+        out["reference_hhl_whole_build"] = {"threads": ref["reference_threads"], "stdout": ref["reference_stdout"][0]}
+    return out
+
+
+def c4_leg(chl, t_start, budget_s, serve=True):
+    """C4 at its own size, on the same GPU: 16M points in the unit square, 3-NN
+    symmetrised, w = round(1e4 d) + 1, the 1024-root sampled-betweenness order
+    (computed on the device), PLaNT trees. One learned and, when the time left
+    allows, one static build (graph and order resident; the labels of the
+    previous build freed first), their digests compared; the tree kernel's
+    roofline; the query kernel and a C5 replay on the C4 labels with served
+    distances checked against chl_query. Stops before a step that would end past
+    budget_s seconds after t_start."""
+    res = {"workload": "road-like 3-NN, n=16000000, w=round(1e4*d)+1, 1024-root sampled-betweenness order"}
+    t = time.perf_counter()
+    g = chl.Graph.knn(16_000_000, 3, seed=1)
+    order, score, order_ms = g.order_betweenness_device(1024, seed=7, with_score=True)
+    res.update({"n": g.n, "undirected_edges": g.arcs(1) // 2, "graph_and_order_s": time.perf_counter() - t,
+                "device_order_ms": order_ms})
+    log(f"C4 graph + device order in {res['graph_and_order_s']:.1f} s (order {order_ms:.0f} ms)")
+    builds = {}
+    est = 200.0  # a first C4 build's time before any was measured
+    for name, batching in (("learned", 1), ("static", 0)):
+        left = budget_s - (time.perf_counter() - t_start)
+        if left < 1.3 * est:
+            res[f"{name}_skipped"] = f"{left:.0f} s left of the time budget, a build takes about {est:.0f} s"
+            log(f"C4 {name} build skipped: {res[name + '_skipped']}")
+            continue
+        t0 = time.perf_counter()
+        try:
+            L = chl.build(g, order, batching=batching, vertex_score=score.ctypes.data)
+        except chl.ChlError as e:
+            res[f"{name}_error"] = str(e)
+            log(f"C4 {name} build failed: {e}")
+            continue
+        dt = time.perf_counter() - t0
+        est = dt
+        s = L.stats.as_dict()
+        relax_s = s["relax_ms"] / 1e3
+        achieved = s["relax_bytes"] / relax_s / 1e9 if relax_s > 0 else 0.0
+        builds[name] = {"build_s": dt, "labels": int(s["labels"]), "labels_per_s": s["labels"] / dt,
+                        "avg_label": s["avg_label"], "batches": s["batches"], "slots": s["slots_used"],
+                        "lt_s": s["lt"], "st_s": s["st"], "ct_s": s["ct"], "segments": s["segments"],
+                        "store_GB": s["store_bytes"] / 1e9, "digest": hex(L.checksum(0)),
+                        "roofline": {"bound": "hbm", "kernel": "tree_kernel", "achieved": achieved,
+                                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                                     "bytes_per_build": s["relax_bytes"], "kernel_ms_per_build": s["relax_ms"]}}
+        log(f"C4 {name} build: {dt:.1f} s, {s['labels']} labels, {s['segments']} segments, "
+            f"{s['store_bytes'] / 1e9:.1f} GB")
+        if name == "learned":
+            rng = np.random.default_rng(17)
+            big = rng.integers(0, g.n, (1 << 20, 2), dtype=np.uint32)
+            res["query"] = query_kernel_rate(chl, L, big)
+            if serve:
+                srv = chl.Server(L, streams=4, max_inflight_pairs=1 << 21, max_batch_pairs=1 << 16)
+                res["serving"] = []
+                for rate in (5.0e7, 1.0e8):
+                    nreq = int(rate * 0.2 / 4096)
+                    arrivals = np.cumsum(rng.exponential(4096.0 / rate * 1e6, nreq))
+                    pairs = rng.integers(0, g.n, (nreq * 4096, 2), dtype=np.uint32)
+                    r = replay(srv, L, pairs, np.full(nreq, 4096), arrivals, rng)
+                    r["offered_queries_per_s"] = rate
+                    res["serving"].append(r)
+                    log(f"C4 serving 4096-pair requests offered {rate:.2e}/s: served {r['served_queries_per_s']:.3e}/s "
+                        f"p50 {r['p50_us']:.0f} us p99 {r['p99_us']:.0f} us")
+                srv.close()
+                del srv
+        del L
+    res["builds"] = builds
+    if "learned" in builds and "static" in builds:
+        res["learned_speedup"] = builds["static"]["build_s"] / builds["learned"]["build_s"]
+        res["labels_identical"] = builds["static"]["digest"] == builds["learned"]["digest"]
+    return res
 
 
 def main():
@@ -259,11 +350,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-query", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="query kernel only, no server replays")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 16M-vertex C4 leg")
+    ap.add_argument("--budget", type=float, default=480.0,
+                    help="seconds after the start past which the C4 leg starts no further build")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
+    t_start = time.perf_counter()
     from plant_amd import chl
     if chl.device_count() < 1:
         print("bench.py needs a gfx950 device", file=sys.stderr)
@@ -351,8 +446,12 @@ def main():
         log(f"other batching: {t_other:.3f} s, labels identical: {same}")
     if not args.no_query:
         line["query"] = query_bench(chl, L, g, serve=not args.no_serve)
+    if not args.no_c4 and args.workload == "road" and not args.n:
+        del L
+        line["c4"] = c4_leg(chl, t_start, args.budget, serve=not args.no_serve)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(g, order, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(g, order, args.cpu_seconds,
+                                            whole_build_ratio=args.workload == "road" and not args.n)
     print(json.dumps(line), flush=True)
 
 

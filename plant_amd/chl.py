@@ -448,6 +448,7 @@ class Server:
         if getattr(self, "_h", None):
             lib().chl_server_free(self._h)
             self._h = None
+        self._labels = None  # the label handle may be freed now
 
     def __del__(self):
         try:

@@ -925,19 +925,25 @@ struct Builder {
     }
     CHL_HIP(hipStreamSynchronize(stream));
     {
-      // the search slots and batch buffers go back to the device, then the
-      // store's segments are merged into one where memory allows (queries then
-      // merge-join one row pair per side)
-      for (auto* b : {&cell}) b->release();
-      for (auto* b : {&qa, &qb, &vis, &gkey, &gval}) b->release();
-      qd.release();
-      for (auto& b : sb) {
-        b.row.release();
-        b.hd.release();
-        b.keep.release();
-        b.seg.reset();
+      // the store's segments are merged into one where memory allows (queries
+      // then merge-join one row pair per side); when the merged copy does not
+      // fit beside the search slots and batch buffers, those go back to the
+      // device first (otherwise they stay in the allocation cache for the next
+      // build)
+      size_t freeb = 0, totalb = 0;
+      CHL_HIP(hipMemGetInfo(&freeb, &totalb));
+      if (!o.keep_segments && L.nseg() > 1 && (double)L.bytes() * 1.2 > (double)freeb) {
+        for (auto* b : {&cell}) b->release();
+        for (auto* b : {&qa, &qb, &vis, &gkey, &gval}) b->release();
+        qd.release();
+        for (auto& b : sb) {
+          b.row.release();
+          b.hd.release();
+          b.keep.release();
+          b.seg.reset();
+        }
+        dev_cache_flush();
       }
-      dev_cache_flush();
       CHL_HIP(hipEventRecord(ev[6], stream));
       if (!o.keep_segments) {
         rc = L.consolidate(stream);
