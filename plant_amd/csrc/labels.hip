@@ -240,13 +240,18 @@ __global__ __launch_bounds__(256) void sort_long_rows_kernel(const uint64_t* off
 
 // ---- in-batch cleaning -----------------------------------------------------------
 // clean_cover (shared_memory/labeling.hpp:78-88) over the batch labels. One
-// wavefront per row v of this side: v's row (hub-ascending) is staged in LDS;
-// for each candidate (h, d) of it the lanes walk row h of the other side's
-// batch labels 64 entries at a time and look each hub x < h up among v's
-// entries before the candidate (those are exactly v's batch hubs below h); the
-// candidate is covered when some x gives L(v)[x] + L(h)[x] <= d.
+// wavefront per row v of this side: v's row (hub-ascending) is staged in LDS.
+// For a candidate (h, d) of it the lanes take 64 entries at a time of row h of
+// the other side's batch labels and look each hub x < h up among v's entries
+// before the candidate (exactly v's batch hubs below h); the candidate is
+// covered when some x gives L(v)[x] + L(h)[x] <= d. The first chunks of
+// kCleanAhead candidates are loaded together, so their memory latencies
+// overlap; a candidate still open after its first chunk walks on alone. A
+// candidate that is kept has read its whole other row below h, as the
+// reference's merge does: those reads dominate the kernel.
 constexpr int kCleanWaves = 4;
 constexpr uint32_t kCleanTile = 1024;
+constexpr uint32_t kCleanAhead = 4;
 
 __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* a, uint32_t len, uint32_t x) {
   uint32_t lo = 0, hi = len;
@@ -256,6 +261,31 @@ __device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* a, uint32_t 
     else hi = mid;
   }
   return lo;
+}
+
+// Whether entry (x, dx) of the other row covers candidate c = (h, d): x is among
+// v's entries before c with L(v)[x] + dx <= d.
+__device__ __forceinline__ bool covers(const SegView& seg, uint64_t rb, bool staged, const uint32_t* sh,
+                                       const uint32_t* sd, uint32_t c, uint32_t d, uint32_t x, uint32_t dx) {
+  uint32_t hv = 0xFFFFFFFFu, dv = 0;
+  if (staged) {
+    const uint32_t at = lds_lower_bound(sh, c, x);
+    if (at < c) {
+      hv = sh[at];
+      dv = sd[at];
+    }
+  } else {
+    uint32_t lo = 0, hi = c;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      uint32_t mh, md;
+      seg_entry(seg, rb + mid, mh, md);
+      if (mh < x) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < c) seg_entry(seg, rb + lo, hv, dv);
+  }
+  return hv == x && dv + dx <= d;
 }
 
 __global__ __launch_bounds__(256) void clean_kernel(SegView seg, SegView oseg, uint32_t rows, uint8_t* keep,
@@ -270,6 +300,8 @@ __global__ __launch_bounds__(256) void clean_kernel(SegView seg, SegView oseg, u
     const uint32_t len = (uint32_t)(re - rb);
     if (len == 0) continue;
     const bool staged = len <= kCleanTile;
+    const uint32_t* sh = s_h[wave];
+    const uint32_t* sd = s_d[wave];
     if (staged) {
       for (uint32_t j = lane; j < len; j += 64) {
         uint32_t h, d;
@@ -287,8 +319,8 @@ __global__ __launch_bounds__(256) void clean_kernel(SegView seg, SegView oseg, u
       uint64_t ob = 0, oe = 0;
       if (c0 + lane < len) {
         if (staged) {
-          ch = s_h[wave][c0 + lane];
-          cd = s_d[wave][c0 + lane];
+          ch = sh[c0 + lane];
+          cd = sd[c0 + lane];
         } else {
           seg_entry(seg, rb + c0 + lane, ch, cd);
         }
@@ -297,49 +329,43 @@ __global__ __launch_bounds__(256) void clean_kernel(SegView seg, SegView oseg, u
       }
       uint64_t kept = 0;  // keep bits of these candidates
       const uint32_t cn = min(64u, len - c0);
-      for (uint32_t i = 0; i < cn; ++i) {
-        const uint32_t c = c0 + i;
-        const uint32_t h = __shfl(ch, i), d = __shfl(cd, i);
-        const uint64_t b = __shfl(ob, i), e = __shfl(oe, i);
-        bool covered = false;
-        for (uint64_t base = b; base < e; base += 64) {
-          const uint64_t j = base + lane;
-          bool hit = false, past = true;
-          if (j < e) {
-            uint32_t x, dx;
-            seg_entry(oseg, j, x, dx);
-            past = x >= h;
-            if (!past) {
-              uint32_t at, hv, dv;
-              if (staged) {
-                at = lds_lower_bound(s_h[wave], c, x);
-                hv = at < c ? s_h[wave][at] : 0xFFFFFFFFu;
-                dv = at < c ? s_d[wave][at] : 0;
-              } else {
-                uint32_t lo = 0, hi = c;
-                while (lo < hi) {
-                  const uint32_t mid = (lo + hi) >> 1;
-                  uint32_t mh, md;
-                  seg_entry(seg, rb + mid, mh, md);
-                  if (mh < x) lo = mid + 1;
-                  else hi = mid;
-                }
-                at = lo;
-                hv = 0xFFFFFFFFu;
-                dv = 0;
-                if (at < c) seg_entry(seg, rb + at, hv, dv);
-              }
-              hit = hv == x && dv + dx <= d;
-            }
-          }
-          reads += (j < e) ? 1 : 0;
-          if (__any(hit)) {
-            covered = true;
-            break;
-          }
-          if (__all(past)) break;
+      for (uint32_t i0 = 0; i0 < cn; i0 += kCleanAhead) {
+        // the first chunks of kCleanAhead candidates, loaded together
+        uint32_t x[kCleanAhead], dx[kCleanAhead], h[kCleanAhead], d[kCleanAhead];
+        uint64_t b[kCleanAhead], e[kCleanAhead];
+        bool in[kCleanAhead];
+#pragma unroll
+        for (uint32_t k = 0; k < kCleanAhead; ++k) {
+          const uint32_t i = min(i0 + k, cn - 1);
+          h[k] = __shfl(ch, i);
+          d[k] = __shfl(cd, i);
+          b[k] = __shfl(ob, i);
+          e[k] = __shfl(oe, i);
+          in[k] = i0 + k < cn && b[k] + lane < e[k];
+          x[k] = 0xFFFFFFFFu;
+          dx[k] = 0;
+          if (in[k]) seg_entry(oseg, b[k] + lane, x[k], dx[k]);
         }
-        if (!covered) kept |= 1ull << i;
+#pragma unroll
+        for (uint32_t k = 0; k < kCleanAhead; ++k) {
+          if (i0 + k >= cn) break;
+          const uint32_t c = c0 + i0 + k;
+          reads += in[k] ? 1 : 0;
+          bool past = x[k] >= h[k];
+          bool covered = __any(!past && covers(seg, rb, staged, sh, sd, c, d[k], x[k], dx[k]));
+          // the rest of the other row, 64 entries at a time
+          for (uint64_t base = b[k] + 64; !covered && !__all(past) && base < e[k]; base += 64) {
+            const uint64_t j = base + lane;
+            uint32_t xx = 0xFFFFFFFFu, dd = 0;
+            if (j < e[k]) {
+              seg_entry(oseg, j, xx, dd);
+              ++reads;
+            }
+            past = xx >= h[k];
+            covered = __any(!past && covers(seg, rb, staged, sh, sd, c, d[k], xx, dd));
+          }
+          if (!covered) kept |= 1ull << (i0 + k);
+        }
       }
       if (lane < cn) keep[rb + c0 + lane] = (uint8_t)((kept >> lane) & 1ull);
     }
