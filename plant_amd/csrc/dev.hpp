@@ -158,10 +158,30 @@ enum : uint32_t {
                     // its vis list (touched[b] of them); no labels, no reset
 };
 
-// Packed per-vertex search state: (dist << 32) | low, where low is the smallest
-// rank on the best path found so far (TREE_FLAGS) or the root rank, and 0 once
-// the vertex is pruned at that distance. Unreached = all ones.
+// Packed per-vertex search state: (epoch key << 34) | (dist << 2) | tag.
+//  * tag: kTagClean when every vertex on the best path found so far, the vertex
+//    itself included, is ranked at or below the root (TREE_FLAGS; without it
+//    every reached vertex is clean), kTagCovered when some vertex on it is more
+//    important than the root, kTagPruned once the vertex is pruned at that
+//    distance. Among equal distances the 64-bit minimum keeps the smaller tag,
+//    so a path through a more important vertex wins a tie, and an
+//    equal-distance update cannot revive a pruned vertex.
+//  * epoch key: kEpochMax - (the slot's tree count). Every tree a slot starts
+//    gets a smaller key, so a cell left by an earlier tree of the slot compares
+//    above every state of the current one and reads as unreached: the cells
+//    need no reset between trees. Unreached = all ones (key kEpochMax).
 static constexpr uint64_t kStateInf = ~0ull;
+static constexpr uint32_t kTagPruned = 0, kTagCovered = 1, kTagClean = 2;
+static constexpr uint32_t kEpochMax = (1u << 30) - 1;
+// A slot's stamps grow by two per round over all its trees; past this bound
+// (and past kEpochMax trees) the slot's cells are reset and both counts restart.
+static constexpr uint32_t kStampMax = 0x7FFFFFFFu;
+__host__ __device__ inline uint64_t state_word(uint32_t ekey, uint32_t d, uint32_t tag) {
+  return ((uint64_t)ekey << 34) | ((uint64_t)d << 2) | tag;
+}
+__host__ __device__ inline uint32_t state_dist(uint64_t x) { return (uint32_t)(x >> 2); }
+__host__ __device__ inline uint32_t state_tag(uint64_t x) { return (uint32_t)x & 3u; }
+__host__ __device__ inline uint32_t state_ekey(uint64_t x) { return (uint32_t)(x >> 34); }
 
 // Counters of algorithmic work, summed over a launch.
 // CTR_PROCESSED / CTR_ARCS: vertices processed and arcs scanned, every
@@ -200,8 +220,11 @@ struct TreeParams {
   uint32_t R;
   uint32_t delta;
   // per-slot scratch, slot = workgroup index: cell holds 2 n words per slot
-  // (state word, stamp and rank), the lists and the pile n entries per slot
+  // (state word, stamp and rank), the lists and the pile n entries per slot;
+  // seq holds two words per slot (trees started, next stamp base), zero when
+  // the slot's cells are initialised and carried from launch to launch
   uint64_t* cell;
+  uint32_t* seq;
   uint32_t* qa;
   uint32_t* qb;
   uint32_t* vis;
@@ -225,7 +248,8 @@ struct TreeParams {
 // 256 or 1024.
 hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, int threads_per_root, hipStream_t s);
 // count cells set to the unreached state (state all ones, stamp 0) with the
-// rank of their vertex (cell i: device id i mod n)
+// rank of their vertex (cell i: device id i mod n); the slots' seq words must be
+// zeroed with them
 hipError_t launch_init_cells(uint64_t* cell, size_t count, const uint32_t* rank, uint32_t n, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t v, size_t count, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t count, hipStream_t s);

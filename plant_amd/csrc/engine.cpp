@@ -195,7 +195,7 @@ struct Builder {
   hipStream_t stream = nullptr;
 
   DBuf<uint64_t> cell;
-  DBuf<uint32_t> qa, qb, vis, gkey, gval, grank;
+  DBuf<uint32_t> qa, qb, vis, gkey, gval, grank, seq;
   DBuf<uint64_t> qd;  // deferred piles
   std::vector<uint32_t> id_of_rank;  // rank -> device id
   std::vector<uint32_t> hids;        // staging of root ids
@@ -271,7 +271,9 @@ struct Builder {
     CHL_HIP(qb.reserve(cells));
     CHL_HIP(vis.reserve(cells));
     CHL_HIP(qd.reserve(cells));
+    CHL_HIP(seq.reserve(2 * (size_t)slots));
     CHL_HIP(launch_init_cells(cell.p, cells, grank.p, (uint32_t)n, stream));
+    CHL_HIP(launch_fill_u32(seq.p, 0, 2 * (size_t)slots, stream));
     return CHL_OK;
   }
 
@@ -358,6 +360,7 @@ struct Builder {
     p.delta = (uint32_t)o.delta;
     p.rank = grank.p;
     p.cell = cell.p;
+    p.seq = seq.p;
     p.qa = qa.p;
     p.qb = qb.p;
     p.vis = vis.p;
@@ -934,7 +937,7 @@ struct Builder {
       CHL_HIP(hipMemGetInfo(&freeb, &totalb));
       if (!o.keep_segments && L.nseg() > 1 && (double)L.bytes() * 1.2 > (double)freeb) {
         for (auto* b : {&cell}) b->release();
-        for (auto* b : {&qa, &qb, &vis, &gkey, &gval}) b->release();
+        for (auto* b : {&qa, &qb, &vis, &gkey, &gval, &seq}) b->release();
         qd.release();
         for (auto& b : sb) {
           b.row.release();

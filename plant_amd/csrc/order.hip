@@ -31,8 +31,10 @@ namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
+// distance of v in the slot's search (all ones when unreached: the slots hold
+// one search each, on cells initialised before it)
 __device__ __forceinline__ uint32_t tree_dist(const uint64_t* cell, uint32_t v) {
-  return (uint32_t)(cell[2 * (size_t)v] >> 32);
+  return state_dist(cell[2 * (size_t)v]);
 }
 
 __global__ void bc_init_kernel(BcTrees t) {

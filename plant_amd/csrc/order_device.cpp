@@ -70,7 +70,7 @@ int order_impl(const Graph& g, DeviceGraph& dg, int device, int64_t roots, uint6
   const size_t cells = (size_t)slots * n;
   *scratch = cells * 48;
   DBuf<uint64_t> cell, qd, acc, score_d, key, val, skey, sval;
-  DBuf<uint32_t> qa, qb, vis, parent, leaf, zero, inv, droots, touched, nleaf, tsize, lcount, spill;
+  DBuf<uint32_t> qa, qb, vis, parent, leaf, zero, inv, droots, touched, nleaf, tsize, lcount, spill, seq;
   DBuf<unsigned long long> ctr;
   DBuf<uint8_t> tmp;
   StreamDrain drain{stream};
@@ -87,12 +87,14 @@ int order_impl(const Graph& g, DeviceGraph& dg, int device, int64_t roots, uint6
   CHL_HIP(inv.reserve(n));
   CHL_HIP(droots.reserve(std::max<uint32_t>(R, 1)));
   for (auto* b : {&touched, &nleaf, &tsize, &lcount, &spill}) CHL_HIP(b->reserve(slots));
+  CHL_HIP(seq.reserve(2 * (size_t)slots));
   CHL_HIP(ctr.reserve(CTR_COUNT));
   CHL_HIP(hipMemcpyAsync(inv.p, dg.inv.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream));
   if (R) CHL_HIP(hipMemcpyAsync(droots.p, rs.data(), sizeof(uint32_t) * R, hipMemcpyHostToDevice, stream));
   // every vertex has rank 0 here: the searches expand every vertex they reach
   CHL_HIP(launch_fill_u32(zero.p, 0, n, stream));
   CHL_HIP(launch_init_cells(cell.p, cells, zero.p, (uint32_t)n, stream));
+  CHL_HIP(launch_fill_u32(seq.p, 0, 2 * (size_t)slots, stream));
   CHL_HIP(launch_fill_u64(score_d.p, 0, n, stream));
 
   const int ir = dg.sym ? 1 : 0;
@@ -104,6 +106,7 @@ int order_impl(const Graph& g, DeviceGraph& dg, int device, int64_t roots, uint6
   p.n = (uint32_t)n;
   p.delta = delta;
   p.cell = cell.p;
+  p.seq = seq.p;
   p.qa = qa.p;
   p.qb = qb.p;
   p.vis = vis.p;

@@ -17,23 +17,32 @@
 // queued once per round (stamp) and processed again, so every vertex is finally
 // processed at its final state.
 //
-// State word = (dist << 32) | low. With TREE_FLAGS, low is the smallest rank on
-// the best path found so far, including the vertex itself; a 64-bit atomicMin
-// then keeps the shortest distance and, among equal distances, the path through
-// the most important vertex. That is the tie rule of update_wo_prune
-// (distributed_memory/dijkstra.hpp:56-78): a label (root, d) is emitted iff the
-// final low equals the root's rank. Without TREE_FLAGS (GLL), low is the root
-// rank and vertices ranked above the root are not expanded (akiba.hpp:92).
+// State word = (epoch key << 34) | (dist << 2) | tag (state_word in dev.hpp).
+// With TREE_FLAGS the tag is clean while every vertex on the best path found so
+// far, the vertex itself included, is ranked at or below the root, and covered
+// otherwise; a 64-bit atomicMin keeps the shortest distance and, among equal
+// distances, the covered tag: the path through a more important vertex. That is
+// the tie rule of update_wo_prune (distributed_memory/dijkstra.hpp:56-78): a
+// label (root, d) is emitted iff the final tag is clean. Without TREE_FLAGS
+// (GLL) every reached vertex is clean and vertices ranked above the root are not
+// expanded (akiba.hpp:92).
+//
+// The epoch key falls by one with every tree a slot starts, so the cells a tree
+// leaves behind read as unreached to the next one: the search state is never
+// written back after a tree, only read once to emit its labels. Round stamps
+// likewise keep growing over a slot's trees, so a stale stamp never equals a
+// current one. The slot's two counts (seq) persist from launch to launch; past
+// their bounds the slot's cells are reset in place and the counts restart.
 //
 // With TREE_PRUNE a processed vertex u at distance d is dropped when the
 // completed labels (all hubs < prefix) already give d(root, u) <= d: one
 // wavefront scans u's label row coalesced and probes each hub in the root's
-// label, held in an LDS hash table (akiba.hpp:94-110). A pruned vertex gets low
-// = 0 at that distance, so an equal-distance update cannot revive it.
+// label, held in an LDS hash table (akiba.hpp:94-110). A pruned vertex gets the
+// pruned tag at that distance, so an equal-distance update cannot revive it.
 //
 // With TREE_FLAGS the search stops at a bucket boundary once no queued vertex
-// has low == root: every later vertex is then reached through a more important
-// vertex (the queue_left test, distributed_memory/akiba.hpp:95).
+// is clean: every later vertex is then reached through a more important vertex
+// (the queue_left test, distributed_memory/akiba.hpp:95).
 #include "dev.hpp"
 
 namespace chl {
@@ -101,24 +110,26 @@ __device__ __forceinline__ ulonglong2 load_cell(const uint64_t* cell, uint32_t v
   return *reinterpret_cast<const ulonglong2*>(cell + 2 * (size_t)v);
 }
 
-// Relaxes arc (u -> v, w) from state (d, lo), given a recent read `seen` of v's
-// cell (v's rank is fixed; a stale state only costs an atomic that changes
-// nothing); queues v in N once per round.
+// Relaxes arc (u -> v, w) from u's distance d and tag, given a recent read
+// `seen` of v's cell (v's rank is fixed; a stale state only costs an atomic
+// that changes nothing); registers v in vis at its first state of this tree
+// (epoch key ekey) and queues it in N once per round. rr: the root's rank.
 template <bool FLAGS>
 __device__ __forceinline__ void relax_seen(uint64_t* cell, uint32_t* N, uint32_t* vis, uint32_t* s_nsize,
-                                           uint32_t* s_nvis, uint32_t nstamp, uint64_t arc, ulonglong2 seen,
-                                           uint32_t d, uint32_t lo, uint32_t B, Book& bk) {
+                                           uint32_t* s_nvis, uint32_t nstamp, uint32_t ekey, uint32_t rr,
+                                           uint64_t arc, ulonglong2 seen, uint32_t d, uint32_t tag, uint32_t B,
+                                           Book& bk) {
   const uint32_t v = (uint32_t)(arc >> 32);
   const uint32_t rv = (uint32_t)(seen.y >> 32);
   const uint32_t nd = d + (uint32_t)arc;
-  const uint32_t nlo = FLAGS ? min(lo, rv) : lo;
-  const uint64_t nv = ((uint64_t)nd << 32) | nlo;
+  const uint32_t ntag = FLAGS && rv < rr ? kTagCovered : tag;
+  const uint64_t nv = state_word(ekey, nd, ntag);
   uint64_t* sv = cell + 2 * (size_t)v;
   if (nv < seen.x) {
     const uint64_t old = atomicMin((unsigned long long*)sv, (unsigned long long)nv);
     if (nv < old) {
       bk.note(nd, B);
-      if (old == kStateInf) vis[atomicAdd(s_nvis, 1u)] = v;
+      if (state_ekey(old) != ekey) vis[atomicAdd(s_nvis, 1u)] = v;
       if (atomicExch(stamp_of(cell, v), nstamp) != nstamp) N[atomicAdd(s_nsize, 1u)] = v;
     }
   }
@@ -126,10 +137,10 @@ __device__ __forceinline__ void relax_seen(uint64_t* cell, uint32_t* N, uint32_t
 
 template <bool FLAGS>
 __device__ __forceinline__ void relax(uint64_t* cell, uint32_t* N, uint32_t* vis, uint32_t* s_nsize,
-                                      uint32_t* s_nvis, uint32_t nstamp, uint64_t arc, uint32_t d, uint32_t lo,
-                                      uint32_t B, Book& bk) {
-  relax_seen<FLAGS>(cell, N, vis, s_nsize, s_nvis, nstamp, arc, load_cell(cell, (uint32_t)(arc >> 32)), d, lo, B,
-                    bk);
+                                      uint32_t* s_nvis, uint32_t nstamp, uint32_t ekey, uint32_t rr, uint64_t arc,
+                                      uint32_t d, uint32_t tag, uint32_t B, Book& bk) {
+  relax_seen<FLAGS>(cell, N, vis, s_nsize, s_nvis, nstamp, ekey, rr, arc, load_cell(cell, (uint32_t)(arc >> 32)), d,
+                    tag, B, bk);
 }
 
 // Arcs of one vertex relaxed per step of the lane-per-vertex loop: their head
@@ -171,6 +182,8 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
   // (a spilled root runs again in a later launch and is counted there)
   unsigned long long r_arcs = 0, r_prune = 0, r_settled = 0, r_sarcs = 0;
   if (tid < 6) s_ctr[tid] = 0;
+  // the slot's tree count and next stamp base (the same in every thread)
+  uint32_t epoch = p.seq[2 * slot], sbase = p.seq[2 * slot + 1];
 
   for (;;) {
     if (tid == 0 && KEEP) s_ri = blockIdx.x < p.R ? (uint32_t)blockIdx.x : kEmpty;
@@ -189,6 +202,19 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
     __syncthreads();
     const uint32_t ri = s_ri;
     if (ri == kEmpty) break;
+    if (epoch >= kEpochMax || sbase > kStampMax) {
+      // both counts at their bounds: the slot's cells back to unreached, then
+      // the counts restart
+      for (size_t i = tid; i < n; i += kThreads) {
+        cell[2 * i] = kStateInf;
+        cell[2 * i + 1] = (uint64_t)p.rank[i] << 32;
+      }
+      epoch = 0;
+      sbase = 0;
+      __syncthreads();
+    }
+    ++epoch;
+    const uint32_t ekey = kEpochMax - epoch;
     const uint32_t root = p.roots[ri];  // device id
     const uint32_t rr = p.rank[root];   // its rank = the hub of its labels
     const uint32_t skip = FLAGS ? (PRUNE ? p.prefix : 0u) : rr;  // ranks below are not expanded
@@ -222,8 +248,8 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
       }
     }
     if (tid == 0) {
-      cell[2 * (size_t)root] = (uint64_t)rr;  // dist 0, low = root rank
-      cell[2 * (size_t)root + 1] = ((uint64_t)rr << 32) | 1u;  // rank, stamp 1
+      cell[2 * (size_t)root] = state_word(ekey, 0, kTagClean);
+      cell[2 * (size_t)root + 1] = ((uint64_t)rr << 32) | (sbase + 1);  // rank, the stamp before round 1
       F[0] = root;
       vis[0] = root;
       s_fsize = 1;
@@ -269,8 +295,8 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
       // stamps of this round: queued for the next round, or put on the pile
       // (distinct, so that an improvement of a vertex put on the pile in the same
       // round still queues it for the next round)
-      const uint32_t nstamp = 2 * round;
-      const uint32_t pstamp = 2 * round + 1;
+      const uint32_t nstamp = sbase + 2 * round;
+      const uint32_t pstamp = nstamp + 1;
       Book bk;
       uint32_t dmin = kEmpty;
 
@@ -294,8 +320,8 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
           }
           const uint64_t x = c.x;
           const uint32_t ab = p.off[u], ae = p.off[u + 1];  // read beside the state
-          const uint32_t d = (uint32_t)(x >> 32);
-          const uint32_t lo = (uint32_t)x;
+          const uint32_t d = state_dist(x);
+          const uint32_t tag = state_tag(x);
           if (d >= B) {
             uint32_t* st = stamp_of(cell, u);
             bool to_next = adv;
@@ -338,7 +364,8 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
 #pragma unroll
             for (uint32_t j = 0; j < kArcStep; ++j)
               if (a0 + j < ae)
-                relax_seen<FLAGS>(cell, N, vis, &s_nsize, &s_nvis, nstamp, av[j], seen[j], d, lo, B, bk);
+                relax_seen<FLAGS>(cell, N, vis, &s_nsize, &s_nvis, nstamp, ekey, rr, av[j], seen[j], d, tag, B,
+                                  bk);
           }
         }
       } else {
@@ -346,8 +373,8 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
         for (uint32_t k = wave; k < fsize; k += kWaves) {
           const uint32_t u = F[k];
           const uint64_t x = cell[2 * (size_t)u];
-          const uint32_t d = (uint32_t)(x >> 32);
-          const uint32_t lo = (uint32_t)x;
+          const uint32_t d = state_dist(x);
+          const uint32_t tag = state_tag(x);
           if (d >= B) {
             if (lane == 0) {
               bk.note(d, B);
@@ -356,7 +383,7 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
             continue;
           }
           const uint32_t ru = rank_of(cell, u);
-          if (ru < skip || lo == 0) continue;  // lo == 0: pruned at this distance already
+          if (ru < skip || tag == kTagPruned) continue;  // pruned at this distance already
           if (lane == 0) ++c_proc;
           bool pruned = false;
           uint64_t scanned = 0;
@@ -383,13 +410,13 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
           if (lane == 0) r_prune += scanned;
           if (pruned) {
             if (lane == 0)
-              atomicMin((unsigned long long*)&cell[2 * (size_t)u], (unsigned long long)((uint64_t)d << 32));
+              atomicMin((unsigned long long*)&cell[2 * (size_t)u], (unsigned long long)state_word(ekey, d, kTagPruned));
             continue;
           }
           const uint32_t ab = p.off[u], ae = p.off[u + 1];
           if (lane == 0) r_arcs += ae - ab;
           for (uint32_t a = ab + lane; a < ae; a += 64)
-            relax<FLAGS>(cell, N, vis, &s_nsize, &s_nvis, nstamp, p.arc[a], d, lo, B, bk);
+            relax<FLAGS>(cell, N, vis, &s_nsize, &s_nvis, nstamp, ekey, rr, p.arc[a], d, tag, B, bk);
         }
       }
       // bookkeeping of the next frontier (wavefront reductions, then LDS)
@@ -412,11 +439,11 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
         for (uint32_t k = tid; k < nsize + pile; k += kThreads) {
           if (s_clean) break;
           if (k < nsize) {
-            if ((uint32_t)cell[2 * (size_t)N[k]] == rr) s_clean = 1;
+            if (state_tag(cell[2 * (size_t)N[k]]) == kTagClean) s_clean = 1;
           } else {
             const uint64_t e = D[k - nsize];
             const ulonglong2 c = load_cell(cell, (uint32_t)(e >> 32));
-            if ((uint32_t)c.y == (uint32_t)e && (uint32_t)c.x == rr) s_clean = 1;
+            if ((uint32_t)c.y == (uint32_t)e && state_tag(c.x) == kTagClean) s_clean = 1;
           }
         }
         __syncthreads();
@@ -429,17 +456,18 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
       ++c_rounds;
       __syncthreads();
     }
+    sbase += 2 * round + 2;  // above every stamp of this tree
 
     if (KEEP) {
       if (tid == 0) p.touched[ri] = s_nvis;
       break;
     }
 
-    // Emit (root rank, d) at every touched vertex whose final low is the root's
-    // rank. One pass reads each touched cell once, keeps the label's (rank, d) in
-    // the two frontier lists (free once the search has ended) and restores the
-    // cell to its unreached state with its rank; then the root's output range is
-    // reserved and the kept labels are written out, coalesced.
+    // Emit (root rank, d) at every touched vertex whose final tag is clean. One
+    // pass reads each touched cell once (and writes none: the next tree's epoch
+    // key retires them) and keeps the label's (rank, d) in the two frontier lists
+    // (free once the search has ended); then the root's output range is reserved
+    // and the kept labels are written out, coalesced.
     const uint32_t nvis = s_nvis;
     if (tid == 0) s_emit = 0;
     __syncthreads();
@@ -447,8 +475,8 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
       const uint32_t v = vis[k];
       const ulonglong2 c = load_cell(cell, v);
       const uint32_t rv = (uint32_t)(c.y >> 32);
-      const uint32_t dv = (uint32_t)(c.x >> 32), lv = (uint32_t)c.x;
-      if (lv == rr && rv >= rr) {
+      const uint32_t dv = state_dist(c.x), tv = state_tag(c.x);
+      if (tv == kTagClean && rv >= rr) {
         const uint32_t j = atomicAdd(&s_emit, 1u);
         F[j] = rv;
         N[j] = dv;
@@ -458,9 +486,8 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
         // settled: processed at its final distance; expanded unless pruned
         // (low 0) or ranked above a GLL root
         ++r_settled;
-        if ((!PRUNE || lv != 0) && (FLAGS || rv >= skip)) r_sarcs += p.off[v + 1] - p.off[v];
+        if ((!PRUNE || tv != kTagPruned) && (FLAGS || rv >= skip)) r_sarcs += p.off[v + 1] - p.off[v];
       }
-      *reinterpret_cast<ulonglong2*>(cell + 2 * (size_t)v) = make_ulonglong2(kStateInf, (uint64_t)rv << 32);
     }
     __syncthreads();
     const uint32_t emit = s_emit;
@@ -512,6 +539,8 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
   if (lane == 0 && c_maxd) atomicMax(&p.ctr[CTR_MAXD], (unsigned long long)c_maxd);
   __syncthreads();
   if (tid == 0) {
+    p.seq[2 * slot] = epoch;
+    p.seq[2 * slot + 1] = sbase;
     if (s_ctr[1]) atomicAdd(&p.ctr[CTR_PROCESSED], s_ctr[1]);
     if (s_ctr[2]) atomicAdd(&p.ctr[CTR_ARCS], s_ctr[2]);
     if (s_ctr[3]) atomicAdd(&p.ctr[CTR_PRUNE_ENT], s_ctr[3]);

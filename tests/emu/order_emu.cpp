@@ -40,7 +40,7 @@ int main(int argc, char** argv) {
   const size_t cells = (size_t)S * n;
   std::vector<uint64_t> cell(2 * cells), qd(cells), acc(cells), score(n, 0);
   std::vector<uint32_t> qa(cells), qb(cells), vis(cells), parent(cells), leaf(cells), zero(n, 0);
-  std::vector<uint32_t> touched(S), nleaf(S), tsize(S), lcount(S), spill(S);
+  std::vector<uint32_t> touched(S), nleaf(S), tsize(S), lcount(S), spill(S), seq(2 * S, 0);
   std::vector<unsigned long long> ctr(CTR_COUNT, 0);
   launch_init_cells(cell.data(), cells, zero.data(), n, nullptr);
   TreeParams p{};
@@ -50,6 +50,7 @@ int main(int argc, char** argv) {
   p.n = n;
   p.delta = delta;
   p.cell = cell.data();
+  p.seq = seq.data();
   p.qa = qa.data();
   p.qb = qb.data();
   p.vis = vis.data();

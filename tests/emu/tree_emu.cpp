@@ -4,8 +4,11 @@
 //   in:  u32 n, m, R, delta, threads, slots, flags; u32 off[n + 1]; u64 arc[m];
 //        u32 rank[n]; u32 roots[R] (vertex ids)
 //   out: u64 labels; (u64 key = row << 32 | hub, u64 ent = hub << 32 | dist)
-//        per label; u32 1 when every search
-//        cell came back to its unreached state (rank kept), else 0
+//        per label; u32 1 when every search cell is consistent with its
+//        slot's counts after the launch (rank kept, no state of a later tree
+//        than the slot's last, no stamp at or above its next stamp base), else 0
+// EMU_EPOCH0 / EMU_SBASE0 start every slot's tree count and stamp base at the
+// given values, so a launch crosses their bounds and resets the slot's cells.
 #include "../../plant_amd/csrc/tree.hip"
 
 #include <cstdio>
@@ -36,7 +39,12 @@ int main(int argc, char** argv) {
     cell[2 * i] = kStateInf;
     cell[2 * i + 1] = (uint64_t)rank[i % n] << 32;
   }
-  std::vector<uint32_t> qa(cells), qb(cells), vis(cells), spill(R), ts(R), lc(R), gk(1), gv(1);
+  std::vector<uint32_t> qa(cells), qb(cells), vis(cells), spill(R), ts(R), lc(R), gk(1), gv(1), seq(2 * S, 0);
+  for (uint32_t b = 0; b < S; ++b) {
+    if (const char* e = getenv("EMU_EPOCH0")) seq[2 * b] = (uint32_t)strtoul(e, nullptr, 10);
+    if (const char* e = getenv("EMU_SBASE0")) seq[2 * b + 1] = (uint32_t)strtoul(e, nullptr, 10);
+  }
+  const std::vector<uint32_t> seq0 = seq;
   const uint64_t cap = (uint64_t)n * R + 1;
   std::vector<uint32_t> orow(cap);
   std::vector<uint64_t> ohd(cap);
@@ -51,6 +59,7 @@ int main(int argc, char** argv) {
   p.R = R;
   p.delta = delta;
   p.cell = cell.data();
+  p.seq = seq.data();
   p.qa = qa.data();
   p.qb = qb.data();
   p.vis = vis.data();
@@ -69,8 +78,15 @@ int main(int argc, char** argv) {
   if (const char* e = getenv("EMU_SEED")) emu_seed = strtoull(e, nullptr, 10);
   launch_tree(p, flags, S, (int)threads, nullptr);
   uint32_t restored = 1;
-  for (size_t i = 0; i < cells; ++i)
-    if (cell[2 * i] != kStateInf || cell[2 * i + 1] != ((uint64_t)rank[i % n] << 32)) restored = 0;
+  for (size_t i = 0; i < cells; ++i) {
+    const uint32_t b = (uint32_t)(i / n);
+    const uint64_t x = cell[2 * i], y = cell[2 * i + 1];
+    if ((uint32_t)(y >> 32) != rank[i % n]) restored = 0;
+    if (x != kStateInf && state_ekey(x) < kEpochMax - seq[2 * b]) restored = 0;
+    if ((uint32_t)y != 0 && (uint32_t)y >= seq[2 * b + 1]) restored = 0;
+    // a slot that ran trees has counted them
+    if (x != kStateInf && seq[2 * b] == seq0[2 * b] && seq0[2 * b] < kEpochMax) restored = 0;
+  }
   FILE* o = fopen(argv[2], "wb");
   if (!o) return 2;
   const uint64_t L = ctr[CTR_LABELS];

@@ -2,10 +2,13 @@
 with g++ against tests/emu/hip/hip_runtime.h and run as one persistent workgroup
 of 64 or 256 lanes (cooperative fibers that switch at barriers and, at
 random, at atomics). Its PLaNT labels must equal the oracle's canonical labels, and
-every search cell must come back to its unreached state.
+every search cell must be consistent with its slot's counts afterwards: the
+vertex's rank kept, no state newer than the slot's last tree, no stamp at or
+above the slot's next stamp base.
 
 This checks the kernel's indexing, bucket bookkeeping, deferred pile, queueing
-stamps, early stop and emission without a GPU. It is not a model of the GPU
+stamps, epoch keys (including the in-place reset of a slot whose counts reach
+their bounds), early stop and emission without a GPU. It is not a model of the GPU
 memory system: the GPU tests remain the parity tests proper."""
 import os
 import shutil
@@ -30,7 +33,7 @@ def emulator(tmp_path_factory):
     return exe
 
 
-def run_emulator(exe, tmp_path, g, order, threads, delta, slots=2, seed=12345):
+def run_emulator(exe, tmp_path, g, order, threads, delta, slots=2, seed=12345, env_extra=None):
     off, head, ln = g.csr(1)
     n = g.n
     arc = (head.astype(np.uint64) << np.uint64(32)) | ln.astype(np.uint32).astype(np.uint64)
@@ -44,7 +47,7 @@ def run_emulator(exe, tmp_path, g, order, threads, delta, slots=2, seed=12345):
         f.write(arc.tobytes())
         f.write(rank.tobytes())
         f.write(roots.tobytes())
-    env = dict(os.environ, EMU_SEED=str(seed))
+    env = dict(os.environ, EMU_SEED=str(seed), **(env_extra or {}))
     r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=600, env=env)
     assert r.returncode == 0, (r.returncode, r.stderr[-2000:])
     data = np.fromfile(out, np.uint8)
@@ -104,3 +107,21 @@ def test_pile_race_schedules(chl, orc, emulator, tmp_path, gseed, delta, seed):
     assert restored == 1
     for a, b in zip(got, ref.copy(0)):
         assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
+EPOCH_MAX = (1 << 30) - 1
+STAMP_MAX = 0x7FFFFFFF
+
+
+@pytest.mark.parametrize("start", [{"EMU_EPOCH0": str(EPOCH_MAX - 3)}, {"EMU_SBASE0": str(STAMP_MAX - 40)},
+                                   {"EMU_EPOCH0": str(EPOCH_MAX), "EMU_SBASE0": str(STAMP_MAX + 1)}])
+def test_slot_counts_at_their_bounds(chl, orc, emulator, tmp_path, start):
+    """A slot whose tree count or stamp base reaches its bound resets its cells in
+    place and restarts both: the labels stay canonical."""
+    for name, g, order in cases(chl):
+        src, dst, w = g.arc_list(1)
+        ref, _ = orc.gll(orc.OGraph.from_arcs(g.n, src, dst, w), order, threads=4)
+        got, restored = run_emulator(emulator, tmp_path, g, order, 64, 3, env_extra=start)
+        assert restored == 1, (name, start)
+        for a, b, what in zip(got, ref.copy(0), ("off", "hub", "dist")):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), (name, start, what)
