@@ -29,6 +29,7 @@ namespace chl {
 hipError_t dev_alloc(void** p, size_t bytes, size_t* got);
 void dev_free(void* p, size_t bytes);
 void dev_cache_flush();
+size_t dev_cached_bytes();  // bytes of freed blocks the allocator keeps for the current device
 
 // Device array that grows in place: a virtual range as large as the device
 // memory is reserved once and physical memory is mapped behind it as the array

@@ -64,6 +64,16 @@ void dev_free(void* p, size_t bytes) {
   (void)hipFree(p);
 }
 
+size_t dev_cached_bytes() {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_mu);
+  size_t b = 0;
+  for (auto& kv : g_cache)
+    if (kv.second.second == dev) b += kv.first;
+  return b;
+}
+
 void dev_cache_flush() {
   std::lock_guard<std::mutex> lk(g_mu);
   int cur = 0;

@@ -175,6 +175,16 @@ struct SideBatch {
   uint32_t maxd = 0;    // largest label distance
 };
 
+// Default search slots: 2048 workgroups of 256 threads, eight wavefronts per
+// SIMD on a 256-CU device (the tree kernel uses 32 VGPRs). The searches are
+// bound by memory latency, not bandwidth: on the 1M-vertex road-like graph
+// 512, 1024 and 2048 slots took 1122, 934 and 875 ms of tree time per build
+// while their fetched bytes rose from 1.80e12 to 2.54e12.
+constexpr uint32_t kDefaultSlots = 2048;
+// Below this many slots (memory-limited: a large graph) the default
+// granularity gives large trees a workgroup of 1024 threads.
+constexpr uint32_t kWideBelow = 1024;
+
 // Search slots for a granularity: base_slots workgroups of 256 threads, as
 // many threads in flight at 64 or 1024 threads per root.
 static uint32_t slots_for(uint32_t base_slots, uint32_t tpr) {
@@ -259,6 +269,7 @@ struct Builder {
   int alloc_slots(uint32_t want) {
     size_t freeb = 0, totalb = 0;
     CHL_HIP(hipMemGetInfo(&freeb, &totalb));
+    freeb += dev_cached_bytes();  // blocks the allocator keeps are free to reuse
     const size_t per = (size_t)n * 36;  // 16-byte cell, three u32 lists and a u64 pile per vertex
     // a third of the free memory, at most a quarter of the device: on large
     // graphs the label store (twice, while a batch is merged) needs the rest
@@ -687,25 +698,29 @@ struct Builder {
     // the same threads in flight at every granularity: a wavefront per root runs
     // four roots where a workgroup runs one, a wide workgroup a quarter
     tpr = o.granularity == 1 ? 64u : o.granularity == 3 ? 1024u : 256u;
-    const uint32_t base_slots = o.slots > 0 ? (uint32_t)o.slots : 1024u;
+    const uint32_t base_slots = o.slots > 0 ? (uint32_t)o.slots : kDefaultSlots;
     rc = alloc_slots(o.autotune && o.slots <= 0 ? 4 * base_slots : slots_for(base_slots, tpr));
     active = std::min<uint32_t>(slots, slots_for(base_slots, tpr));
     if (rc) return rc;
-    // When device memory allows fewer search slots than asked for (a large
+    // When device memory allows fewer than kWideBelow search slots (a large
     // graph: every slot holds 36 bytes per vertex), the default granularity
     // gives each slot a workgroup of 1024 threads while the trees are large, so
-    // the threads in flight stay near what the full slot count would have; once
-    // the mean tree of a batch is small the next batch uses 256 threads again.
-    const bool auto_wide = o.granularity == 0 && !o.autotune && slots < slots_for(base_slots, 256);
+    // that more threads stay in flight; once the mean tree of a batch is small
+    // the next batch uses 256 threads again.
+    const bool auto_wide = o.granularity == 0 && !o.autotune && slots < kWideBelow;
     double last_tree_mean = -1.0;  // processed vertices per root in the last batch (-1: none yet)
     // Static schedule: a fixed width when batch_roots is given; otherwise widths
-    // doubling from the slot count up to 65536 roots (later roots have small
-    // trees, and fewer batches mean fewer appends to the store).
+    // doubling from the slot count (GLL: from 1024 roots) up to 65536 roots
+    // (later roots have small trees, and fewer batches mean fewer appends to the
+    // store). A GLL batch prunes only by the labels of earlier batches, so wider
+    // first batches leave more candidates to clean: at 200K vertices a first
+    // batch of 2048 roots doubled the build time against 1024.
     const bool fixed = o.batch_roots > 0;
     // (a batching policy may ask for batches of any width)
     const uint32_t static_cap = (uint32_t)std::max<int64_t>(1, std::min<int64_t>(n, fixed ? o.batch_roots : 65536));
     W = (uint32_t)(hooks && hooks->next_width && !fixed ? n : static_cap);
-    uint32_t width = fixed ? static_cap : std::min<uint32_t>(static_cap, std::max<uint32_t>(slots, 1024));
+    uint32_t width = fixed ? static_cap
+                           : std::min<uint32_t>(static_cap, o.mode == CHL_MODE_GLL ? 1024u : std::max<uint32_t>(slots, 1024));
     CHL_HIP(roots.reserve(W));
     CHL_HIP(roots2.reserve(W));
     CHL_HIP(spill.reserve(W));
