@@ -21,6 +21,7 @@ the continuous-batching server's p50/p99 latency).
 """
 import argparse
 import ctypes as C
+import hashlib
 import json
 import os
 import sys
@@ -33,20 +34,32 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 TREE_PROFILE = os.path.join(HERE, "profiles", "tree_kernel_knn1M.txt")
+TREE_PROFILE_C4 = os.path.join(HERE, "profiles", "tree_kernel_knn16M.txt")
+KERNEL_SOURCES = [os.path.join(HERE, "plant_amd", "csrc", f) for f in ("tree.hip", "dev.hpp")]
 
 
-def profiled_traffic(bytes_per_build, launches):
+def kernel_source_hash():
+    """sha256 over the tree kernel's sources (tree.hip, then dev.hpp)."""
+    h = hashlib.sha256()
+    for path in KERNEL_SOURCES:
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def profiled_traffic(bytes_per_build, launches, path=TREE_PROFILE):
     """HBM bytes per tree-kernel launch from the committed rocprofv3 summary of
     this workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by a 1 GiB copy),
-    when that summary's run did the same work: the same launch count and its
-    algorithmic bytes within 2% (label-correcting searches re-process a few
-    vertices differently from run to run). None otherwise."""
+    when that summary's run used the same kernel sources (their sha256) and did
+    the same work: the same launch count and its algorithmic bytes within 2%
+    (label-correcting searches re-process a few vertices differently from run to
+    run). None otherwise."""
     try:
-        with open(TREE_PROFILE) as f:
+        with open(path) as f:
             text = f.read()
     except OSError:
         return None
-    got_bytes = got_launches = traffic = None
+    got_bytes = got_launches = traffic = src = None
     for ln in text.splitlines():
         w = ln.split()
         if "bytes_per_build" in w and "(algorithmic)," in w:
@@ -55,11 +68,17 @@ def profiled_traffic(bytes_per_build, launches):
             got_launches = int(w[w.index("launches_per_build") + 1].rstrip(","))
         if ln.startswith("Per dispatch (the bench's traffic figure):"):
             traffic = float(ln.split(":", 1)[1].split()[0])
-    if None in (got_bytes, got_launches, traffic) or got_launches != launches:
+        if ln.startswith("Kernel sources sha256 (tree.hip, dev.hpp):"):
+            src = ln.split(":", 1)[1].strip()
+    if None in (got_bytes, got_launches, traffic, src) or got_launches != launches:
+        return None
+    if src != kernel_source_hash():
         return None
     if abs(got_bytes - bytes_per_build) > 0.02 * got_bytes:
         return None
     return traffic
+
+
 MODES = {"gll": 0, "plant": 1, "hybrid": 2}
 
 
@@ -301,6 +320,9 @@ def c4_leg(chl, t_start, budget_s, serve=True):
                         "store_GB": s["store_bytes"] / 1e9, "digest": hex(L.checksum(0)),
                         "roofline": {"bound": "hbm", "kernel": "tree_kernel", "achieved": achieved,
                                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                                     "traffic": profiled_traffic(s["relax_bytes"], s["tree_launches"],
+                                                                 TREE_PROFILE_C4),
+                                     "traffic_per": "launch", "launches_per_build": s["tree_launches"],
                                      "bytes_per_build": s["relax_bytes"], "kernel_ms_per_build": s["relax_ms"]}}
         log(f"C4 {name} build: {dt:.1f} s, {s['labels']} labels, {s['segments']} segments, "
             f"{s['store_bytes'] / 1e9:.1f} GB")
@@ -335,9 +357,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="road", choices=["road", "rmat", "rmat-w16"],
-                    help="road: 1M-vertex 3-NN road-like graph (default); rmat / rmat-w16: R-MAT scale 20, "
-                         "edge factor 3, unit weights / weights in [1, 16], degree order, hybrid mode")
+    ap.add_argument("--workload", default="road", choices=["road", "road16M", "rmat", "rmat-w16"],
+                    help="road: 1M-vertex 3-NN road-like graph (default); road16M: the 16M-vertex road-like "
+                         "graph with the 1024-root order computed on the device (the timed workload of the "
+                         "profiles at that size); rmat / rmat-w16: R-MAT scale 20, edge factor 3, unit weights / "
+                         "weights in [1, 16], degree order, hybrid mode")
     ap.add_argument("--n", type=int, default=0, help="vertices (road) or 2^scale (rmat); 0 = the workload's own")
     ap.add_argument("--mode", default=None, choices=sorted(MODES), help="default: plant (road), hybrid (rmat)")
     ap.add_argument("--static", action="store_true", help="time static rank-order batching (default: learned)")
@@ -351,6 +375,7 @@ def main():
     ap.add_argument("--no-query", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="query kernel only, no server replays")
     ap.add_argument("--no-c4", action="store_true", help="skip the 16M-vertex C4 leg")
+    ap.add_argument("--verbose", type=int, default=0, help="build option verbose (per-batch lines on stderr)")
     ap.add_argument("--budget", type=float, default=480.0,
                     help="seconds after the start past which the C4 leg starts no further build")
     args = ap.parse_args()
@@ -371,6 +396,12 @@ def main():
         order, score = g.order_betweenness(256, seed=7, with_score=True)
         desc = (f"road-like 3-NN, n={g.n}, {g.arcs(1) // 2} undirected edges, w=round(1e4*d)+1, "
                 f"256-root sampled-betweenness order")
+    elif args.workload == "road16M":
+        mode = args.mode or "plant"
+        g = chl.Graph.knn(16_000_000, 3, seed=1)
+        order, score, _ = g.order_betweenness_device(1024, seed=7, with_score=True)
+        desc = (f"road-like 3-NN, n={g.n}, {g.arcs(1) // 2} undirected edges, w=round(1e4*d)+1, "
+                f"1024-root sampled-betweenness order (device)")
     else:
         mode = args.mode or "hybrid"
         scale = int(np.log2(args.n)) if args.n else 20
@@ -385,7 +416,7 @@ def main():
         copy_calibration()
     learned = not args.static
     opts = dict(mode=MODES[mode], batching=1 if learned else 0, batch_roots=args.batch,
-                delta=args.delta, slots=args.slots, vertex_score=score.ctypes.data)
+                delta=args.delta, slots=args.slots, vertex_score=score.ctypes.data, verbose=args.verbose)
     for k in range(args.warmup):
         L = chl.build(g, order, **opts)
         log(f"warmup {k}: {L.stats.total_s:.3f} s")
@@ -405,7 +436,8 @@ def main():
     labels = int(s["labels"])
     relax_s = s["relax_ms"] / 1e3
     achieved = s["relax_bytes"] / relax_s / 1e9 if relax_s > 0 else 0.0
-    traffic = profiled_traffic(s["relax_bytes"], s["tree_launches"])
+    traffic = profiled_traffic(s["relax_bytes"], s["tree_launches"],
+                               TREE_PROFILE_C4 if args.workload == "road16M" else TREE_PROFILE)
     line = {
         "metric": "label build s + labels/s (learned vs static batching); query/s, p50/p99 us",
         "value": labels / (ms / 1e3), "unit": "labels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -420,8 +452,11 @@ def main():
                    "switch_rank": s["switch_rank"], "switch_trials": s["switch_trials"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "tree_kernel",
-                     "traffic_source": ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload, scaled by "
-                                        "a 1 GiB copy (profiles/tree_kernel_knn1M.txt)" if traffic else None),
+                     "traffic_per": "launch",
+                     "traffic_source": ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload and kernel "
+                                        "source, scaled by a 1 GiB copy (" + os.path.relpath(
+                                            TREE_PROFILE_C4 if args.workload == "road16M" else TREE_PROFILE, HERE)
+                                        + ")" if traffic else None),
                      "bytes_per_build": s["relax_bytes"], "kernel_ms_per_build": s["relax_ms"],
                      "launches_per_build": s["tree_launches"],
                      "avg_launch_ms": s["relax_ms"] / max(1, s["tree_launches"])},
@@ -449,7 +484,7 @@ def main():
     if not args.no_c4 and args.workload == "road" and not args.n:
         del L
         line["c4"] = c4_leg(chl, t_start, args.budget, serve=not args.no_serve)
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload != "road16M":
         line["cpu_baseline"] = cpu_baseline(g, order, args.cpu_seconds,
                                             whole_build_ratio=args.workload == "road" and not args.n)
     print(json.dumps(line), flush=True)
