@@ -576,173 +576,6 @@ __global__ void copy_u64_kernel(const uint64_t* src, uint64_t count, uint64_t* d
     dst[i] = src[i];
 }
 
-// ---- query ---------------------------------------------------------------------
-// One wavefront per pair. Lane s fetches the bounds of u's side-1 row and v's
-// side-0 row in segment s (one vertex-major table per side, so every segment's
-// bounds arrive in one round trip); then, per segment where both rows are
-// non-empty, u's row is staged into LDS in tiles of up to kQT entries, each lane
-// takes entries of v's row and binary-searches their hub among the staged hubs;
-// the answer is the minimum sum over common hubs, INT_MAX when there is none
-// (labeling.hpp:56-65). Every lane issues all its loads of a tile (kQPer
-// entries) and of v's chunk before it uses any of them, and its kQPer searches
-// advance in lock step (their LDS reads overlap). Segment fields come from the
-// kernel argument with a wavefront-uniform index (scalar loads).
-//
-// Chunk layout: a SEG_F8 chunk starting at entry c covers [c, c + kQT), entry
-// c + lane + 64 i in slot i of a lane. A SEG_F5 chunk covers [c & ~3, that +
-// kQT): a lane loads four 4-entry groups (16 bytes of keys and 4 bytes of low
-// distance bytes each, aligned), entry (c & ~3) + 4 lane + 256 j + t in slot
-// 4 j + t. Slots outside [c, end) hold a hub of all ones (past every hub).
-constexpr int kQWaves = 4;
-constexpr int kQT = 1024;
-constexpr int kQPer = kQT / 64;
-
-__device__ __forceinline__ uint64_t chunk_base(uint32_t fmt, uint64_t c) { return fmt == SEG_F8 ? c : (c & ~3ull); }
-__device__ __forceinline__ uint64_t slot_entry(uint32_t fmt, uint64_t base, uint32_t lane, int i) {
-  return fmt == SEG_F8 ? base + lane + 64u * i : base + 4u * lane + 256u * (i >> 2) + (i & 3);
-}
-
-template <int FMT>
-__device__ __forceinline__ void load_chunk(const QuerySide& S, uint32_t sg, uint64_t c, uint64_t end, uint32_t lane,
-                                           uint64_t (&e)[kQPer]) {
-  const uint32_t fmt = FMT ? (uint32_t)FMT : S.fmt[sg];
-  if (fmt == SEG_F8) {
-    const uint64_t* ent = S.ent[sg];
-#pragma unroll
-    for (int i = 0; i < kQPer; ++i) {
-      const uint64_t j = c + lane + 64u * i;
-      e[i] = j < end ? ent[j] : ~0ull;
-    }
-  } else {
-    const uint4* key4 = reinterpret_cast<const uint4*>(S.key[sg]);
-    const uint32_t* lo4 = reinterpret_cast<const uint32_t*>(S.lo[sg]);
-    const uint32_t h0 = S.h0[sg];
-    const uint64_t g0 = (c >> 2) + lane;
-    uint4 kk[kQPer / 4];
-    uint32_t ll[kQPer / 4];
-#pragma unroll
-    for (int j = 0; j < kQPer / 4; ++j) {
-      const uint64_t g = g0 + 64u * j;
-      // a group that starts before `end` lies within the arrays (kF5Pad spare entries)
-      if (4 * g < end) {
-        kk[j] = key4[g];
-        ll[j] = lo4[g];
-      } else {
-        kk[j] = make_uint4(0, 0, 0, 0);
-        ll[j] = 0;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < kQPer / 4; ++j) {
-      const uint64_t g = g0 + 64u * j;
-      const uint32_t x[4] = {kk[j].x, kk[j].y, kk[j].z, kk[j].w};
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const uint64_t idx = 4 * g + t;
-        e[4 * j + t] = (idx >= c && idx < end)
-                           ? (((uint64_t)(h0 + (x[t] >> 8)) << 32) | ((x[t] & 0xFFu) << 8) | ((ll[j] >> (8 * t)) & 0xFFu))
-                           : ~0ull;
-      }
-    }
-  }
-}
-
-template <int FMT>
-__global__ __launch_bounds__(256) void query_kernel(const QueryView qv, const uint32_t* pairs, uint32_t n, uint64_t q,
-                                                    int32_t* out) {
-  __shared__ uint32_t s_h[kQWaves][kQT];
-  __shared__ uint32_t s_d[kQWaves][kQT];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  const uint32_t ks = qv.nseg;
-  for (uint64_t k = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6; k < q; k += nw) {
-    const uint32_t pu = pairs[2 * k], pv = pairs[2 * k + 1];
-    if (pu >= n || pv >= n) {
-      if (lane == 0) out[k] = -1;
-      continue;
-    }
-    ulonglong2 ru = make_ulonglong2(0, 0), rv = make_ulonglong2(0, 0);
-    if (lane < ks) {
-      ru = reinterpret_cast<const ulonglong2*>(qv.s1.rng)[(uint64_t)pu * ks + lane];
-      rv = reinterpret_cast<const ulonglong2*>(qv.s0.rng)[(uint64_t)pv * ks + lane];
-    }
-    uint64_t segs = __ballot(ru.x < ru.y && rv.x < rv.y);
-    uint32_t best = kInfDist;
-    while (segs) {
-      const uint32_t sg = __builtin_amdgcn_readfirstlane((uint32_t)__ffsll((long long)segs) - 1);
-      segs &= segs - 1;
-      const uint64_t ab = __shfl(ru.x, sg), ae = __shfl(ru.y, sg), bb = __shfl(rv.x, sg), be = __shfl(rv.y, sg);
-      const uint32_t f1 = FMT ? (uint32_t)FMT : qv.s1.fmt[sg];
-      const uint32_t f0 = FMT ? (uint32_t)FMT : qv.s0.fmt[sg];
-      for (uint64_t t0 = ab; t0 < ae;) {
-        const uint64_t tb = chunk_base(f1, t0);
-        const uint64_t tend = min(ae, tb + (uint64_t)kQT);
-        const uint32_t tl = (uint32_t)(tend - t0);
-        uint64_t eu[kQPer], ev[kQPer];
-        load_chunk<FMT>(qv.s1, sg, t0, tend, lane, eu);
-        uint64_t c0 = bb;
-        load_chunk<FMT>(qv.s0, sg, c0, be, lane, ev);
-#pragma unroll
-        for (int i = 0; i < kQPer; ++i) {
-          const uint64_t idx = slot_entry(f1, tb, lane, i);
-          if (idx >= t0 && idx < tend) {
-            s_h[wave][idx - t0] = (uint32_t)(eu[i] >> 32);
-            s_d[wave][idx - t0] = (uint32_t)eu[i];
-          }
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const uint32_t lo_h = s_h[wave][0], hi_h = s_h[wave][tl - 1];
-        for (;;) {
-          // kQPer branch-free lower-bound searches in lock step: every step is one
-          // LDS read per search, all independent, so their latencies overlap
-          uint32_t base[kQPer];
-          bool past = true;
-#pragma unroll
-          for (int i = 0; i < kQPer; ++i) {
-            base[i] = 0;
-            past = past && (uint32_t)(ev[i] >> 32) > hi_h;
-          }
-          for (uint32_t len = tl; len > 1;) {
-            const uint32_t half = len >> 1;
-#pragma unroll
-            for (int i = 0; i < kQPer; ++i) {
-              const uint32_t h = (uint32_t)(ev[i] >> 32);
-              base[i] = s_h[wave][base[i] + half] < h ? base[i] + half : base[i];
-            }
-            len -= half;
-          }
-#pragma unroll
-          for (int i = 0; i < kQPer; ++i) {
-            const uint32_t h = (uint32_t)(ev[i] >> 32);
-            // base = last index with hub < h (or 0); the match, if any, is there or next
-            const uint32_t at = s_h[wave][base[i]] < h ? base[i] + 1 : base[i];
-            if (h >= lo_h && h <= hi_h && at < tl && s_h[wave][at] == h) {
-              const uint32_t sum = s_d[wave][at] + (uint32_t)ev[i];
-              best = sum < best ? sum : best;
-            }
-          }
-          // the side-0 row is hub-ascending: stop once every lane is past this tile
-          c0 = chunk_base(f0, c0) + kQT;
-          if (__all(past) || c0 >= be) break;
-          load_chunk<FMT>(qv.s0, sg, c0, be, lane, ev);
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        t0 = tend;
-      }
-    }
-    // wave minimum
-    for (int o = 32; o > 0; o >>= 1) {
-      const uint32_t other = __shfl_xor(best, o);
-      best = other < best ? other : best;
-    }
-    if (lane == 0) out[k] = (int32_t)best;
-  }
-}
-
 }  // namespace
 
 // ---- launches ------------------------------------------------------------------
@@ -861,16 +694,6 @@ hipError_t launch_window(const SegView* views, uint32_t k, const uint32_t* row_o
 hipError_t launch_decode(const SegView& seg, uint64_t count, uint64_t* out, hipStream_t s) {
   if (!count) return hipSuccess;
   hipLaunchKernelGGL(decode_kernel, dim3(grid_for(count, 256, 16384)), dim3(256), 0, s, seg, count, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_query(const QueryView& v, const uint32_t* pairs, uint32_t n, uint64_t q, int32_t* out,
-                        hipStream_t s) {
-  if (!q) return hipSuccess;
-  const dim3 grid(grid_for(q * 64, 256, 8192)), block(256);
-  if (v.fmts == SEG_F5) hipLaunchKernelGGL(query_kernel<SEG_F5>, grid, block, 0, s, v, pairs, n, q, out);
-  else if (v.fmts == SEG_F8) hipLaunchKernelGGL(query_kernel<SEG_F8>, grid, block, 0, s, v, pairs, n, q, out);
-  else hipLaunchKernelGGL(query_kernel<0>, grid, block, 0, s, v, pairs, n, q, out);
   return hipGetLastError();
 }
 

@@ -119,6 +119,47 @@ T __shfl_xor(T x, int m) {
   emu_wait(emu_blk->waves[w]);
   return r;
 }
+// Wavefront exchange: every lane of the wavefront publishes a value, then reads.
+inline int64_t* emu_wave_publish(int64_t x) {
+  const unsigned w = threadIdx.x / 64, l = threadIdx.x % 64;
+  int64_t* v = &emu_blk->wave_val[64 * (size_t)w];
+  v[l] = x;
+  emu_wait(emu_blk->waves[w]);
+  return v;
+}
+inline void emu_wave_done() { emu_wait(emu_blk->waves[threadIdx.x / 64]); }
+inline unsigned emu_wave_lanes() {
+  const unsigned w = threadIdx.x / 64;
+  return 64 * w + 64 <= blockDim.x ? 64u : blockDim.x - 64 * w;
+}
+inline uint64_t __ballot(int pred) {
+  const int64_t* v = emu_wave_publish(pred != 0);
+  uint64_t r = 0;
+  for (unsigned i = 0; i < emu_wave_lanes(); ++i) r |= (uint64_t)(v[i] != 0) << i;
+  emu_wave_done();
+  return r;
+}
+inline int __all(int pred) {
+  const int64_t* v = emu_wave_publish(pred != 0);
+  int r = 1;
+  for (unsigned i = 0; i < emu_wave_lanes(); ++i) r &= (int)(v[i] != 0);
+  emu_wave_done();
+  return r;
+}
+template <class T>
+T __shfl(T x, int src) {
+  const int64_t* v = emu_wave_publish((int64_t)x);
+  T r = (T)v[(unsigned)src % 64];
+  emu_wave_done();
+  return r;
+}
+// every lane of the wavefront is active where the kernels call these
+#define __builtin_amdgcn_readfirstlane(x) __shfl((x), 0)
+#define __builtin_amdgcn_wave_barrier() emu_wave_done()
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
+inline int __ffsll(long long x) { return __builtin_ffsll(x); }
+struct uint4 { uint32_t x, y, z, w; };
+inline uint4 make_uint4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) { return uint4{a, b, c, d}; }
 inline int __clz(int x) { return x ? __builtin_clz((unsigned)x) : 32; }
 template <class T> T min(T a, T b) { return a < b ? a : b; }
 template <class T> T max(T a, T b) { return a < b ? b : a; }
