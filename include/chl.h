@@ -172,7 +172,10 @@ typedef struct chl_build_stats {
   double segment_ms;            /* device time building, cleaning and merging label segments */
   double store_bytes;           /* device bytes of the finished label store */
   int64_t segments;             /* segments per side of the finished store */
-  double reserved[4];
+  int64_t slot_shrinks;         /* memory-limited builds: times search slots went back to the device */
+  int64_t slots_final;          /* search slots at the last batch (slots_used: at the first) */
+  double slot_s;                /* wall seconds spent giving search slots back (memory-limited builds) */
+  double reserved[1];
 } chl_build_stats;
 
 void chl_build_opts_default(chl_build_opts* o);

@@ -69,7 +69,8 @@ class BuildStats(C.Structure):
                 ("clean_bytes", C.c_double), ("clean_kernel_ms", C.c_double),
                 ("settled_distinct", C.c_int64), ("settled_arcs", C.c_int64), ("relax_bytes_processed", C.c_double),
                 ("segment_ms", C.c_double), ("store_bytes", C.c_double), ("segments", C.c_int64),
-                ("reserved", C.c_double * 4)]
+                ("slot_shrinks", C.c_int64), ("slots_final", C.c_int64), ("slot_s", C.c_double),
+                ("reserved", C.c_double * 1)]
 
     def as_dict(self):
         return {k: (getattr(self, k) if k != "reserved" else None) for k, _ in self._fields_ if k != "reserved"}

@@ -31,39 +31,47 @@ void dev_free(void* p, size_t bytes);
 void dev_cache_flush();
 size_t dev_cached_bytes();  // bytes of freed blocks the allocator keeps for the current device
 
-// Device array that grows in place: a virtual range as large as the device
-// memory is reserved once and physical memory is mapped behind it as the array
-// grows (HIP virtual memory management), so growing never copies and never
-// needs the old and the new array at the same time. Where the runtime refuses
-// virtual memory the array falls back to allocate-copy-free.
-struct GrowArray {
+// Device array whose trailing part can go back to the device without moving
+// what is kept: a virtual range is reserved and backed by physical chunks of a
+// fixed size (HIP virtual memory management); shrink unmaps and releases the
+// chunks wholly past the kept length. The search slots of a memory-limited
+// build live in these, so that slots go back as the label store grows without
+// freeing, allocating and clearing the slots that stay. Where the runtime
+// refuses virtual memory the array is one plain allocation and shrink keeps it.
+struct ChunkArray {
   char* base = nullptr;
-  size_t reserved = 0;  // bytes of the virtual range (virtual memory only)
+  size_t reserved = 0;  // bytes of the virtual range
   size_t mapped = 0;    // usable bytes
+  size_t chunk = 0;     // bytes per physical chunk
   int device = 0;
   bool vmm = false;
-  std::vector<std::pair<size_t, hipMemGenericAllocationHandle_t>> maps;  // (size, handle) in order
-  GrowArray() = default;
-  GrowArray(const GrowArray&) = delete;
-  GrowArray& operator=(const GrowArray&) = delete;
-  ~GrowArray() { release(); }
-  // At least `bytes` usable; the first keep_bytes are preserved.
-  hipError_t grow(size_t bytes, size_t keep_bytes, hipStream_t s);
+  std::vector<hipMemGenericAllocationHandle_t> maps;  // chunk handles in address order
+  ChunkArray() = default;
+  ChunkArray(const ChunkArray&) = delete;
+  ChunkArray& operator=(const ChunkArray&) = delete;
+  ~ChunkArray() { release(); }
+  // A new array of at least `bytes` (contents undefined) in chunks of about
+  // chunk_bytes.
+  hipError_t alloc(size_t bytes, size_t chunk_bytes);
+  // Keeps the first `bytes` (rounded up to whole chunks); returns the rest.
+  hipError_t shrink(size_t bytes);
   void release();
-
- private:
-  hipError_t vmm_map(size_t bytes);
 };
 
 template <class T>
-struct GrowBuf {
-  GrowArray a;
+struct ChunkBuf {
+  ChunkArray a;
   T* p = nullptr;
   size_t n = 0;  // elements usable
-  hipError_t reserve(size_t count, size_t keep = 0, hipStream_t s = nullptr) {
-    if (count <= n && p) return hipSuccess;
-    hipError_t e = a.grow((count ? count : 1) * sizeof(T), keep * sizeof(T), s);
+  hipError_t alloc(size_t count, size_t chunk_bytes) {
+    release();
+    hipError_t e = a.alloc((count ? count : 1) * sizeof(T), chunk_bytes);
     p = reinterpret_cast<T*>(a.base);
+    n = a.mapped / sizeof(T);
+    return e;
+  }
+  hipError_t shrink(size_t count) {
+    hipError_t e = a.shrink(count * sizeof(T));
     n = a.mapped / sizeof(T);
     return e;
   }
@@ -108,6 +116,30 @@ struct DBuf {
     T* tp = p; p = o.p; o.p = tp;
     size_t tn = n; n = o.n; o.n = tn;
     size_t tb = bytes; bytes = o.bytes; o.bytes = tb;
+  }
+};
+
+// Search-slot array: from the allocation cache (repeated builds reuse it), or,
+// in a memory-limited build, a ChunkBuf whose trailing slots can go back.
+template <class T>
+struct SlotBuf {
+  DBuf<T> d;
+  ChunkBuf<T> c;
+  T* p = nullptr;
+  bool chunked = false;
+  hipError_t alloc(size_t count, bool in_chunks, size_t chunk_bytes) {
+    release();
+    chunked = in_chunks;
+    hipError_t e = chunked ? c.alloc(count, chunk_bytes) : d.reserve(count);
+    p = chunked ? c.p : d.p;
+    return e;
+  }
+  // Keeps the first `count` elements; returns whole chunks past them.
+  hipError_t shrink(size_t count) { return chunked ? c.shrink(count) : hipSuccess; }
+  void release() {
+    d.release();
+    c.release();
+    p = nullptr;
   }
 };
 

@@ -87,7 +87,7 @@ void dev_cache_flush() {
   g_cached = 0;
 }
 
-// ---- GrowArray ----------------------------------------------------------------
+// ---- ChunkArray ---------------------------------------------------------------
 
 namespace {
 hipMemAllocationProp device_prop(int dev) {
@@ -100,108 +100,87 @@ hipMemAllocationProp device_prop(int dev) {
 size_t round_up(size_t x, size_t g) { return (x + g - 1) / g * g; }
 }  // namespace
 
-hipError_t GrowArray::grow(size_t bytes, size_t keep_bytes, hipStream_t s) {
-  if (base && bytes <= mapped) return hipSuccess;
-  if (!base) {
-    (void)hipGetDevice(&device);
-    // a virtual range as large as the device, mapped as needed
-    hipMemAllocationProp prop = device_prop(device);
-    size_t gran = 0, f = 0, t = 0;
-    if (hipMemGetInfo(&f, &t) == hipSuccess &&
-        hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityRecommended) == hipSuccess &&
-        gran) {
-      const size_t res = round_up(t, std::max<size_t>(gran, (size_t)2 << 20));
-      void* ptr = nullptr;
-      if (hipMemAddressReserve(&ptr, res, 0, nullptr, 0) == hipSuccess && ptr) {
-        base = (char*)ptr;
-        reserved = res;
-        vmm = true;
-      }
-    }
-    (void)hipGetLastError();
-  }
-  if (vmm) {
-    hipError_t e = vmm_map(bytes);
-    if (e == hipSuccess) return hipSuccess;
-    (void)hipGetLastError();
-    // the runtime refused: continue as a plain allocation (the kept prefix is
-    // copied out of the mapped range below, then the range is released)
-  }
-  // fallback: a larger allocation, the kept prefix copied over
-  size_t want = std::max(bytes, mapped + mapped / 4);
-  void* nb = nullptr;
-  hipError_t e = hipMalloc(&nb, want);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    dev_cache_flush();
-    want = bytes;
-    e = hipMalloc(&nb, want);
-    if (e != hipSuccess) return e;
-  }
-  if (base && keep_bytes) {
-    e = hipMemcpyAsync(nb, base, std::min(keep_bytes, mapped), hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) {
-      (void)hipFree(nb);
-      return e;
-    }
-  }
+hipError_t ChunkArray::alloc(size_t bytes, size_t chunk_bytes) {
   release();
-  base = (char*)nb;
-  mapped = want;
-  return hipSuccess;
-}
-
-// Maps physical memory behind the range up to at least `bytes`; access is set
-// on the whole mapped range after each new mapping.
-hipError_t GrowArray::vmm_map(size_t bytes) {
+  (void)hipGetDevice(&device);
   hipMemAllocationProp prop = device_prop(device);
   size_t gran = 0;
   hipError_t e = hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityRecommended);
-  if (e != hipSuccess || !gran) return e != hipSuccess ? e : hipErrorInvalidValue;
-  gran = std::max<size_t>(gran, (size_t)2 << 20);
-  // at least a quarter of what is mapped (few mappings) and 32 MB
-  const size_t need = bytes - mapped;
-  size_t step = round_up(std::max(need, std::max(mapped / 4, (size_t)32 << 20)), gran);
-  if (mapped + step > reserved) step = round_up(need, gran);
-  if (mapped + step > reserved) return hipErrorOutOfMemory;
-  hipMemGenericAllocationHandle_t h{};
-  e = hipMemCreate(&h, step, &prop, 0);
+  if (e == hipSuccess && gran) {
+    chunk = round_up(std::max<size_t>(chunk_bytes, 1), std::max<size_t>(gran, (size_t)2 << 20));
+    const size_t total = round_up(std::max<size_t>(bytes, 1), chunk);
+    void* ptr = nullptr;
+    e = hipMemAddressReserve(&ptr, total, 0, nullptr, 0);
+    if (e == hipSuccess && ptr) {
+      base = (char*)ptr;
+      reserved = total;
+      vmm = true;
+      for (size_t off = 0; off < total && e == hipSuccess; off += chunk) {
+        hipMemGenericAllocationHandle_t h{};
+        e = hipMemCreate(&h, chunk, &prop, 0);
+        if (e != hipSuccess) {
+          (void)hipGetLastError();
+          dev_cache_flush();
+          e = hipMemCreate(&h, chunk, &prop, 0);
+          if (e != hipSuccess) break;
+        }
+        e = hipMemMap(base + off, chunk, 0, h, 0);
+        if (e != hipSuccess) {
+          (void)hipMemRelease(h);
+          break;
+        }
+        maps.push_back(h);
+        mapped = off + chunk;
+      }
+      if (e == hipSuccess) {
+        hipMemAccessDesc desc = {};
+        desc.location = prop.location;
+        desc.flags = hipMemAccessFlagsProtReadWrite;
+        e = hipMemSetAccess(base, mapped, &desc, 1);
+      }
+      if (e == hipSuccess) return hipSuccess;
+      (void)hipGetLastError();
+      release();
+    }
+    (void)hipGetLastError();
+  }
+  // plain allocation (shrink keeps it whole)
+  void* p = nullptr;
+  e = hipMalloc(&p, bytes ? bytes : 1);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     dev_cache_flush();
-    step = round_up(need, gran);
-    e = hipMemCreate(&h, step, &prop, 0);
+    e = hipMalloc(&p, bytes ? bytes : 1);
     if (e != hipSuccess) return e;
   }
-  e = hipMemMap(base + mapped, step, 0, h, 0);
-  if (e != hipSuccess) {
-    (void)hipMemRelease(h);
-    return e;
-  }
-  hipMemAccessDesc desc = {};
-  desc.location = prop.location;
-  desc.flags = hipMemAccessFlagsProtReadWrite;
-  e = hipMemSetAccess(base, mapped + step, &desc, 1);
-  if (e != hipSuccess) {
-    (void)hipMemUnmap(base + mapped, step);
-    (void)hipMemRelease(h);
-    return e;
-  }
-  maps.emplace_back(step, h);
-  mapped += step;
+  base = (char*)p;
+  mapped = bytes ? bytes : 1;
+  vmm = false;
   return hipSuccess;
 }
 
-void GrowArray::release() {
+hipError_t ChunkArray::shrink(size_t bytes) {
+  if (!vmm || !base) return hipSuccess;
+  const size_t keep = std::max<size_t>(1, (bytes + chunk - 1) / chunk);
+  while (maps.size() > keep) {
+    const size_t off = (maps.size() - 1) * chunk;
+    hipError_t e = hipMemUnmap(base + off, chunk);
+    if (e != hipSuccess) return e;
+    e = hipMemRelease(maps.back());
+    if (e != hipSuccess) return e;
+    maps.pop_back();
+    mapped = off;
+  }
+  return hipSuccess;
+}
+
+void ChunkArray::release() {
   if (!base) return;
   if (vmm) {
     (void)hipDeviceSynchronize();
-    size_t off = 0;
-    for (auto& m : maps) {
-      (void)hipMemUnmap(base + off, m.first);
-      (void)hipMemRelease(m.second);
-      off += m.first;
+    for (size_t k = 0; k < maps.size(); ++k) {
+      (void)hipMemUnmap(base + k * chunk, chunk);
+      (void)hipMemRelease(maps[k]);
     }
     (void)hipMemAddressFree(base, reserved);
   } else {
@@ -209,7 +188,7 @@ void GrowArray::release() {
   }
   maps.clear();
   base = nullptr;
-  reserved = mapped = 0;
+  reserved = mapped = chunk = 0;
   vmm = false;
 }
 

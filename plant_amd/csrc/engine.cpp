@@ -204,9 +204,10 @@ struct Builder {
   int nsides;
   hipStream_t stream = nullptr;
 
-  DBuf<uint64_t> cell;
-  DBuf<uint32_t> qa, qb, vis, gkey, gval, grank, seq;
-  DBuf<uint64_t> qd;  // deferred piles
+  SlotBuf<uint64_t> cell;
+  SlotBuf<uint32_t> qa, qb, vis;
+  DBuf<uint32_t> gkey, gval, grank, seq;
+  SlotBuf<uint64_t> qd;  // deferred piles
   std::vector<uint32_t> id_of_rank;  // rank -> device id
   std::vector<uint32_t> hids;        // staging of root ids
   uint32_t slots = 0, gcap = 0;
@@ -266,25 +267,75 @@ struct Builder {
     return CHL_OK;
   }
 
+  // Search slots. Each holds 36 bytes per vertex (16-byte cell, three u32
+  // lists and a u64 pile). On a graph where the wanted count does not fit, the
+  // slots take the free memory beyond a reserve of a quarter of the device (the
+  // label store is still small when the largest trees run, at the top ranks),
+  // in arrays whose trailing slots can go back to the device (ChunkBuf), and
+  // fit_slots gives slots back as the store grows.
+  static constexpr size_t kSlotBytesPerVertex = 36;
+  uint32_t slots_wanted = 0;
+
   int alloc_slots(uint32_t want) {
     size_t freeb = 0, totalb = 0;
     CHL_HIP(hipMemGetInfo(&freeb, &totalb));
     freeb += dev_cached_bytes();  // blocks the allocator keeps are free to reuse
-    const size_t per = (size_t)n * 36;  // 16-byte cell, three u32 lists and a u64 pile per vertex
-    // a third of the free memory, at most a quarter of the device: on large
-    // graphs the label store (twice, while a batch is merged) needs the rest
-    size_t budget = std::min(freeb / 3, totalb / 4);
+    const size_t per = (size_t)n * kSlotBytesPerVertex;
+    const size_t reserve = totalb / 4;
+    const size_t budget = freeb > reserve + totalb / 8 ? freeb - reserve : freeb / 3;
     uint32_t fit = (uint32_t)std::max<size_t>(1, budget / std::max<size_t>(per, 1));
+    slots_wanted = want;
     slots = std::max<uint32_t>(1, std::min(want, fit));
+    const bool limited = slots < want;
+    if (limited) dev_cache_flush();  // the chunks come from the device, not the cache
     const size_t cells = (size_t)slots * n;
-    CHL_HIP(cell.reserve(2 * cells));
-    CHL_HIP(qa.reserve(cells));
-    CHL_HIP(qb.reserve(cells));
-    CHL_HIP(vis.reserve(cells));
-    CHL_HIP(qd.reserve(cells));
+    // chunks of whole slots, about 512 MB each
+    auto chunk = [&](size_t slot_bytes) {
+      return slot_bytes * std::max<size_t>(1, ((size_t)512 << 20) / std::max<size_t>(slot_bytes, 1));
+    };
+    CHL_HIP(cell.alloc(2 * cells, limited, chunk(16 * (size_t)n)));
+    CHL_HIP(qa.alloc(cells, limited, chunk(4 * (size_t)n)));
+    CHL_HIP(qb.alloc(cells, limited, chunk(4 * (size_t)n)));
+    CHL_HIP(vis.alloc(cells, limited, chunk(4 * (size_t)n)));
+    CHL_HIP(qd.alloc(cells, limited, chunk(8 * (size_t)n)));
     CHL_HIP(seq.reserve(2 * (size_t)slots));
     CHL_HIP(launch_init_cells(cell.p, cells, grank.p, (uint32_t)n, stream));
     CHL_HIP(launch_fill_u32(seq.p, 0, 2 * (size_t)slots, stream));
+    return CHL_OK;
+  }
+
+  // Before a batch of a memory-limited build: when the free memory has fallen
+  // below a fifth of the device (the store grows behind the batches), slots go
+  // back until a quarter is free again, keeping at least kMinSlots. The slots
+  // that stay keep their memory and contents (cells, counts).
+  static constexpr uint32_t kMinSlots = 32;
+  int fit_slots() {
+    if (slots >= slots_wanted || slots <= kMinSlots || !cell.chunked) return CHL_OK;
+    size_t freeb = 0, totalb = 0;
+    CHL_HIP(hipMemGetInfo(&freeb, &totalb));
+    freeb += dev_cached_bytes();
+    if (freeb >= totalb / 5) return CHL_OK;
+    const size_t per = (size_t)n * kSlotBytesPerVertex;
+    const size_t give = (totalb / 4 - freeb + per - 1) / per;
+    const uint32_t keep = (uint32_t)std::max<int64_t>(kMinSlots, (int64_t)slots - (int64_t)give);
+    if (keep >= slots) return CHL_OK;
+    const double t0 = now_s();
+    CHL_HIP(hipStreamSynchronize(stream));
+    const size_t cells = (size_t)keep * n;
+    CHL_HIP(cell.shrink(2 * cells));
+    CHL_HIP(qa.shrink(cells));
+    CHL_HIP(qb.shrink(cells));
+    CHL_HIP(vis.shrink(cells));
+    CHL_HIP(qd.shrink(cells));
+    if (o.verbose) fprintf(stderr, "slots: %u -> %u (%.1f GB free before)\n", slots, keep, freeb / 1e9);
+    slots = keep;
+    active = std::min(active, slots);
+    if (hooks) hooks->slots = slots;
+    if (stats) {
+      stats->slot_shrinks++;
+      stats->slot_s += now_s() - t0;
+    }
+    if (o.verbose) fprintf(stderr, "slots: resized in %.3f s\n", now_s() - t0);
     return CHL_OK;
   }
 
@@ -783,13 +834,17 @@ struct Builder {
       }
     }
     for (int64_t r0 = 0; r0 < n;) {
+      // memory for the next batch: oversized batch buffers first, then (a
+      // memory-limited build) search slots go back to the device
+      rc = trim_batch_buffers(next_need);
+      if (rc) return rc;
+      rc = fit_slots();
+      if (rc) return rc;
       uint32_t bw = width;
       if (hooks && hooks->next_width) {
         const uint32_t lw = hooks->next_width(hooks->ctx, r0, width, W);
         if (lw) bw = std::min<uint32_t>(lw, W);
       }
-      rc = trim_batch_buffers(next_need);
-      if (rc) return rc;
       {
         // memory cap under either policy: a batch's labels (output 12 bytes,
         // new segment up to 8, cleaning flag and compacted copy 9: kBatchBytes
@@ -908,9 +963,12 @@ struct Builder {
       }
       if (o.verbose) {
         uint64_t emitted = sb[0].kept + (nsides == 2 ? sb[1].kept : 0);
-        fprintf(stderr, "batch %lld: ranks [%lld, %lld) flags %u labels %llu store %llu (%u segments, %.2f GB)\n",
+        fprintf(stderr,
+                "batch %lld: ranks [%lld, %lld) flags %u labels %llu store %llu (%u segments, %.2f GB) lt %.2f st "
+                "%.2f ct %.2f s wall %.2f s\n",
                 (long long)stats_batches, (long long)r0, (long long)(r0 + R), flags, (unsigned long long)emitted,
-                (unsigned long long)L.total[0], L.nseg(), L.bytes() / 1e9);
+                (unsigned long long)L.total[0], L.nseg(), L.bytes() / 1e9, lt / 1e3, stt / 1e3, ct / 1e3,
+                now_s() - t0);
       }
       ++stats_batches;
       {
@@ -951,9 +1009,10 @@ struct Builder {
       size_t freeb = 0, totalb = 0;
       CHL_HIP(hipMemGetInfo(&freeb, &totalb));
       if (!o.keep_segments && L.nseg() > 1 && (double)L.bytes() * 1.2 > (double)freeb) {
-        for (auto* b : {&cell}) b->release();
-        for (auto* b : {&qa, &qb, &vis, &gkey, &gval, &seq}) b->release();
+        cell.release();
         qd.release();
+        for (auto* b : {&qa, &qb, &vis}) b->release();
+        for (auto* b : {&gkey, &gval, &seq}) b->release();
         for (auto& b : sb) {
           b.row.release();
           b.hd.release();
@@ -974,6 +1033,7 @@ struct Builder {
     const double t1 = now_s();
     if (stats) {
       stats->segments = L.nseg();
+      stats->slots_final = active;
       stats->store_bytes = (double)L.bytes();
       stats->lt = lt / 1e3;
       stats->st = stt / 1e3;
