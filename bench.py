@@ -316,7 +316,8 @@ def c4_leg(chl, t_start, budget_s, serve=True):
         achieved = s["relax_bytes"] / relax_s / 1e9 if relax_s > 0 else 0.0
         builds[name] = {"build_s": dt, "labels": int(s["labels"]), "labels_per_s": s["labels"] / dt,
                         "avg_label": s["avg_label"], "batches": s["batches"], "slots": s["slots_used"],
-                        "slots_final": s["slots_final"], "slot_shrinks": s["slot_shrinks"],
+                        "slots_final": s["slots_final"], "slot_shrinks": s["slot_shrinks"], "slot_s": s["slot_s"],
+                        "small_tree_roots": s["small_roots"],
                         "lt_s": s["lt"], "st_s": s["st"], "ct_s": s["ct"], "segments": s["segments"],
                         "store_GB": s["store_bytes"] / 1e9, "digest": hex(L.checksum(0)),
                         "roofline": {"bound": "hbm", "kernel": "tree_kernel", "achieved": achieved,
@@ -454,7 +455,8 @@ def main():
                    "slots": s["slots_used"], "lt_s": s["lt"], "st_s": s["st"], "ct_s": s["ct"],
                    "mlp_ms": s["mlp_ms"], "pred_mae_learned": s["pred_mae_learned"],
                    "pred_mae_static": s["pred_mae_static"], "pred_roots": s["pred_roots"],
-                   "switch_rank": s["switch_rank"], "switch_trials": s["switch_trials"]},
+                   "switch_rank": s["switch_rank"], "switch_trials": s["switch_trials"],
+                   "small_tree_roots": s["small_roots"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "tree_kernel",
                      "traffic_per": "launch",

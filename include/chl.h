@@ -134,7 +134,10 @@ typedef struct chl_build_opts {
                                 feature of the learned batching predictor; NULL = not used */
   int keep_segments;    /* 1: keep the label store's segments as the build left them (no final merge);
                            0: merge them into one where device memory allows */
-  int reserved[7];
+  int small_trees;      /* learned batching, PLaNT batches: 1 (default) runs the roots predicted to grow
+                           small trees one wavefront each with a hash-table search state, the others
+                           with the tree kernel; 0: the tree kernel only. Labels are the same. */
+  int reserved[6];
 } chl_build_opts;
 
 typedef struct chl_build_stats {
@@ -175,7 +178,7 @@ typedef struct chl_build_stats {
   int64_t slot_shrinks;         /* memory-limited builds: times search slots went back to the device */
   int64_t slots_final;          /* search slots at the last batch (slots_used: at the first) */
   double slot_s;                /* wall seconds spent giving search slots back (memory-limited builds) */
-  double reserved[1];
+  int64_t small_roots;          /* roots whose trees the small-tree kernel completed */
 } chl_build_stats;
 
 void chl_build_opts_default(chl_build_opts* o);

@@ -275,11 +275,27 @@ struct TreeParams {
   uint32_t* label_count;
   uint32_t* touched;  // optional: vertices reached per root
   unsigned long long* ctr;  // CTR_* (CTR_NEXT is the work-queue cursor)
+  // small-tree searches (small.hip), per small slot: a hash table of scap
+  // entries (state word, vertex + 1 with 0 for empty, stamp) and three lists of
+  // scap table indices (two frontiers, the inserted entries); big[ri] = 1 marks
+  // a root predicted too large, which goes to spill[] unstarted
+  uint64_t* sstate;
+  uint32_t* skey;
+  uint32_t* sstamp;
+  uint32_t* slist;
+  uint32_t scap;
+  const uint8_t* big;
 };
 
 // One workgroup per slot of threads_per_root threads: 64 (a wavefront per root),
 // 256 or 1024.
 hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, int threads_per_root, hipStream_t s);
+// PLaNT trees of the batch's roots, one wavefront per root over `slots` small
+// slots (small.hip); a root marked big, or whose tree outgrows half its table,
+// goes to spill[] (counted in CTR_SKIPPED) for a launch of tree_kernel.
+hipError_t launch_small_trees(const TreeParams& p, uint32_t slots, hipStream_t s);
+// scap-entry tables of `slots` small slots set to empty
+hipError_t launch_init_small(const TreeParams& p, uint32_t slots, hipStream_t s);
 // count cells set to the unreached state (state all ones, stamp 0) with the
 // rank of their vertex (cell i: device id i mod n); the slots' seq words must be
 // zeroed with them

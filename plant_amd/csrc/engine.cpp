@@ -185,6 +185,15 @@ constexpr uint32_t kDefaultSlots = 2048;
 // granularity gives large trees a workgroup of 1024 threads.
 constexpr uint32_t kWideBelow = 1024;
 
+// Small-tree slots (small.hip): 4096 wavefronts, each with a hash table of
+// 32768 entries (28 bytes each: 3.8 GB in all). A root goes there when its
+// predicted tree processes at most kSmallRoute vertices, an eighth of the
+// entries a tree may insert before it is handed to the tree kernel (the
+// predictions' mean error is a factor of about 2.4).
+constexpr uint32_t kSmallSlots = 4096;
+constexpr uint32_t kSmallCap = 32768;
+constexpr uint32_t kSmallRoute = kSmallCap / 16;
+
 // Search slots for a granularity: base_slots workgroups of 256 threads, as
 // many threads in flight at 64 or 1024 threads per root.
 static uint32_t slots_for(uint32_t base_slots, uint32_t tpr) {
@@ -208,6 +217,12 @@ struct Builder {
   SlotBuf<uint32_t> qa, qb, vis;
   DBuf<uint32_t> gkey, gval, grank, seq;
   SlotBuf<uint64_t> qd;  // deferred piles
+  // small-tree slots (small.hip): tables of kSmallCap entries, lists of 3 kSmallCap
+  DBuf<uint64_t> sstate;
+  DBuf<uint32_t> skey, sstamp, slist;
+  DBuf<uint8_t> bigd;
+  std::vector<uint8_t> bigh;
+  uint32_t sslots = 0;
   std::vector<uint32_t> id_of_rank;  // rank -> device id
   std::vector<uint32_t> hids;        // staging of root ids
   uint32_t slots = 0, gcap = 0;
@@ -349,6 +364,24 @@ struct Builder {
     return CHL_OK;
   }
 
+  int ensure_small() {
+    if (sslots) return CHL_OK;
+    const size_t cells = (size_t)kSmallSlots * kSmallCap;
+    CHL_HIP(sstate.reserve(cells));
+    CHL_HIP(skey.reserve(cells));
+    CHL_HIP(sstamp.reserve(cells));
+    CHL_HIP(slist.reserve(3 * cells));
+    TreeParams p;
+    memset(&p, 0, sizeof(p));
+    p.sstate = sstate.p;
+    p.skey = skey.p;
+    p.sstamp = sstamp.p;
+    p.scap = kSmallCap;
+    CHL_HIP(launch_init_small(p, kSmallSlots, stream));
+    sslots = kSmallSlots;
+    return CHL_OK;
+  }
+
   int ensure_out(SideBatch& b, uint64_t cap) {
     CHL_HIP(b.row.reserve(cap));
     CHL_HIP(b.hd.reserve(cap));
@@ -402,8 +435,11 @@ struct Builder {
   // b.m = labels written. Roots whose labels did not fit, and the roots not
   // started once the buffer was full, run in a further launch after the buffer
   // has grown by what the former reported and an estimate for the latter.
+  // small_pass: the first launch runs the small-tree kernel over the batch, with
+  // bigd[k] = 1 for roots it leaves to the tree kernel (they, and trees that
+  // outgrow a small slot, go to the spill list like roots not started).
   int run_trees(int side, uint32_t R, const uint32_t* hroots, uint32_t prefix, uint32_t flags, LabelStore& L,
-                double& ms) {
+                double& ms, bool small_pass = false) {
     SideBatch& b = sb[side];
     const int range = side == 0 ? 1 : 0;  // forward tree walks out-arcs and fills side 0
     int rc = ensure_out(b, out_cap);
@@ -438,6 +474,14 @@ struct Builder {
     p.touched = o.verbose ? touched.p : nullptr;
     p.label_count = label_count.p;
     p.ctr = ctr.p;
+    if (small_pass) {
+      p.sstate = sstate.p;
+      p.skey = skey.p;
+      p.sstamp = sstamp.p;
+      p.slist = slist.p;
+      p.scap = kSmallCap;
+      p.big = bigd.p;
+    }
     CHL_HIP(launch_fill_u64((uint64_t*)ctr.p, 0, CTR_COUNT, stream));
     CHL_HIP(launch_fill_u64((uint64_t*)ctr.p + CTR_SPILL_START, ~0ull, 1, stream));
     std::vector<uint32_t> cur(hroots, hroots + R), idx, next, orig(R), next_orig;
@@ -446,7 +490,8 @@ struct Builder {
     uint64_t lab0 = 0;  // label counter at the start of the pass
     for (int pass = 0;; ++pass) {
       CHL_HIP(hipEventRecord(ev[0], stream));
-      CHL_HIP(launch_tree(p, flags, std::min<uint32_t>(active, p.R), (int)tpr, stream));
+      if (pass == 0 && small_pass) CHL_HIP(launch_small_trees(p, std::min<uint32_t>(sslots, p.R), stream));
+      else CHL_HIP(launch_tree(p, flags, std::min<uint32_t>(active, p.R), (int)tpr, stream));
       CHL_HIP(hipEventRecord(ev[1], stream));
       if (pass > 0) {
         // this pass's statistics to the roots' positions in the batch
@@ -465,6 +510,7 @@ struct Builder {
       }
       if (o.verbose)
         fprintf(stderr, "  side %d pass %d: %u roots, %.3f ms, written %llu, spilled %llu (%llu labels, %llu not started), processed %llu arcs %llu rounds %llu\n", side, pass, p.R, t, hc[CTR_LABELS], hc[CTR_OVERFLOW], hc[CTR_SPILL], hc[CTR_SKIPPED], hc[CTR_PROCESSED], hc[CTR_ARCS], hc[CTR_ROUNDS]);
+      if (pass == 0 && small_pass && stats) stats->small_roots += (int64_t)(p.R - hc[CTR_OVERFLOW]);
       if (!hc[CTR_OVERFLOW]) break;
       // next launch for the roots that did not fit or were not started
       const uint32_t ns = (uint32_t)hc[CTR_OVERFLOW];
@@ -479,7 +525,9 @@ struct Builder {
       }
       cur.swap(next);
       orig.swap(next_orig);
-      const uint64_t written = hc[CTR_SPILL_START];  // contiguous prefix of fitting roots
+      // contiguous prefix of fitting roots (all labels written when only roots
+      // not started, such as the small-tree kernel's hand-overs, are left)
+      const uint64_t written = std::min<uint64_t>(hc[CTR_SPILL_START], hc[CTR_LABELS]);
       // labels still to write: exact for the roots that ran, and for the roots
       // not started 1.5 times the mean of the roots that ran in this pass
       // (bounded by a 48th of the free memory: key and entry now, their sorted
@@ -800,6 +848,7 @@ struct Builder {
     if (hooks) hooks->slots = slots;
     rc = L.init(n, sym, o.device);
     if (rc) return rc;
+    L.verbose = o.verbose;
     if (stats) {
       stats->switch_rank = -1;
       stats->delta_used = o.delta;
@@ -912,8 +961,26 @@ struct Builder {
         rc = ensure_gtab(mr);
         if (rc) return rc;
       }
+      // learned batching, PLaNT batches of a memory-limited build (too few search
+      // slots for the many small trees of the lower ranks): roots predicted to
+      // grow small trees run on the small-tree kernel first. With the full slot
+      // count the tree kernel is the faster of the two for small trees too (1M
+      // road-like graph: 22.6 against 18.2 ms for the batch of 250K roots
+      // after rank 400K).
+      bool small_pass = false;
+      if (flags == TREE_FLAGS && hooks && hooks->route_small && slots < slots_wanted) {
+        bigh.resize(R);
+        const uint32_t ns = hooks->route_small(hooks->ctx, hroots.data(), R, kSmallRoute, bigh.data());
+        if (ns >= std::min<uint32_t>(R, 1024)) {
+          rc = ensure_small();
+          if (rc) return rc;
+          CHL_HIP(bigd.reserve(std::max<size_t>(W, 1)));
+          CHL_HIP(hipMemcpyAsync(bigd.p, bigh.data(), R, hipMemcpyHostToDevice, stream));
+          small_pass = true;
+        }
+      }
       for (int s = 0; s < nsides; ++s) {
-        rc = run_trees(s, R, hroots.data(), prefix, flags, L, lt);
+        rc = run_trees(s, R, hroots.data(), prefix, flags, L, lt, small_pass);
         if (rc) return rc;
         if (s == 0 && nsides == 2) {
           // tree statistics are per side; keep the forward side for the hooks

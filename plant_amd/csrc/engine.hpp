@@ -36,6 +36,7 @@ struct LabelStore {
   DBuf<SegView> dview[2];  // device copies of the segment views
   BigBuf<uint64_t> rng[2];  // per side, built for queries: vertex-major row bounds in every segment
   uint64_t total[2] = {0, 0};
+  int verbose = 0;  // 1: a line per merge on stderr
   int nsides() const { return shared_sides ? 1 : 2; }
   const std::vector<std::unique_ptr<Segment>>& side(int s) const { return seg[shared_sides ? 0 : s]; }
   const SegView* d_views(int s) const { return dview[shared_sides ? 0 : s].p; }
@@ -96,6 +97,10 @@ struct BuildHooks {
   uint32_t (*pilot)(void* ctx, uint32_t* roots, uint32_t cap) = nullptr;
   void (*on_pilot)(void* ctx, const uint32_t* roots, const uint32_t* tree_size, const uint32_t* label_count,
                    uint32_t R) = nullptr;
+  // PLaNT batch of roots[0..R): big[k] = 1 for a root predicted to grow a tree of
+  // more than `limit` processed vertices, else 0; returns the roots predicted
+  // small (0: no prediction, every root to the tree kernel).
+  uint32_t (*route_small)(void* ctx, const uint32_t* roots, uint32_t R, uint32_t limit, uint8_t* big) = nullptr;
   uint32_t slots = 0;  // set by the builder before the first batch
 };
 

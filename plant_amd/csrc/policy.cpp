@@ -363,6 +363,22 @@ struct Learned {
     return (uint64_t)std::min(label_budget, std::max(2.0 * s, s > 4e6 ? label_budget : 0.0)) + 4096;
   }
 
+  // Small-tree routing: a root goes to the small-tree kernel when its predicted
+  // tree (processed vertices) is at most `limit`. The caller's limit leaves room
+  // for the prediction's error (its table holds several times the limit); a root
+  // predicted small whose tree outgrows the table runs again on the tree kernel.
+  uint32_t route_small(const uint32_t* roots, uint32_t R, uint32_t limit, uint8_t* big) {
+    if (!trained || switched || trial_pending || pred_r0 < 0) return 0;
+    uint32_t small = 0;
+    for (uint32_t k = 0; k < R; ++k) {
+      if (roots[k] < pred_r0 || roots[k] >= pred_r0 + pred_cnt) return 0;
+      const bool b = lin_tree(roots[k]) > (float)limit;
+      big[k] = b ? 1 : 0;
+      small += b ? 0 : 1;
+    }
+    return small;
+  }
+
   // Pilot sample: 512 ranks log-uniform over [max(1024, n/128), n) (trees there are
   // cheap on road-like graphs), so the first real batch is already packed by
   // prediction; the top ranks are left to the trend. Not in hybrid mode: on
@@ -602,6 +618,10 @@ uint64_t hk_out(void* c, const uint32_t* roots, uint32_t R) {
   HostTimer t(c);
   return ((Learned*)c)->out_estimate(roots, R);
 }
+uint32_t hk_small(void* c, const uint32_t* roots, uint32_t R, uint32_t limit, uint8_t* big) {
+  HostTimer t(c);
+  return ((Learned*)c)->route_small(roots, R, limit, big);
+}
 uint32_t hk_pilot(void* c, uint32_t* roots, uint32_t cap) {
   HostTimer t(c);
   return ((Learned*)c)->pilot(roots, cap);
@@ -637,6 +657,7 @@ int run_build(const Graph& g, DeviceGraph& dg, const std::vector<uint32_t>& orde
   h.next_width = hk_next_width;
   h.order_roots = hk_order;
   h.out_estimate = hk_out;
+  if (opts.small_trees) h.route_small = hk_small;
   h.on_batch = hk_batch;
   h.pilot = hk_pilot;
   h.on_pilot = hk_on_pilot;

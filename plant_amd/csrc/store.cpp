@@ -10,6 +10,7 @@
 // order, which is hub order. Both sides of a store always have the same
 // segments (the same hub ranges), so a query merges segment by segment.
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -95,6 +96,9 @@ int LabelStore::merge_range(size_t a, size_t b, hipStream_t st) {
   DBuf<uint64_t> tmp;
   CHL_HIP(cnt.reserve(std::max<uint64_t>(rows, 1)));
   CHL_HIP(tmp.reserve(scan_tmp_words(rows)));
+  const auto wall = [] { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+  double t_alloc = 0, t_copy = 0, t_free = 0;
+  uint64_t moved = 0;
   for (int s = 0; s < nsides(); ++s) {
     auto& L = seg[s];
     uint64_t count = 0;
@@ -106,17 +110,27 @@ int LabelStore::merge_range(size_t a, size_t b, hipStream_t st) {
     std::unique_ptr<Segment> m(new Segment());
     m->h0 = L[a]->h0;
     m->h1 = L[b - 1]->h1;
+    double t = wall();
     int rc = m->alloc(rows, count, f5 ? SEG_F5 : SEG_F8);
     if (rc) return rc;
+    t_alloc += wall() - t;
+    t = wall();
     const SegView* views = dview[s].p + a;  // the device copies of segments a..b
     CHL_HIP(launch_row_total(views, (uint32_t)(b - a), (uint32_t)rows, cnt.p, st));
     CHL_HIP(launch_scan_u32(cnt.p, rows, m->off.p, tmp.p, st));
     CHL_HIP(launch_concat_rows(views, (uint32_t)(b - a), (uint32_t)rows, m->off.p, m->fmt, m->h0, m->key.p, m->lo.p,
                                m->ent.p, st));
     CHL_HIP(hipStreamSynchronize(st));
+    t_copy += wall() - t;
+    moved += m->entry_bytes();
+    t = wall();
     L.erase(L.begin() + a, L.begin() + b);
     L.insert(L.begin() + a, std::move(m));
+    t_free += wall() - t;
   }
+  if (verbose)
+    fprintf(stderr, "merge segments [%zu, %zu): %.2f GB, alloc %.3f s, copy %.3f s, free %.3f s\n", a, b, moved / 1e9,
+            t_alloc, t_copy, t_free);
   return sync_views();
 }
 

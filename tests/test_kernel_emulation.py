@@ -125,3 +125,70 @@ def test_slot_counts_at_their_bounds(chl, orc, emulator, tmp_path, start):
         assert restored == 1, (name, start)
         for a, b, what in zip(got, ref.copy(0), ("off", "hub", "dist")):
             assert np.array_equal(np.asarray(a), np.asarray(b)), (name, start, what)
+
+
+SMALL_SRC = os.path.join(HERE, "emu", "small_emu.cpp")
+
+
+@pytest.fixture(scope="module")
+def small_emulator(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    exe = str(tmp_path_factory.mktemp("emu") / "small_emu")
+    cmd = ["g++", "-std=c++20", "-O2", "-pthread", "-I", os.path.join(HERE, "emu"), "-x", "c++", SMALL_SRC, "-o",
+           exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return exe
+
+
+def run_small(exe, tmp_path, g, order, delta, cap, slots, bigstep, seed=7):
+    off, head, ln = g.csr(1)
+    n = g.n
+    arc = (head.astype(np.uint64) << np.uint64(32)) | ln.astype(np.uint32).astype(np.uint64)
+    rank = np.empty(n, np.uint32)
+    rank[order] = np.arange(n, dtype=np.uint32)
+    roots = np.asarray(order, np.uint32)
+    inp, out = tmp_path / "sin.bin", tmp_path / "sout.bin"
+    with open(inp, "wb") as f:
+        f.write(np.array([n, arc.size, n, delta, cap, slots, bigstep], np.uint32).tobytes())
+        f.write(off.astype(np.uint32).tobytes())
+        f.write(arc.tobytes())
+        f.write(rank.tobytes())
+        f.write(roots.tobytes())
+    r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, EMU_SEED=str(seed)))
+    assert r.returncode == 0, (r.returncode, r.stderr[-2000:])
+    data = np.fromfile(out, np.uint8)
+    count = int(data[:8].view(np.uint64)[0])
+    pairs = data[8:8 + 16 * count].view(np.uint64).reshape(-1, 2)
+    done_small, empty = data[8 + 16 * count:].view(np.uint32)
+    vrank = (pairs[:, 0] >> np.uint64(32)).astype(np.int64)
+    hub = (pairs[:, 1] >> np.uint64(32)).astype(np.uint32)
+    dist = (pairs[:, 1] & np.uint64(0xFFFFFFFF)).astype(np.int32)
+    vert = np.asarray(order, np.int64)[vrank]
+    srt = np.lexsort((hub, vert))
+    cnt = np.bincount(vert, minlength=n)
+    offs = np.concatenate([[0], np.cumsum(cnt)]).astype(np.uint64)
+    return (offs, hub[srt], dist[srt]), int(done_small), int(empty)
+
+
+# (table entries, slots, every k-th root marked big): a table too small for the
+# top roots' trees (they are abandoned and handed over), and big roots skipped
+@pytest.mark.parametrize("cap,slots,bigstep", [(64, 2, 0), (256, 3, 5), (4096, 1, 0)])
+def test_small_tree_kernel_then_tree_kernel_equal_oracle(chl, orc, small_emulator, tmp_path, cap, slots, bigstep):
+    """The small-tree kernel (small.hip) over every root, then the tree kernel over
+    the roots it handed over (marked big, or trees that outgrew half a table), as
+    the label build runs them: the labels must be the oracle's canonical labels,
+    and every small table must be empty afterwards."""
+    for name, g, order in cases(chl):
+        src, dst, w = g.arc_list(1)
+        ref, _ = orc.gll(orc.OGraph.from_arcs(g.n, src, dst, w), order, threads=4)
+        for delta in (3, 64):
+            got, done_small, empty = run_small(small_emulator, tmp_path, g, order, delta, cap, slots, bigstep)
+            assert empty == 1, (name, cap, delta)
+            assert done_small > 0, (name, cap, delta)
+            if cap == 64:
+                assert done_small < g.n  # some trees outgrew the table and went to the tree kernel
+            for a, b, what in zip(got, ref.copy(0), ("off", "hub", "dist")):
+                assert np.array_equal(np.asarray(a), np.asarray(b)), (name, cap, delta, what)
