@@ -190,9 +190,15 @@ constexpr uint32_t kWideBelow = 1024;
 // predicted tree processes at most kSmallRoute vertices, an eighth of the
 // entries a tree may insert before it is handed to the tree kernel (the
 // predictions' mean error is a factor of about 2.4).
+#ifndef CHL_SMALL_CAP
+#define CHL_SMALL_CAP 32768
+#endif
+#ifndef CHL_SMALL_ROUTE_DIV
+#define CHL_SMALL_ROUTE_DIV 16
+#endif
 constexpr uint32_t kSmallSlots = 4096;
-constexpr uint32_t kSmallCap = 32768;
-constexpr uint32_t kSmallRoute = kSmallCap / 16;
+constexpr uint32_t kSmallCap = CHL_SMALL_CAP;
+constexpr uint32_t kSmallRoute = kSmallCap / CHL_SMALL_ROUTE_DIV;
 
 // Search slots for a granularity: base_slots workgroups of 256 threads, as
 // many threads in flight at 64 or 1024 threads per root.
