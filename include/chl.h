@@ -134,10 +134,17 @@ typedef struct chl_build_opts {
                                 feature of the learned batching predictor; NULL = not used */
   int keep_segments;    /* 1: keep the label store's segments as the build left them (no final merge);
                            0: merge them into one where device memory allows */
-  int small_trees;      /* learned batching, PLaNT batches: 1 (default) runs the roots predicted to grow
-                           small trees one wavefront each with a hash-table search state, the others
-                           with the tree kernel; 0: the tree kernel only. Labels are the same. */
-  int reserved[6];
+  int small_trees;      /* learned batching, PLaNT batches of a memory-limited build: 1 (default) runs
+                           the roots predicted to grow small trees one wavefront each with a hash-table
+                           search state, the others with the tree kernel; 0: the tree kernel only.
+                           Labels are the same. */
+  int medium_trees;     /* with small_trees: 1 also runs the roots predicted to grow medium trees 256
+                           threads each with a larger hash table (7.5 GB); 0 (default): those go to the
+                           tree kernel. Labels are the same. */
+  int slot_memory_mb;   /* cap on the device memory of the search slots in MiB (0 = automatic: the free
+                           memory beyond a reserve); a cap below what the default slot count needs makes
+                           the build memory-limited, as on a large graph */
+  int reserved[4];
 } chl_build_opts;
 
 typedef struct chl_build_stats {
@@ -179,6 +186,7 @@ typedef struct chl_build_stats {
   int64_t slots_final;          /* search slots at the last batch (slots_used: at the first) */
   double slot_s;                /* wall seconds spent giving search slots back (memory-limited builds) */
   int64_t small_roots;          /* roots whose trees the small-tree kernel completed */
+  int64_t medium_roots;         /* roots whose trees the medium-tree kernel completed */
 } chl_build_stats;
 
 void chl_build_opts_default(chl_build_opts* o);

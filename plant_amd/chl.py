@@ -38,7 +38,7 @@ class BuildOpts(C.Structure):
                 ("delta", C.c_int32), ("slots", C.c_int32), ("granularity", C.c_int), ("switch_ratio", C.c_double),
                 ("plant_prune", C.c_int), ("device", C.c_int), ("verbose", C.c_int), ("out_capacity", C.c_int64),
                 ("vertex_score", C.c_void_p), ("keep_segments", C.c_int), ("small_trees", C.c_int),
-                ("reserved", C.c_int * 6)]
+                ("medium_trees", C.c_int), ("slot_memory_mb", C.c_int), ("reserved", C.c_int * 4)]
 
 
 class ServerOpts(C.Structure):
@@ -71,7 +71,7 @@ class BuildStats(C.Structure):
                 ("settled_distinct", C.c_int64), ("settled_arcs", C.c_int64), ("relax_bytes_processed", C.c_double),
                 ("segment_ms", C.c_double), ("store_bytes", C.c_double), ("segments", C.c_int64),
                 ("slot_shrinks", C.c_int64), ("slots_final", C.c_int64), ("slot_s", C.c_double),
-                ("small_roots", C.c_int64)]
+                ("small_roots", C.c_int64), ("medium_roots", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -120,6 +120,7 @@ void chl_build_opts_default(chl_build_opts* o) {
   o->delta = 0;
   o->slots = 0;
   o->small_trees = 1;
+  o->medium_trees = 0;
   o->switch_ratio = 1024.0;
   o->plant_prune = 0;
   o->device = 0;

@@ -285,16 +285,20 @@ struct TreeParams {
   uint32_t* slist;
   uint32_t scap;
   const uint8_t* big;
+  // tree_kernel: frontiers of at least sort_min entries (0: none) and at most
+  // the kernel's LDS bound are sorted by device id before they are processed
+  uint32_t sort_min;
 };
 
 // One workgroup per slot of threads_per_root threads: 64 (a wavefront per root),
 // 256 or 1024.
 hipError_t launch_tree(const TreeParams& p, uint32_t flags, uint32_t slots, int threads_per_root, hipStream_t s);
-// PLaNT trees of the batch's roots, one wavefront per root over `slots` small
-// slots (small.hip); a root marked big, or whose tree outgrows half its table,
-// goes to spill[] (counted in CTR_SKIPPED) for a launch of tree_kernel.
-hipError_t launch_small_trees(const TreeParams& p, uint32_t slots, hipStream_t s);
-// scap-entry tables of `slots` small slots set to empty
+// PLaNT trees of the batch's roots over `slots` hashed slots of scap entries
+// (small.hip), `threads` per root: 64 (small tables) or 256 (medium tables); a
+// root marked big, or whose tree outgrows half its table, goes to spill[]
+// (counted in CTR_SKIPPED) for a later launch.
+hipError_t launch_small_trees(const TreeParams& p, uint32_t slots, int threads, hipStream_t s);
+// scap-entry tables of `slots` hashed slots set to empty
 hipError_t launch_init_small(const TreeParams& p, uint32_t slots, hipStream_t s);
 // count cells set to the unreached state (state all ones, stamp 0) with the
 // rank of their vertex (cell i: device id i mod n); the slots' seq words must be

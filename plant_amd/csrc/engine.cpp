@@ -200,6 +200,45 @@ constexpr uint32_t kSmallSlots = 4096;
 constexpr uint32_t kSmallCap = CHL_SMALL_CAP;
 constexpr uint32_t kSmallRoute = kSmallCap / CHL_SMALL_ROUTE_DIV;
 
+// Medium-tree slots (small.hip, 256 threads per root; build option
+// medium_trees): kMediumSlots workgroups, each with a hash table of kMediumCap
+// entries (28 bytes each: 7.5 GB in all), for the roots the small slots leave
+// whose predicted tree processes at most kMediumRoute vertices; a medium pass
+// runs when at least kMediumMin such roots are in the batch. Off by default: on
+// the 16M-vertex road-like graph a medium pass costs 0.4-0.8 ns per processed
+// vertex against 0.25-0.38 ns in the tree kernel, and the tree kernel's pass
+// after it, bound by the batch's largest trees, hardly shortens (learned build
+// 105.5 s against 98.4 s without).
+#ifndef CHL_MEDIUM_CAP
+#define CHL_MEDIUM_CAP 262144
+#endif
+#ifndef CHL_MEDIUM_SLOTS
+#define CHL_MEDIUM_SLOTS 1024
+#endif
+#ifndef CHL_MEDIUM_ROUTE_DIV
+#define CHL_MEDIUM_ROUTE_DIV 8
+#endif
+constexpr uint32_t kMediumSlots = CHL_MEDIUM_SLOTS;
+constexpr uint32_t kMediumCap = CHL_MEDIUM_CAP;
+constexpr uint32_t kMediumRoute = kMediumCap / CHL_MEDIUM_ROUTE_DIV;
+constexpr uint32_t kMediumMin = 256;
+
+// Frontier sorting in the tree kernel (tree.hip): frontiers of at least
+// kSortMin entries, in launches of at least kSortWaves wavefronts (half of what
+// 256 compute units hold). 1M road-like graph (2048 slots of 256 threads): tree
+// time 0.915 -> 0.864 s per learned build; 16M (a few hundred slots, fewer as
+// the store grows): sorting in every launch cost 2-16% of the tree time.
+constexpr uint32_t kSortMin = 512;
+constexpr uint64_t kSortWaves = 4096;
+
+// Device memory a memory-limited build keeps free beside the search slots (for
+// the batch buffers, segment building and the store's appends and merges): a
+// kReserveDiv-th of the device.
+#ifndef CHL_RESERVE_DIV
+#define CHL_RESERVE_DIV 4
+#endif
+constexpr size_t kReserveDiv = CHL_RESERVE_DIV;
+
 // Search slots for a granularity: base_slots workgroups of 256 threads, as
 // many threads in flight at 64 or 1024 threads per root.
 static uint32_t slots_for(uint32_t base_slots, uint32_t tpr) {
@@ -229,6 +268,13 @@ struct Builder {
   DBuf<uint8_t> bigd;
   std::vector<uint8_t> bigh;
   uint32_t sslots = 0;
+  // medium-tree slots: tables of kMediumCap entries; bigmh[k] = 1 for batch
+  // position k predicted too large for them, bigmd its copy for the current roots
+  DBuf<uint64_t> mstate;
+  DBuf<uint32_t> mkey, mstamp, mlist;
+  DBuf<uint8_t> bigmd;
+  std::vector<uint8_t> bigmh, bigm_cur;
+  uint32_t mslots = 0;
   std::vector<uint32_t> id_of_rank;  // rank -> device id
   std::vector<uint32_t> hids;        // staging of root ids
   uint32_t slots = 0, gcap = 0;
@@ -302,8 +348,9 @@ struct Builder {
     CHL_HIP(hipMemGetInfo(&freeb, &totalb));
     freeb += dev_cached_bytes();  // blocks the allocator keeps are free to reuse
     const size_t per = (size_t)n * kSlotBytesPerVertex;
-    const size_t reserve = totalb / 4;
-    const size_t budget = freeb > reserve + totalb / 8 ? freeb - reserve : freeb / 3;
+    const size_t reserve = totalb / kReserveDiv;
+    size_t budget = freeb > reserve + totalb / 8 ? freeb - reserve : freeb / 3;
+    if (o.slot_memory_mb > 0) budget = std::min(budget, (size_t)o.slot_memory_mb << 20);
     uint32_t fit = (uint32_t)std::max<size_t>(1, budget / std::max<size_t>(per, 1));
     slots_wanted = want;
     slots = std::max<uint32_t>(1, std::min(want, fit));
@@ -326,18 +373,18 @@ struct Builder {
   }
 
   // Before a batch of a memory-limited build: when the free memory has fallen
-  // below a fifth of the device (the store grows behind the batches), slots go
-  // back until a quarter is free again, keeping at least kMinSlots. The slots
-  // that stay keep their memory and contents (cells, counts).
+  // below four fifths of the reserve (the store grows behind the batches), slots
+  // go back until the reserve is free again, keeping at least kMinSlots. The
+  // slots that stay keep their memory and contents (cells, counts).
   static constexpr uint32_t kMinSlots = 32;
   int fit_slots() {
     if (slots >= slots_wanted || slots <= kMinSlots || !cell.chunked) return CHL_OK;
     size_t freeb = 0, totalb = 0;
     CHL_HIP(hipMemGetInfo(&freeb, &totalb));
     freeb += dev_cached_bytes();
-    if (freeb >= totalb / 5) return CHL_OK;
+    if (freeb >= totalb * 4 / (5 * kReserveDiv)) return CHL_OK;
     const size_t per = (size_t)n * kSlotBytesPerVertex;
-    const size_t give = (totalb / 4 - freeb + per - 1) / per;
+    const size_t give = (totalb / kReserveDiv - freeb + per - 1) / per;
     const uint32_t keep = (uint32_t)std::max<int64_t>(kMinSlots, (int64_t)slots - (int64_t)give);
     if (keep >= slots) return CHL_OK;
     const double t0 = now_s();
@@ -388,6 +435,24 @@ struct Builder {
     return CHL_OK;
   }
 
+  int ensure_medium() {
+    if (mslots) return CHL_OK;
+    const size_t cells = (size_t)kMediumSlots * kMediumCap;
+    CHL_HIP(mstate.reserve(cells));
+    CHL_HIP(mkey.reserve(cells));
+    CHL_HIP(mstamp.reserve(cells));
+    CHL_HIP(mlist.reserve(3 * cells));
+    TreeParams p;
+    memset(&p, 0, sizeof(p));
+    p.sstate = mstate.p;
+    p.skey = mkey.p;
+    p.sstamp = mstamp.p;
+    p.scap = kMediumCap;
+    CHL_HIP(launch_init_small(p, kMediumSlots, stream));
+    mslots = kMediumSlots;
+    return CHL_OK;
+  }
+
   int ensure_out(SideBatch& b, uint64_t cap) {
     CHL_HIP(b.row.reserve(cap));
     CHL_HIP(b.hd.reserve(cap));
@@ -414,7 +479,7 @@ struct Builder {
     if (!need) return CHL_OK;
     size_t freeb = 0, totalb = 0;
     CHL_HIP(hipMemGetInfo(&freeb, &totalb));
-    if (freeb > totalb / 4) return CHL_OK;
+    if (freeb > totalb / kReserveDiv) return CHL_OK;
     const uint64_t keep = 2 * need + ((uint64_t)1 << 20);
     size_t freed = 0;
     auto trim = [&](auto& buf) {
@@ -442,10 +507,12 @@ struct Builder {
   // started once the buffer was full, run in a further launch after the buffer
   // has grown by what the former reported and an estimate for the latter.
   // small_pass: the first launch runs the small-tree kernel over the batch, with
-  // bigd[k] = 1 for roots it leaves to the tree kernel (they, and trees that
+  // bigd[k] = 1 for roots it leaves to later launches (they, and trees that
   // outgrow a small slot, go to the spill list like roots not started).
+  // medium_pass: the next launch runs the medium-tree kernel over the roots left
+  // (bigmh by batch position: those it leaves to the tree kernel).
   int run_trees(int side, uint32_t R, const uint32_t* hroots, uint32_t prefix, uint32_t flags, LabelStore& L,
-                double& ms, bool small_pass = false) {
+                double& ms, bool small_pass = false, bool medium_pass = false) {
     SideBatch& b = sb[side];
     const int range = side == 0 ? 1 : 0;  // forward tree walks out-arcs and fills side 0
     int rc = ensure_out(b, out_cap);
@@ -494,10 +561,31 @@ struct Builder {
     for (uint32_t j = 0; j < R; ++j) orig[j] = j;  // position of each current root in the batch
     unsigned long long hc[CTR_COUNT];
     uint64_t lab0 = 0;  // label counter at the start of the pass
+    const int medium_at = medium_pass ? (small_pass ? 1 : 0) : -1;  // the medium launch's pass
     for (int pass = 0;; ++pass) {
+      if (pass == medium_at) {
+        // the medium tables, and their big flags for the roots of this pass
+        bigm_cur.resize(p.R);
+        for (uint32_t j = 0; j < p.R; ++j) bigm_cur[j] = bigmh[orig[j]];
+        CHL_HIP(bigmd.reserve(std::max<size_t>(p.R, 1)));
+        CHL_HIP(hipMemcpyAsync(bigmd.p, bigm_cur.data(), p.R, hipMemcpyHostToDevice, stream));
+        p.sstate = mstate.p;
+        p.skey = mkey.p;
+        p.sstamp = mstamp.p;
+        p.slist = mlist.p;
+        p.scap = kMediumCap;
+        p.big = bigmd.p;
+      }
       CHL_HIP(hipEventRecord(ev[0], stream));
-      if (pass == 0 && small_pass) CHL_HIP(launch_small_trees(p, std::min<uint32_t>(sslots, p.R), stream));
-      else CHL_HIP(launch_tree(p, flags, std::min<uint32_t>(active, p.R), (int)tpr, stream));
+      if (pass == 0 && small_pass) CHL_HIP(launch_small_trees(p, std::min<uint32_t>(sslots, p.R), 64, stream));
+      else if (pass == medium_at) CHL_HIP(launch_small_trees(p, std::min<uint32_t>(mslots, p.R), 256, stream));
+      else {
+        // sorting the frontiers pays when the launch keeps the memory system busy
+        // (many trees in flight), not when a few slots wait on latency
+        const uint32_t wgs = std::min<uint32_t>(active, p.R);
+        p.sort_min = (uint64_t)wgs * tpr / 64 >= kSortWaves ? kSortMin : 0;
+        CHL_HIP(launch_tree(p, flags, wgs, (int)tpr, stream));
+      }
       CHL_HIP(hipEventRecord(ev[1], stream));
       if (pass > 0) {
         // this pass's statistics to the roots' positions in the batch
@@ -517,6 +605,7 @@ struct Builder {
       if (o.verbose)
         fprintf(stderr, "  side %d pass %d: %u roots, %.3f ms, written %llu, spilled %llu (%llu labels, %llu not started), processed %llu arcs %llu rounds %llu\n", side, pass, p.R, t, hc[CTR_LABELS], hc[CTR_OVERFLOW], hc[CTR_SPILL], hc[CTR_SKIPPED], hc[CTR_PROCESSED], hc[CTR_ARCS], hc[CTR_ROUNDS]);
       if (pass == 0 && small_pass && stats) stats->small_roots += (int64_t)(p.R - hc[CTR_OVERFLOW]);
+      if (pass == medium_at && stats) stats->medium_roots += (int64_t)(p.R - hc[CTR_OVERFLOW]);
       if (!hc[CTR_OVERFLOW]) break;
       // next launch for the roots that did not fit or were not started
       const uint32_t ns = (uint32_t)hc[CTR_OVERFLOW];
@@ -973,9 +1062,13 @@ struct Builder {
       // count the tree kernel is the faster of the two for small trees too (1M
       // road-like graph: 22.6 against 18.2 ms for the batch of 250K roots
       // after rank 400K).
-      bool small_pass = false;
+      // Roots predicted between the small tables and kMediumRoute run on the
+      // medium-tree kernel next (256 threads per root, a table of kMediumCap
+      // entries: about 80 times as many trees in flight per byte as a dense slot).
+      bool small_pass = false, medium_pass = false;
       if (flags == TREE_FLAGS && hooks && hooks->route_small && slots < slots_wanted) {
         bigh.resize(R);
+        bigmh.resize(R);
         const uint32_t ns = hooks->route_small(hooks->ctx, hroots.data(), R, kSmallRoute, bigh.data());
         if (ns >= std::min<uint32_t>(R, 1024)) {
           rc = ensure_small();
@@ -984,9 +1077,16 @@ struct Builder {
           CHL_HIP(hipMemcpyAsync(bigd.p, bigh.data(), R, hipMemcpyHostToDevice, stream));
           small_pass = true;
         }
+        const uint32_t nm =
+            o.medium_trees ? hooks->route_small(hooks->ctx, hroots.data(), R, kMediumRoute, bigmh.data()) : 0;
+        if (nm >= (small_pass ? ns : 0) + kMediumMin) {
+          rc = ensure_medium();
+          if (rc) return rc;
+          medium_pass = true;
+        }
       }
       for (int s = 0; s < nsides; ++s) {
-        rc = run_trees(s, R, hroots.data(), prefix, flags, L, lt, small_pass);
+        rc = run_trees(s, R, hroots.data(), prefix, flags, L, lt, small_pass, medium_pass);
         if (rc) return rc;
         if (s == 0 && nsides == 2) {
           // tree statistics are per side; keep the forward side for the hooks

@@ -54,6 +54,24 @@ constexpr uint32_t kLdsHashBits = 12;
 constexpr uint32_t kLdsHash = 1u << kLdsHashBits;  // entries of the LDS root-label table
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
+// Frontiers of p.sort_min to kFrontSort entries are sorted by device id in LDS
+// before they are processed (kFrontSort 0 or sort_min 0: never), so that the
+// lanes of a wavefront take vertices that lie close together in the locality
+// order and their cell loads share cache lines.
+#ifndef CHL_FRONT_SORT
+#define CHL_FRONT_SORT 4096
+#endif
+constexpr uint32_t kFrontSort = CHL_FRONT_SORT;
+static_assert(kFrontSort == 0 || (kFrontSort >= 64 && (kFrontSort & (kFrontSort - 1)) == 0),
+              "the frontier sort bound is 0 or a power of two of at least 64");
+
+// Orders the LDS accesses of one wavefront's lanes (no workgroup barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ uint32_t hash_slot(uint32_t h, uint32_t shift) { return (h * 0x9E3779B1u) >> shift; }
 
 __device__ __forceinline__ uint32_t table_find(const uint32_t* key, const uint32_t* val, uint32_t shift,
@@ -163,6 +181,9 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
   constexpr int kWaves = THREADS / 64;
   __shared__ uint32_t s_key[PRUNE ? kLdsHash : 1];
   __shared__ uint32_t s_val[PRUNE ? kLdsHash : 1];
+  // (wavefront-per-root launches run many workgroups per CU: no sort there)
+  constexpr uint32_t kSortCap = THREADS >= 256 ? kFrontSort : 0u;
+  __shared__ uint32_t s_sort[kSortCap ? kSortCap : 1];
   __shared__ uint32_t s_ri, s_fsize, s_nsize, s_nvis, s_below, s_clean, s_minnext, s_emit, s_dsize, s_dmin;
   __shared__ unsigned long long s_base;
   __shared__ unsigned long long s_ctr[6];
@@ -447,6 +468,42 @@ __global__ __launch_bounds__(THREADS) void tree_kernel(TreeParams p) {
           }
         }
         __syncthreads();
+      }
+      if (kSortCap && p.sort_min && nsize > 1 && nsize >= p.sort_min && nsize <= kSortCap) {
+        // bitonic sort of the next frontier by device id, padded with all ones to
+        // a power of two. Thread t owns the pair (2t, 2t + 1) at loading and at
+        // write-back and, at strides up to 64, only pairs inside its wavefront's
+        // 128-entry block: a stage needs a workgroup barrier only when it or the
+        // stage before it crosses those blocks, else a wavefront sync. (The
+        // round's closing barrier orders the write-back before the next round
+        // reads it.)
+        uint32_t P = 64;
+        while (P < nsize) P <<= 1;
+        for (uint32_t t = tid; t < P / 2; t += kThreads) {
+          s_sort[2 * t] = 2 * t < nsize ? N[2 * t] : kEmpty;
+          s_sort[2 * t + 1] = 2 * t + 1 < nsize ? N[2 * t + 1] : kEmpty;
+        }
+        bool crossed = false;  // the previous stage crossed wavefront blocks
+        for (uint32_t size = 2; size <= P; size <<= 1)
+          for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            if (crossed || stride > 64)
+              __syncthreads();
+            else
+              wave_sync();
+            crossed = stride > 64;
+            for (uint32_t t = tid; t < P / 2; t += kThreads) {
+              const uint32_t i = 2 * t - (t & (stride - 1)), j = i + stride;
+              const uint32_t a = s_sort[i], b = s_sort[j];
+              if ((a > b) == ((i & size) == 0)) {
+                s_sort[i] = b;
+                s_sort[j] = a;
+              }
+            }
+          }
+        for (uint32_t t = tid; t < P / 2; t += kThreads) {
+          if (2 * t < nsize) N[2 * t] = s_sort[2 * t];
+          if (2 * t + 1 < nsize) N[2 * t + 1] = s_sort[2 * t + 1];
+        }
       }
       if (tid == 0) s_fsize = nsize;
       uint32_t* t = F;

@@ -9,6 +9,8 @@
 //        than the slot's last, no stamp at or above its next stamp base), else 0
 // EMU_EPOCH0 / EMU_SBASE0 start every slot's tree count and stamp base at the
 // given values, so a launch crosses their bounds and resets the slot's cells.
+// EMU_SORT_MIN sets the smallest frontier sorted in LDS (default 2: every
+// frontier within the kernel's bound; 0: none).
 #include "../../plant_amd/csrc/tree.hip"
 
 #include <cstdio>
@@ -74,6 +76,8 @@ int main(int argc, char** argv) {
   p.tree_size = ts.data();
   p.label_count = lc.data();
   p.ctr = ctr.data();
+  p.sort_min = 2;
+  if (const char* e = getenv("EMU_SORT_MIN")) p.sort_min = (uint32_t)strtoul(e, nullptr, 10);
   emu_only_block = (int)S - 1;  // the last slot: the largest scratch offsets
   if (const char* e = getenv("EMU_SEED")) emu_seed = strtoull(e, nullptr, 10);
   launch_tree(p, flags, S, (int)threads, nullptr);

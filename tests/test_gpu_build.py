@@ -123,6 +123,29 @@ def test_knn_road_like(chl, orc):
         assert_same_labels(L, ref)
 
 
+def test_hashed_trees_under_slot_memory_cap(chl, orc):
+    """Search slots capped at 32 of 2048 (memory-limited, as on a 16M-vertex
+    graph): learned batching runs the roots predicted to grow small trees on the
+    small-table kernel (a wavefront per root), those predicted medium on the
+    medium-table kernel (256 threads per root) and the rest on the tree kernel.
+    The labels are the oracle's with and without the medium tables, and both
+    hashed kernels completed roots."""
+    g = chl.Graph.knn(100_000, 3, seed=21)
+    order, score = g.order_betweenness(64, seed=5, with_score=True)
+    _, ref = oracle_of(chl, orc, g, order)
+    cap_mb = max(1, int(32 * 36 * g.n / 2**20))
+    for medium in (1, 0):
+        L = chl.build(g, order, batching=chl.BATCH_LEARNED, vertex_score=score.ctypes.data, slot_memory_mb=cap_mb,
+                      medium_trees=medium)
+        s = L.stats
+        print(f"medium_trees={medium}: slots {s.slots_used}, small roots {s.small_roots}, "
+              f"medium roots {s.medium_roots}, batches {s.batches}", flush=True)
+        assert s.slots_used <= 32
+        assert_same_labels(L, ref)
+        assert s.small_roots > 0
+        assert (s.medium_roots > 0) == (medium == 1)
+
+
 def test_directed_random(chl, orc):
     rng = np.random.default_rng(7)
     n = 500

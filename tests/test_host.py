@@ -544,3 +544,33 @@ def test_degree_tool_unreadable_graph(tmp_path):
     assert r.stdout == ""
     assert r.stderr.endswith(f"Unable to read graph from file {missing}\n")
     assert not (tmp_path / "o").exists()
+
+
+def test_ctypes_structs_match_header(tmp_path):
+    """The binding's ctypes structures have the sizes and field offsets that the C
+    compiler gives the structures of include/chl.h."""
+    import ctypes as C
+    from plant_amd import chl as chlmod
+    structs = {"chl_build_opts": chlmod.BuildOpts, "chl_build_stats": chlmod.BuildStats,
+               "chl_server_opts": chlmod.ServerOpts, "chl_server_stats": chlmod.ServerStats}
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "chl.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-std=c11", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = {}
+    for ln in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        a, b, c = ln.split()
+        got[(a, b)] = int(c)
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == C.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)

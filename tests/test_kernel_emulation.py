@@ -22,15 +22,20 @@ SRC = os.path.join(HERE, "emu", "tree_emu.cpp")
 TREE_FLAGS = 1
 
 
-@pytest.fixture(scope="module")
-def emulator(tmp_path_factory):
+def compile_emulator(tmp_path_factory, src, name, defines=()):
     if shutil.which("g++") is None:
         pytest.skip("g++ not available")
-    exe = str(tmp_path_factory.mktemp("emu") / "tree_emu")
-    cmd = ["g++", "-std=c++20", "-O2", "-pthread", "-I", os.path.join(HERE, "emu"), "-x", "c++", SRC, "-o", exe]
+    exe = str(tmp_path_factory.mktemp("emu") / name)
+    cmd = ["g++", "-std=c++20", "-O2", "-pthread", *defines, "-I", os.path.join(HERE, "emu"), "-x", "c++", src,
+           "-o", exe]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
     return exe
+
+
+@pytest.fixture(scope="module")
+def emulator(tmp_path_factory):
+    return compile_emulator(tmp_path_factory, SRC, "tree_emu")
 
 
 def run_emulator(exe, tmp_path, g, order, threads, delta, slots=2, seed=12345, env_extra=None):
@@ -90,6 +95,43 @@ def test_tree_kernel_emulated_equals_oracle(chl, orc, emulator, tmp_path, thread
                 assert np.array_equal(np.asarray(a), np.asarray(b)), (name, delta, what)
 
 
+def wide_star(chl, leaves=300, seed=5):
+    """A centre joined to every leaf and the leaves joined in a ring: a tree that
+    reaches the centre queues every leaf in one round."""
+    rng = np.random.default_rng(seed)
+    n = leaves + 1
+    lv = np.arange(1, n, dtype=np.uint32)
+    nxt = np.roll(lv, -1)
+    src = np.concatenate([np.zeros(leaves, np.uint32), lv, lv, nxt])
+    dst = np.concatenate([lv, np.zeros(leaves, np.uint32), nxt, lv])
+    w = rng.integers(1, 9, src.size).astype(np.int32)
+    g = chl.Graph.from_arcs(n, src, dst, w)
+    return g, rng.permutation(n).astype(np.uint32)
+
+
+# (LDS bound, smallest frontier sorted): every frontier of up to 300 entries
+# (the default bound); no frontier; every one up to 128 entries (the star's
+# rounds run unsorted); every one that fits one wavefront's block; only the
+# star's widest
+@pytest.mark.parametrize("sort", [(None, 2), (None, 0), (128, 2), (64, 2), (None, 256)])
+def test_frontier_sort_wide_frontiers(chl, orc, tmp_path_factory, tmp_path, sort):
+    """Frontiers of up to 300 entries, sorted in LDS before they are processed
+    (bitonic stages across wavefronts behind workgroup barriers, within one
+    wavefront's block behind a wavefront sync) or left unsorted outside the
+    bounds: the labels must be the oracle's."""
+    bound, smin = sort
+    defines = () if bound is None else (f"-DCHL_FRONT_SORT={bound}",)
+    exe = compile_emulator(tmp_path_factory, SRC, f"tree_emu_sort_{bound}", defines)
+    g, order = wide_star(chl)
+    src, dst, w = g.arc_list(1)
+    ref, _ = orc.gll(orc.OGraph.from_arcs(g.n, src, dst, w), order, threads=4)
+    for delta in (2, 64):
+        got, restored = run_emulator(exe, tmp_path, g, order, 256, delta, env_extra={"EMU_SORT_MIN": str(smin)})
+        assert restored == 1, (sort, delta)
+        for a, b, what in zip(got, ref.copy(0), ("off", "hub", "dist")):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), (sort, delta, what)
+
+
 # (grid seed, bucket width, schedule seed): schedules in which a vertex put on the
 # deferred pile is improved below the bucket bound in the same round. A kernel
 # that used one stamp for piling and queueing lost that improvement here (one
@@ -142,7 +184,7 @@ def small_emulator(tmp_path_factory):
     return exe
 
 
-def run_small(exe, tmp_path, g, order, delta, cap, slots, bigstep, seed=7):
+def run_small(exe, tmp_path, g, order, delta, cap, slots, bigstep, seed=7, medium=(0, 0, 0)):
     off, head, ln = g.csr(1)
     n = g.n
     arc = (head.astype(np.uint64) << np.uint64(32)) | ln.astype(np.uint32).astype(np.uint64)
@@ -151,7 +193,7 @@ def run_small(exe, tmp_path, g, order, delta, cap, slots, bigstep, seed=7):
     roots = np.asarray(order, np.uint32)
     inp, out = tmp_path / "sin.bin", tmp_path / "sout.bin"
     with open(inp, "wb") as f:
-        f.write(np.array([n, arc.size, n, delta, cap, slots, bigstep], np.uint32).tobytes())
+        f.write(np.array([n, arc.size, n, delta, cap, slots, bigstep, *medium], np.uint32).tobytes())
         f.write(off.astype(np.uint32).tobytes())
         f.write(arc.tobytes())
         f.write(rank.tobytes())
@@ -162,7 +204,7 @@ def run_small(exe, tmp_path, g, order, delta, cap, slots, bigstep, seed=7):
     data = np.fromfile(out, np.uint8)
     count = int(data[:8].view(np.uint64)[0])
     pairs = data[8:8 + 16 * count].view(np.uint64).reshape(-1, 2)
-    done_small, empty = data[8 + 16 * count:].view(np.uint32)
+    done_small, done_medium, empty = data[8 + 16 * count:].view(np.uint32)
     vrank = (pairs[:, 0] >> np.uint64(32)).astype(np.int64)
     hub = (pairs[:, 1] >> np.uint64(32)).astype(np.uint32)
     dist = (pairs[:, 1] & np.uint64(0xFFFFFFFF)).astype(np.int32)
@@ -170,7 +212,7 @@ def run_small(exe, tmp_path, g, order, delta, cap, slots, bigstep, seed=7):
     srt = np.lexsort((hub, vert))
     cnt = np.bincount(vert, minlength=n)
     offs = np.concatenate([[0], np.cumsum(cnt)]).astype(np.uint64)
-    return (offs, hub[srt], dist[srt]), int(done_small), int(empty)
+    return (offs, hub[srt], dist[srt]), int(done_small), int(done_medium), int(empty)
 
 
 # (table entries, slots, every k-th root marked big): a table too small for the
@@ -185,10 +227,31 @@ def test_small_tree_kernel_then_tree_kernel_equal_oracle(chl, orc, small_emulato
         src, dst, w = g.arc_list(1)
         ref, _ = orc.gll(orc.OGraph.from_arcs(g.n, src, dst, w), order, threads=4)
         for delta in (3, 64):
-            got, done_small, empty = run_small(small_emulator, tmp_path, g, order, delta, cap, slots, bigstep)
+            got, done_small, _, empty = run_small(small_emulator, tmp_path, g, order, delta, cap, slots, bigstep)
             assert empty == 1, (name, cap, delta)
             assert done_small > 0, (name, cap, delta)
             if cap == 64:
                 assert done_small < g.n  # some trees outgrew the table and went to the tree kernel
             for a, b, what in zip(got, ref.copy(0), ("off", "hub", "dist")):
                 assert np.array_equal(np.asarray(a), np.asarray(b)), (name, cap, delta, what)
+
+
+# (small table, medium table, every k-th root big for the small tables, for the
+# medium ones): small tables that hand most trees over, medium tables too small
+# for the top trees (abandoned there as well), and roots kept from both
+@pytest.mark.parametrize("cap,cap2,bigstep,bigstep2", [(64, 128, 0, 0), (64, 4096, 3, 0), (64, 512, 2, 6)])
+def test_small_then_medium_then_tree_kernel_equal_oracle(chl, orc, small_emulator, tmp_path, cap, cap2, bigstep,
+                                                         bigstep2):
+    """Small tables one wavefront per root, then medium tables 256 threads per root
+    over the roots handed over, then the tree kernel over the rest: the labels
+    must be the oracle's canonical labels and every table empty afterwards."""
+    for name, g, order in cases(chl):
+        src, dst, w = g.arc_list(1)
+        ref, _ = orc.gll(orc.OGraph.from_arcs(g.n, src, dst, w), order, threads=4)
+        for delta in (3, 64):
+            got, done_small, done_medium, empty = run_small(small_emulator, tmp_path, g, order, delta, cap, 2,
+                                                            bigstep, medium=(cap2, 2, bigstep2))
+            assert empty == 1, (name, cap2, delta)
+            assert done_medium > 0, (name, cap2, delta)
+            for a, b, what in zip(got, ref.copy(0), ("off", "hub", "dist")):
+                assert np.array_equal(np.asarray(a), np.asarray(b)), (name, cap2, delta, what)
