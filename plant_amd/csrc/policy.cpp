@@ -385,9 +385,13 @@ struct Learned {
   // scale-free graphs low-ranked PLaNT trees cover most of the graph, and the
   // pilot's slowest tree would cost as much as the first batch; there the first
   // batch (the top ranks, static width) trains the network.
+#ifndef CHL_PILOT_DIV
+#define CHL_PILOT_DIV 128
+#endif
+  static constexpr int64_t kPilotDiv = CHL_PILOT_DIV;
   uint32_t pilot(uint32_t* roots, uint32_t cap) {
     if (hybrid) return 0;
-    const int64_t lo = std::min<int64_t>(std::max<int64_t>(1024, n / 128), n / 2);
+    const int64_t lo = std::min<int64_t>(std::max<int64_t>(1024, n / kPilotDiv), n / 2);
     if (n - lo < 64) return 0;
     const uint32_t R = (uint32_t)std::min<int64_t>(std::min<int64_t>(cap, 512), n - lo);
     std::uniform_real_distribution<double> u(std::log((double)lo + 1.0), std::log((double)n));
