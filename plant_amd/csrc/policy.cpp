@@ -379,14 +379,17 @@ struct Learned {
     return small;
   }
 
-  // Pilot sample: 512 ranks log-uniform over [max(1024, n/128), n) (trees there are
+  // Pilot sample: 512 ranks log-uniform over [max(1024, n/32), n) (trees there are
   // cheap on road-like graphs), so the first real batch is already packed by
-  // prediction; the top ranks are left to the trend. Not in hybrid mode: on
+  // prediction; the top ranks are left to the trend. The launch lasts as long as
+  // its largest tree: on the 1M road-like graph 42 ms from n/128, 17 ms from n/32
+  // (build 1.033-1.051 against 1.020-1.031 s, interleaved on one device; labels and
+  // batches the same; neutral at 16M). Not in hybrid mode: on
   // scale-free graphs low-ranked PLaNT trees cover most of the graph, and the
   // pilot's slowest tree would cost as much as the first batch; there the first
   // batch (the top ranks, static width) trains the network.
 #ifndef CHL_PILOT_DIV
-#define CHL_PILOT_DIV 128
+#define CHL_PILOT_DIV 32
 #endif
   static constexpr int64_t kPilotDiv = CHL_PILOT_DIV;
   uint32_t pilot(uint32_t* roots, uint32_t cap) {
