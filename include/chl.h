@@ -121,7 +121,7 @@ typedef struct chl_build_opts {
   int autotune;         /* 1 (PLaNT mode): pick per-root granularity and bucket width by timing trial trees
                            of sampled ranks before the first batch; ignored in the other modes */
   int64_t batch_roots;  /* roots per batch (0 = default) */
-  int32_t delta;        /* bucket width of the delta-stepping relaxation (0 = four times the mean arc length) */
+  int32_t delta;        /* bucket width of the delta-stepping relaxation (0 = eight times the mean arc length) */
   int32_t slots;        /* concurrent root slots = workgroups of the tree kernel (0 = default) */
   int granularity;      /* per-root parallelism: 0 = default (workgroup), 1 = wavefront, 2 = workgroup of
                            256 threads, 3 = workgroup of 1024 threads (a quarter of the roots in flight) */

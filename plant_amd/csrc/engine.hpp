@@ -105,6 +105,8 @@ struct BuildHooks {
 };
 
 int ensure_device(int device);
+// The default bucket width of the tree searches: a multiple of the mean arc length.
+int32_t default_delta(const Graph& g);
 // dg: the graph on the device (built into *dg first when dg->device differs).
 int build_labels(const Graph& g, DeviceGraph& dg, const std::vector<uint32_t>& order, const chl_build_opts& opts,
                  chl_build_stats* stats, BuildHooks* hooks, LabelStore& L);

@@ -38,11 +38,8 @@ int order_impl(const Graph& g, DeviceGraph& dg, int device, int64_t roots, uint6
   const uint32_t R = (uint32_t)rs.size();
   for (auto& r : rs) r = dg.pos[r];
 
-  // bucket width: four times the mean arc length (the builds' default)
-  double sum = 0;
-  for (int32_t w : g.len[1]) sum += w;
-  const double mean = g.len[1].empty() ? 1.0 : sum / (double)g.len[1].size();
-  const uint32_t delta = (uint32_t)std::max(1.0, std::min(1e9, std::floor(4.0 * mean)));
+  // bucket width: the builds' default
+  const uint32_t delta = (uint32_t)default_delta(g);
 
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
