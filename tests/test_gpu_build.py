@@ -146,6 +146,28 @@ def test_hashed_trees_under_slot_memory_cap(chl, orc):
         assert (s.medium_roots > 0) == (medium == 1)
 
 
+def test_hashed_trees_directed_under_slot_memory_cap(chl, orc):
+    """The same on a directed graph: both sides run their own hashed and
+    tree-kernel passes (forward trees over out-arcs, backward over in-arcs)."""
+    rng = np.random.default_rng(17)
+    n, m = 50_000, 250_000
+    src = rng.integers(0, n, m).astype(np.uint32)
+    dst = ((src.astype(np.int64) + rng.integers(1, 200, m)) % n).astype(np.uint32)  # local arcs
+    w = rng.integers(1, 30, m).astype(np.int32)
+    g = chl.Graph.from_arcs(n, src, dst, w)
+    assert not g.symmetric
+    order, score = g.order_betweenness(64, seed=5, with_score=True)
+    _, ref = oracle_of(chl, orc, g, order, raw=(src, dst, w))
+    cap_mb = max(1, int(32 * 36 * n / 2**20))
+    L = chl.build(g, order, batching=chl.BATCH_LEARNED, vertex_score=score.ctypes.data, slot_memory_mb=cap_mb,
+                  medium_trees=1)
+    s = L.stats
+    print(f"directed: slots {s.slots_used}, small roots {s.small_roots}, medium roots {s.medium_roots}, "
+          f"batches {s.batches}", flush=True)
+    assert_same_labels(L, ref)
+    assert s.small_roots + s.medium_roots > 0  # (these trees outgrow the small tables)
+
+
 def test_directed_random(chl, orc):
     rng = np.random.default_rng(7)
     n = 500
