@@ -228,7 +228,10 @@ constexpr uint32_t kMediumMin = 256;
 // 256 compute units hold). 1M road-like graph (2048 slots of 256 threads): tree
 // time 0.915 -> 0.864 s per learned build; 16M (a few hundred slots, fewer as
 // the store grows): sorting in every launch cost 2-16% of the tree time.
-constexpr uint32_t kSortMin = 512;
+#ifndef CHL_SORT_MIN
+#define CHL_SORT_MIN 512
+#endif
+constexpr uint32_t kSortMin = CHL_SORT_MIN;
 constexpr uint64_t kSortWaves = 4096;
 
 // Device memory a memory-limited build keeps free beside the search slots (for
