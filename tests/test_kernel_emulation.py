@@ -109,26 +109,28 @@ def wide_star(chl, leaves=300, seed=5):
     return g, rng.permutation(n).astype(np.uint32)
 
 
-# (LDS bound, smallest frontier sorted): every frontier of up to 300 entries
-# (the default bound); no frontier; every one up to 128 entries (the star's
-# rounds run unsorted); every one that fits one wavefront's block; only the
-# star's widest
+# (LDS bound, smallest frontier sorted): every frontier (the default bound);
+# no frontier; every one up to 128 entries (the star's
+# widest rounds run unsorted); every one that fits one wavefront's block; only
+# the star's widest
 @pytest.mark.parametrize("sort,threads", [((None, 2), 256), ((None, 0), 256), ((128, 2), 256), ((64, 2), 256),
-                                          ((None, 256), 256), ((None, 2), 1024)])
+                                          ((None, 129), 256), ((None, 2), 1024)])
 def test_frontier_sort_wide_frontiers(chl, orc, tmp_path_factory, tmp_path, sort, threads):
-    """Frontiers of up to 300 entries, sorted in LDS before they are processed
-    (bitonic stages across wavefronts behind workgroup barriers, within one
-    wavefront's block behind a wavefront sync) or left unsorted outside the
-    bounds: the labels must be the oracle's."""
+    """Frontiers of up to 300 entries (sorts of up to 512 entries: bitonic stages
+    across wavefronts behind workgroup barriers, within one wavefront's block
+    behind a wavefront sync), sorted in LDS before they are processed or left
+    unsorted outside the bounds: the labels must be the oracle's."""
     bound, smin = sort
     defines = () if bound is None else (f"-DCHL_FRONT_SORT={bound}",)
     exe = compile_emulator(tmp_path_factory, SRC, f"tree_emu_sort_{bound}", defines)
-    # (sixteen wavefronts per root: a smaller star, whose 150-entry frontiers still
-    # take a stage across wavefront blocks, and one bucket width, to stay quick)
-    g, order = wide_star(chl, leaves=300 if threads == 256 else 150)
+    # (the defaults on a 300-leaf star, whose 512-entry sorts caught a missing
+    # barrier after a cross-wavefront stage; the others on 150 leaves, to stay quick)
+    full = sort == (None, 2) and threads == 256
+    g, order = wide_star(chl, leaves=300 if full else 150)
     src, dst, w = g.arc_list(1)
     ref, _ = orc.gll(orc.OGraph.from_arcs(g.n, src, dst, w), order, threads=4)
-    for delta in (2, 64) if threads == 256 else (64,):
+    # both bucket widths for the defaults, a wide one (larger frontiers) otherwise
+    for delta in (2, 64) if full else (64,):
         got, restored = run_emulator(exe, tmp_path, g, order, threads, delta, env_extra={"EMU_SORT_MIN": str(smin)})
         assert restored == 1, (sort, delta)
         for a, b, what in zip(got, ref.copy(0), ("off", "hub", "dist")):
