@@ -305,7 +305,10 @@ struct Builder {
   static constexpr double kBatchBytes = 32.0;  // device bytes per batch label in flight
   // mean processed vertices per root above which memory-limited slots run
   // 1024-thread workgroups
-  static constexpr double kWideTree = 65536.0;
+#ifndef CHL_WIDE_TREE
+#define CHL_WIDE_TREE 65536.0
+#endif
+  static constexpr double kWideTree = CHL_WIDE_TREE;
   uint64_t out_cap = 0;
   uint64_t next_need = 0;  // output estimate of the next batch (labels)
   uint64_t out_limit = ~0ull;  // output capacity the memory cap allows for the current batch
