@@ -460,7 +460,7 @@ hipError_t launch_query(const QueryView& v, const uint32_t* pairs, uint32_t n, u
                         hipStream_t s);
 hipError_t launch_copy_u64(const uint64_t* src, uint64_t count, uint64_t* dst, hipStream_t s);
 
-// Library primitive (hipCUB) used by the device degree order: radix sort of
+// Library primitive (hipCUB, libsort.hip) used by the device orders: radix sort of
 // 64-bit key/value pairs.
 hipError_t sort_pairs_u64(void* tmp, size_t& tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint64_t* vin,
                           uint64_t* vout, uint64_t m, int end_bit, hipStream_t s);

@@ -10,8 +10,6 @@
 // offsets, a scatter places every label in its row, and a per-row sort puts
 // the row in hub order (labels of one row arrive from different trees in any
 // order). No row of the existing store moves.
-#include <hipcub/hipcub.hpp>
-
 #include "dev.hpp"
 
 namespace chl {
@@ -701,13 +699,6 @@ hipError_t launch_copy_u64(const uint64_t* src, uint64_t count, uint64_t* dst, h
   if (!count) return hipSuccess;
   hipLaunchKernelGGL(copy_u64_kernel, dim3(grid_for(count, 256, 16384)), dim3(256), 0, s, src, count, dst);
   return hipGetLastError();
-}
-
-// ---- library primitive (rocPRIM through hipCUB), for the device degree order -------
-
-hipError_t sort_pairs_u64(void* tmp, size_t& tmp_bytes, const uint64_t* kin, uint64_t* kout, const uint64_t* vin,
-                          uint64_t* vout, uint64_t m, int end_bit, hipStream_t s) {
-  return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kin, kout, vin, vout, (int)m, 0, end_bit, s);
 }
 
 }  // namespace chl

@@ -153,6 +153,16 @@ T __shfl(T x, int src) {
   emu_wave_done();
   return r;
 }
+// lane l reads lane l - delta (its own value for l < delta)
+template <class T>
+T __shfl_up(T x, unsigned delta) {
+  const unsigned l = threadIdx.x % 64;
+  const int64_t* v = emu_wave_publish((int64_t)x);
+  T r = l >= delta ? (T)v[l - delta] : x;
+  emu_wave_done();
+  return r;
+}
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 // every lane of the wavefront is active where the kernels call these
 #define __builtin_amdgcn_readfirstlane(x) __shfl((x), 0)
 #define __builtin_amdgcn_wave_barrier() emu_wave_done()
@@ -185,6 +195,12 @@ template <class T, class U> T atomicMin(T* p, U v) {
   emu_maybe_yield();
   T o = *p;
   if ((T)v < o) *p = (T)v;
+  return o;
+}
+template <class T, class U> T atomicOr(T* p, U v) {
+  emu_maybe_yield();
+  T o = *p;
+  *p = (T)(o | (T)v);
   return o;
 }
 template <class T, class U> T atomicMax(T* p, U v) {
