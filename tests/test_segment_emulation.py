@@ -1,10 +1,12 @@
-"""The segment build of the label store (plant_amd/csrc/labels.hip: row count,
-scan, cursor copy, scatter, per-row sort, launched as engine.cpp's build_segment
-launches them), compiled for the host with g++ against tests/emu/hip/
-hip_runtime.h and run with every workgroup's lanes as cooperative fibers. A
-batch's labels arrive in the tree kernel's emission order (any order); the
-segment must hold each row's labels hub-ascending, with the row offsets of the
-counts, in the 5-byte (hub offset, distance) and the 8-byte entry formats.
+"""The label-store kernels of plant_amd/csrc/labels.hip, compiled for the host
+with g++ against tests/emu/hip/hip_runtime.h and run with every workgroup's
+lanes as cooperative fibers: the segment build (row count, scan, cursor copy,
+scatter, per-row sort, launched as engine.cpp's build_segment launches them),
+the merge of adjacent segments (store.cpp's merge_range) and the compaction
+after cleaning (compact_side). A batch's labels arrive in the tree kernel's
+emission order (any order); every segment must hold each row's labels
+hub-ascending, with the row offsets of the counts, in the 5-byte (hub offset,
+distance) and the 8-byte entry formats.
 
 Rows of every length class are present: up to 16 entries (sorted by one lane),
 up to 4,096 (a bitonic sort in LDS) and longer (the bitonic network in global
@@ -48,14 +50,18 @@ def batch(rng, n, lengths, h0, nhub, maxd):
     return row[perm], hub[perm], dist[perm]
 
 
-def run(exe, tmp_path, n, fmt, h0, row, hub, dist, seed=3):
+def run(exe, tmp_path, n, batches, mode=0, out_fmt=SEG_F8, seed=3):
+    """batches: (fmt, h0, row, hub, dist) each; mode 0 returns the first batch's
+    segment, 1 the merge of all, 2 the first compacted by keep_flags."""
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
-    hd = (hub.astype(np.uint64) << np.uint64(32)) | dist.astype(np.uint64)
     with open(inp, "wb") as f:
-        f.write(np.array([n, fmt, h0], np.uint32).tobytes())
-        f.write(np.array([row.size], np.uint64).tobytes())
-        f.write(row.astype(np.uint32).tobytes())
-        f.write(hd.tobytes())
+        f.write(np.array([n, len(batches), mode, out_fmt], np.uint32).tobytes())
+        for fmt, h0, row, hub, dist in batches:
+            hd = (hub.astype(np.uint64) << np.uint64(32)) | dist.astype(np.uint64)
+            f.write(np.array([fmt, h0], np.uint32).tobytes())
+            f.write(np.array([row.size], np.uint64).tobytes())
+            f.write(row.astype(np.uint32).tobytes())
+            f.write(hd.tobytes())
     r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=900,
                        env=dict(os.environ, EMU_SEED=str(seed)))
     assert r.returncode == 0, (r.returncode, r.stderr[-2000:])
@@ -63,6 +69,19 @@ def run(exe, tmp_path, n, fmt, h0, row, hub, dist, seed=3):
     off = data[:8 * (n + 1)].view(np.uint64)
     ent = data[8 * (n + 1):].view(np.uint32).reshape(-1, 2)
     return off, ent[:, 0], ent[:, 1]
+
+
+def keep_flags(row, hub):
+    """The compaction test's keep rule (the harness computes the same)."""
+    return ((hub.astype(np.uint64) * np.uint64(2654435761)) % np.uint64(1 << 32)
+            ^ row.astype(np.uint64)) % np.uint64(3) != 0
+
+
+def segment_of(n, row, hub, dist):
+    """Expected segment: the labels sorted by (row, hub), the offsets their counts."""
+    order = np.lexsort((hub, row))
+    off = np.concatenate([[0], np.cumsum(np.bincount(row, minlength=n))]).astype(np.uint64)
+    return off, hub[order], dist[order]
 
 
 @pytest.mark.parametrize("fmt", [SEG_F5, SEG_F8])
@@ -76,16 +95,52 @@ def test_segment_build_sorts_rows(emulator, tmp_path, fmt):
     h0 = 1000
     maxd = 1 << 16 if fmt == SEG_F5 else 1 << 31
     row, hub, dist = batch(rng, n, lengths, h0, 6000, maxd)
-    off, got_hub, got_dist = run(emulator, tmp_path, n, fmt, h0, row, hub, dist)
-    # expected: the labels sorted by (row, hub), the offsets their running counts
-    order = np.lexsort((hub, row))
-    want_off = np.concatenate([[0], np.cumsum(np.bincount(row, minlength=n))]).astype(np.uint64)
-    assert np.array_equal(off, want_off)
-    assert np.array_equal(got_hub, hub[order])
-    assert np.array_equal(got_dist, dist[order])
+    got = run(emulator, tmp_path, n, [(fmt, h0, row, hub, dist)])
+    for a, b in zip(got, segment_of(n, row, hub, dist)):
+        assert np.array_equal(a, b)
 
 
 def test_segment_build_empty_batch(emulator, tmp_path):
     z = np.zeros(0, np.uint32)
-    off, h, d = run(emulator, tmp_path, 50, SEG_F5, 7, z, z, z)
+    off, h, d = run(emulator, tmp_path, 50, [(SEG_F5, 7, z, z, z)])
     assert np.array_equal(off, np.zeros(51, np.uint64)) and h.size == 0
+
+
+# (formats of three adjacent batches' segments, format of the merged segment)
+@pytest.mark.parametrize("fmts,out_fmt", [((SEG_F5, SEG_F5, SEG_F5), SEG_F5), ((SEG_F5, SEG_F8, SEG_F5), SEG_F8)])
+def test_segments_merge_into_one(emulator, tmp_path, fmts, out_fmt):
+    """Segments of adjacent hub ranges concatenate row by row into one segment
+    whose rows are still hub-ascending (store.cpp's merge_range)."""
+    rng = np.random.default_rng(5)
+    n = 120
+    parts, h0 = [], 100
+    for fmt in fmts:
+        lengths = rng.integers(0, 40, n)
+        lengths[rng.integers(0, n, 3)] = rng.integers(100, 300, 3)
+        # (a batch gets the 5-byte form only when its distances fit 16 bits)
+        maxd = 1 << 16 if fmt == SEG_F5 else 1 << 31
+        row, hub, dist = batch(rng, n, lengths, h0, 400, maxd)
+        parts.append((fmt, h0, row, hub, dist))
+        h0 += 400  # the next batch's hubs all rank below
+    got = run(emulator, tmp_path, n, parts, mode=1, out_fmt=out_fmt)
+    row = np.concatenate([p[2] for p in parts])
+    hub = np.concatenate([p[3] for p in parts])
+    dist = np.concatenate([p[4] for p in parts])
+    for a, b in zip(got, segment_of(n, row, hub, dist)):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("fmt", [SEG_F5, SEG_F8])
+def test_segment_compaction_keeps_order(emulator, tmp_path, fmt):
+    """Dropping the covered candidates (compact_side) keeps the kept entries of
+    every row, in row and hub order, with the new row offsets."""
+    rng = np.random.default_rng(9 + fmt)
+    n = 150
+    lengths = rng.integers(0, 70, n)
+    lengths[rng.integers(0, n, 2)] = 200  # rows spanning several 64-entry chunks
+    row, hub, dist = batch(rng, n, lengths, 50, 300, 1 << 16)
+    got = run(emulator, tmp_path, n, [(fmt, 50, row, hub, dist)], mode=2)
+    k = keep_flags(row, hub)
+    assert 0 < k.sum() < k.size
+    for a, b in zip(got, segment_of(n, row[k], hub[k], dist[k])):
+        assert np.array_equal(a, b)
