@@ -7,12 +7,15 @@
 //    (row totals, scan, concatenation) into one segment of format out_fmt, as
 //    store.cpp's merge_range does; 2 the compaction of the first segment under
 //    a keep flag per entry (count kept, scan, copy kept), as compact_side does,
-//    with keep = ((hub * 2654435761) ^ row) % 3 != 0.
+//    with keep = ((hub * 2654435761) ^ row) % 3 != 0; 3 the cleaning of the
+//    first segment against the second as the other side's batch rows
+//    (clean_side), the keep flags written after the entries.
 // Input and output are little-endian binary files:
 //   in:  u32 n (rows), u32 batches, u32 mode, u32 out_fmt; per batch: u32 fmt
 //        (SEG_F5 or SEG_F8), u32 h0 (the batch's first hub), u64 m, u32
 //        row[m], u64 hd[m] = (hub << 32) | dist in any order
-//   out: u64 off[n + 1] of the result; per entry in stored order u32 hub, u32 dist
+//   out: u64 off[n + 1] of the result; per entry in stored order u32 hub, u32 dist;
+//        mode 3: then u8 keep per entry of the first segment
 #include "../../plant_amd/csrc/tree.hip"  // (the fill kernels)
 #include "../../plant_amd/csrc/labels.hip"
 
@@ -55,7 +58,7 @@ int main(int argc, char** argv) {
   uint32_t hdr[4];
   if (fread(hdr, 4, 4, f) != 4) return 2;
   const uint32_t n = hdr[0], nb = hdr[1], mode = hdr[2], out_fmt = hdr[3];
-  if (!nb || mode > 2 || (out_fmt != SEG_F5 && out_fmt != SEG_F8)) return 2;
+  if (!nb || mode > 3 || (mode == 3 && nb != 2) || (out_fmt != SEG_F5 && out_fmt != SEG_F8)) return 2;
   if (const char* e = getenv("EMU_SEED")) emu_seed = strtoull(e, nullptr, 10);
   std::vector<HostSeg> segs(nb);
   std::vector<uint32_t> cnt(n), longq(n + 1);
@@ -91,8 +94,14 @@ int main(int argc, char** argv) {
   fclose(f);
 
   HostSeg res;
+  std::vector<uint8_t> clean_keep;
   if (mode == 0) {
     res = segs[0];
+  } else if (mode == 3) {
+    res = segs[0];
+    clean_keep.assign(segs[0].m + 1, 0xFF);
+    std::vector<unsigned long long> ctr(CTR_COUNT, 0);
+    launch_clean(segs[0].view(), segs[1].view(), n, clean_keep.data(), ctr.data(), nullptr);
   } else if (mode == 1) {
     std::vector<SegView> views(nb);
     for (uint32_t b = 0; b < nb; ++b) views[b] = segs[b].view();
@@ -133,6 +142,7 @@ int main(int argc, char** argv) {
     fwrite(&h, 4, 1, o);
     fwrite(&d, 4, 1, o);
   }
+  if (mode == 3) fwrite(clean_keep.data(), 1, res.m, o);
   fclose(o);
   return 0;
 }

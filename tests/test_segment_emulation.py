@@ -2,8 +2,8 @@
 with g++ against tests/emu/hip/hip_runtime.h and run with every workgroup's
 lanes as cooperative fibers: the segment build (row count, scan, cursor copy,
 scatter, per-row sort, launched as engine.cpp's build_segment launches them),
-the merge of adjacent segments (store.cpp's merge_range) and the compaction
-after cleaning (compact_side). A batch's labels arrive in the tree kernel's
+the merge of adjacent segments (store.cpp's merge_range), the GLL cleaning of
+a batch's candidates (clean_side) and the compaction after it (compact_side). A batch's labels arrive in the tree kernel's
 emission order (any order); every segment must hold each row's labels
 hub-ascending, with the row offsets of the counts, in the 5-byte (hub offset,
 distance) and the 8-byte entry formats.
@@ -52,7 +52,8 @@ def batch(rng, n, lengths, h0, nhub, maxd):
 
 def run(exe, tmp_path, n, batches, mode=0, out_fmt=SEG_F8, seed=3):
     """batches: (fmt, h0, row, hub, dist) each; mode 0 returns the first batch's
-    segment, 1 the merge of all, 2 the first compacted by keep_flags."""
+    segment, 1 the merge of all, 2 the first compacted by keep_flags, 3 the
+    first segment with its keep flags after cleaning against the second."""
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(inp, "wb") as f:
         f.write(np.array([n, len(batches), mode, out_fmt], np.uint32).tobytes())
@@ -67,7 +68,10 @@ def run(exe, tmp_path, n, batches, mode=0, out_fmt=SEG_F8, seed=3):
     assert r.returncode == 0, (r.returncode, r.stderr[-2000:])
     data = np.fromfile(out, np.uint8)
     off = data[:8 * (n + 1)].view(np.uint64)
-    ent = data[8 * (n + 1):].view(np.uint32).reshape(-1, 2)
+    m = int(off[-1])
+    ent = data[8 * (n + 1):8 * (n + 1) + 8 * m].view(np.uint32).reshape(-1, 2)
+    if mode == 3:
+        return off, ent[:, 0], ent[:, 1], data[8 * (n + 1) + 8 * m:]
     return off, ent[:, 0], ent[:, 1]
 
 
@@ -144,3 +148,45 @@ def test_segment_compaction_keeps_order(emulator, tmp_path, fmt):
     assert 0 < k.sum() < k.size
     for a, b in zip(got, segment_of(n, row[k], hub[k], dist[k])):
         assert np.array_equal(a, b)
+
+
+def clean_cover(off, hub, dist, ooff, ohub, odist):
+    """shared_memory/labeling.hpp:78-88 by brute force, one candidate at a time:
+    candidate (h, d) of row v is covered when some hub x < h is in both L(v) and
+    the other side's L(h) with L(v)[x] + L'(h)[x] <= d. Rows are taken as sets, so
+    neither the entry order nor a merge is assumed."""
+    keep = np.ones(hub.size, bool)
+    for v in range(off.size - 1):
+        b, e = int(off[v]), int(off[v + 1])
+        mine = dict(zip(hub[b:e].tolist(), dist[b:e].tolist()))
+        for c in range(b, e):
+            h, d = int(hub[c]), int(dist[c])
+            ob, oe = int(ooff[h]), int(ooff[h + 1])
+            other = dict(zip(ohub[ob:oe].tolist(), odist[ob:oe].tolist()))
+            keep[c] = not any(x < h and x in other and dx + other[x] <= d for x, dx in mine.items())
+    return keep
+
+
+@pytest.mark.parametrize("fmt", [SEG_F5, SEG_F8])
+def test_cleaning_matches_clean_cover(emulator, tmp_path, fmt):
+    """The batch's candidates of one side are cleaned against the other side's
+    batch rows (clean_side): the keep flags equal clean_cover's, with rows staged
+    in LDS (up to 1,024 entries) and read from the segment (longer)."""
+    rng = np.random.default_rng(21 + fmt)
+    n, h0, nhub = 1600, 0, 1300  # hubs are ranks in [0, 1300), rows are ranks in [0, 1600)
+    lengths = rng.integers(0, 25, n)
+    lengths[rng.integers(0, n, 2)] = [1100, 1200]  # past the LDS stage
+    lengths[rng.integers(0, nhub, 40)] = rng.integers(60, 200, 40)  # long other rows of some hubs
+    row, hub, dist = batch(rng, n, lengths, h0, nhub, 200)
+    lengths2 = rng.integers(0, 25, n)
+    lengths2[rng.integers(0, nhub, 40)] = rng.integers(60, 200, 40)
+    row2, hub2, dist2 = batch(rng, n, lengths2, h0, nhub, 200)
+    off, got_hub, got_dist, keep = run(emulator, tmp_path, n, [(fmt, h0, row, hub, dist), (fmt, h0, row2, hub2, dist2)],
+                                       mode=3)
+    want = segment_of(n, row, hub, dist)
+    for a, b in zip((off, got_hub, got_dist), want):
+        assert np.array_equal(a, b)
+    o2 = segment_of(n, row2, hub2, dist2)
+    want_keep = clean_cover(*want, *o2)
+    assert 0 < (~want_keep).sum() < want_keep.size  # some covered, some kept
+    assert np.array_equal(keep.astype(bool), want_keep)
