@@ -125,7 +125,8 @@ typedef struct chl_build_opts {
   int32_t slots;        /* concurrent root slots = workgroups of the tree kernel (0 = default) */
   int granularity;      /* per-root parallelism: 0 = default (workgroup), 1 = wavefront, 2 = workgroup of
                            256 threads, 3 = workgroup of 1024 threads (a quarter of the roots in flight) */
-  double switch_ratio;  /* ps of distributed_memory/hhl.cpp:23 (default 1024) */
+  double switch_ratio;  /* ps of distributed_memory/hhl.cpp:23 (default 1024); rounded to float, as
+                           hl::run's parameter is (distributed_memory/akiba.hpp:190) */
   int plant_prune;      /* 1: PLaNT trees also prune by the completed rank-prefix labels */
   int device;           /* HIP device ordinal */
   int verbose;          /* 1: per-batch lines on stderr */

@@ -690,14 +690,17 @@ struct HybridStats { double total_s; int64_t switch_root; int64_t supersteps; };
 // an identity, the label exchange degenerates into moving in-superstep labels into
 // the common (rank prefix within budget) or global labeling, and the pruned-phase
 // filter is computeLocalMask (utils.hpp:249-276) with its AND-reduction a no-op.
+// cb and ps are floats, as run's parameters are (akiba.hpp:190): the budget is a
+// float product (akiba.hpp:204) and the ratio test compares with ps widened to
+// double (akiba.hpp:206, 301).
 static void run_hybrid(const Graph& g, const std::vector<Vertex>& order, const std::vector<Vertex>& rev,
-                       Labeling& out, int P, double cb, double ps, HybridStats* hs) {
+                       Labeling& out, int P, float cb, float ps, HybridStats* hs) {
   const int64_t N = (int64_t)order.size();
   Labeling loc(N), common(N), glob(N);
   Locks lck(2 * N + 2);
   std::vector<Search*> ak;
   for (int t = 0; t < P; ++t) ak.push_back(new Search(g));
-  uint64_t common_budget = std::min<uint64_t>((uint64_t)(unsigned)(N * 2 * 16), (uint64_t)(unsigned)(1000000.0 * cb));
+  uint64_t common_budget = std::min<uint64_t>((uint64_t)(unsigned)(N * 2 * 16), (uint64_t)(unsigned)(1000000 * cb));
   bool do_plant = true, switch_compute = true, comm_done = false;
   int64_t sync_thres = P, prev_sync = 0;
   std::atomic<int64_t> cnt(0);
@@ -722,7 +725,7 @@ static void run_hybrid(const Graph& g, const std::vector<Vertex>& order, const s
           unsigned ls = s1.first + s2.first, ts = s1.second + s2.second;
           per_hub[lc - prev_sync] += ls;
           double ratio = (double)ts / (double)ls;  // akiba.hpp:297
-          if (switch_compute && ratio > ps) phase_flag.store(1);
+          if (switch_compute && ratio > (double)ps) phase_flag.store(1);
         } else {
           per_hub[lc - prev_sync] += iteration_buffer(*ak[th], lc, true, order, rev, glob, loc, common, lck);
           per_hub[lc - prev_sync] += iteration_buffer(*ak[th], lc, false, order, rev, glob, loc, common, lck);
@@ -960,7 +963,7 @@ void* orc_hybrid(void* gp, const uint32_t* order, int64_t n, int P, double cb, d
   if (!make_order(*g, order, n, ord, rev)) return nullptr;
   Labeling L(n);
   HybridStats hs;
-  run_hybrid(*g, ord, rev, L, P, cb, ps, &hs);
+  run_hybrid(*g, ord, rev, L, P, (float)cb, (float)ps, &hs);
   if (stats3) { stats3[0] = hs.total_s; stats3[1] = (double)hs.switch_root; stats3[2] = (double)hs.supersteps; }
   return to_flat(L);
 }

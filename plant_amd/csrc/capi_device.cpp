@@ -146,6 +146,9 @@ int chl_build(const chl_graph* g, const uint32_t* order, int64_t n, const chl_bu
   else chl_build_opts_default(&o);
   if (o.mode < CHL_MODE_GLL || o.mode > CHL_MODE_HYBRID) return set_error(CHL_E_INVAL, "bad mode %d", o.mode);
   if (o.switch_ratio <= 0) o.switch_ratio = 1024.0;
+  // hl::run takes the ratio as a float and widens it to double for the test
+  // (distributed_memory/akiba.hpp:190, 206): a ratio of 0.1 is 0.100000001490116.
+  o.switch_ratio = (double)(float)o.switch_ratio;
   int rc = ensure_device(o.device);
   if (rc) return rc;
   if (stats) memset(stats, 0, sizeof(*stats));

@@ -81,6 +81,25 @@ def test_hybrid_switches(chl, grid):
     assert_same_labels(L, ref)
 
 
+def test_hybrid_switch_ratio_is_a_float(chl, orc):
+    # hl::run takes ps as a float (distributed_memory/akiba.hpp:190, widened at
+    # :206), so 1 - 1e-12 is 1.0 there. On a 12-cycle the top root's tree/label
+    # ratio is exactly 1 and the next root's is above 1: with one root per batch
+    # (one thread, one root per superstep in the oracle) ps = 0.5 switches after
+    # the first root, 1.0 and 1 - 1e-12 after the second.
+    n = 12
+    src = np.arange(n, dtype=np.uint32)
+    dst = (src + 1) % n
+    g = chl.Graph.from_arcs(n, np.r_[src, dst], np.r_[dst, src], np.ones(2 * n, np.int32))
+    order = np.random.default_rng(3).permutation(n).astype(np.uint32)
+    og, ref = oracle_of(chl, orc, g, order)
+    for ps in (0.5, 1.0, 1 - 1e-12):
+        L = chl.build(g, order, mode=chl.MODE_HYBRID, switch_ratio=ps, batch_roots=1)
+        assert_same_labels(L, ref)
+        want = int(orc.hybrid(og, order, threads=1, ps=ps)[1][1])
+        assert int(L.stats.switch_rank) == want == (1 if ps == 0.5 else 2), ps
+
+
 @pytest.mark.parametrize("kind", ["rmat", "knn"])
 def test_learned_hybrid_switch(chl, orc, kind):
     # the learned policy places the switch from predicted tree/label ratios and a

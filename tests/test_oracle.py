@@ -96,6 +96,30 @@ def test_gll_plant_hybrid_equal_brute_force(chl, orc, k):
     assert same(labels_of(hyb), brute)
 
 
+def cycle_graph(orc, n):
+    src = np.arange(n, dtype=np.uint32)
+    dst = (src + 1) % n
+    return orc.OGraph.from_arcs(n, np.r_[src, dst], np.r_[dst, src], np.ones(2 * n, np.int32))
+
+
+def test_hybrid_switch_ratio_is_a_float(orc):
+    """hl::run takes ps as a float (distributed_memory/akiba.hpp:190) and widens it
+    for the ratio test (akiba.hpp:206, 301), so ps = 1 - 1e-12 is 1.0 there. On a
+    cycle the top root's PLaNT tree labels every vertex it settles: its tree/label
+    ratio is exactly 1, which passes ps = 0.5 but neither 1.0 nor 1 - 1e-12. With
+    one thread a superstep is one root, so the switch root tells which root passed."""
+    n = 12
+    g = cycle_graph(orc, n)
+    order = np.random.default_rng(3).permutation(n).astype(np.uint32)
+    _, ts, lc = orc.plant_range(g, order, 0, n, threads=1)
+    ratio = (ts[0::2].astype(np.float64) + ts[1::2]) / (lc[0::2].astype(np.float64) + lc[1::2])
+    assert ratio[0] == 1.0 and ratio[1] > 1.0
+    switch = {ps: int(orc.hybrid(g, order, threads=1, ps=ps)[1][1]) for ps in (0.5, 1.0, 1 - 1e-12)}
+    assert switch[0.5] == 1  # the top root passes
+    assert switch[1.0] == 2  # the next root passes
+    assert switch[1 - 1e-12] == switch[1.0]
+
+
 def test_path_midpoint_order_closed_form(chl, orc):
     # path 0-1-...-14 with unit weights; order = midpoint recursion. The hubs of v
     # are exactly v's ancestors in the recursion tree (and v itself).
